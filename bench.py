@@ -1,0 +1,194 @@
+#!/usr/bin/env python3
+"""Benchmark: cell-steps/s of the full hot-path step on the 100k-cell, 161^3 spheroid.
+
+A step is one ``cell_step`` (simulation/simulator.py:58-81): deposit -> red-black SOR to a
+max-norm residual of 1e-6 (omega 1.8, checked every 10 sweeps) -> sampling -> lifecycle
+decisions and slot allocation -> 4 contact substeps (hash-grid build, gather, integrate).
+Inputs are synthetic (``warpbiocell_amd.workloads.spheroid_100k_161``) and resident on the GPU
+before the timed region. One JSON line goes to stdout; progress goes to stderr.
+
+Roofline: the SOR sweep kernel, timed live with HIP events on the stream it runs on, priced at
+its algorithmic bytes (16 B per node per red+black sweep: the field read by each colour pass,
+the updated colour written, the density read) against MI355X's 8 TB/s HBM peak.
+
+CPU baseline: the oracle (C restatement of the reference, one thread) runs ONE step of the same
+workload from the exact state the GPU's first timed step started from.
+"""
+
+from __future__ import annotations
+
+import argparse
+import json
+import os
+import sys
+import time
+from pathlib import Path
+
+ROOT = Path(__file__).resolve().parent
+sys.path.insert(0, str(ROOT))
+
+HBM_PEAK_GBPS = 8000.0
+
+
+def log(msg: str) -> None:
+    print(msg, file=sys.stderr, flush=True)
+
+
+def setup(workload_name: str, device):
+    from warpbiocell_amd import workloads as W
+    from warpbiocell_amd.cells.lifecycle import LifecycleParams
+    from warpbiocell_amd.cells.mechanics import ContactParams, make_neighbor_grid
+    from warpbiocell_amd.cells.state import CellPopulation
+    from warpbiocell_amd.fields.diffusion import SolverSettings
+    from warpbiocell_amd.fields.oxygen import OxygenField, OxygenParams
+    from warpbiocell_amd.fields.scalar_field import GridGeometry
+    from warpbiocell_amd.simulation.simulator import TimeStepping
+
+    w = getattr(W, workload_name)()
+    geom = GridGeometry.centered_cube(w.box_um, w.dx_um)
+    params = OxygenParams(W.D_UM2_PER_H, W.UPTAKE_MMHG_UM3_PER_H, W.MICHAELIS_K, w.boundary_mmHg)
+    settings = SolverSettings(omega=w.omega, tolerance=w.tolerance, max_sweeps=w.max_sweeps, check_every=w.check_every)
+    lifecycle = LifecycleParams(division_rate=w.division_rate, death_rate=0.0, inhibition_threshold=10 ** 6)
+    contact = ContactParams(stiffness=W.CONTACT_RATE, damping=1.0, max_radius=float(w.radii.max()))
+    stepping = TimeStepping(dt_cells=W.DT_CELLS, dt_mechanics=W.DT_MECHANICS)
+    pop = CellPopulation.from_numpy(w.positions, w.radii, capacity=w.capacity, device=device, seed=w.seed)
+    ox = OxygenField(geom, params, settings, device=device)
+    grid = make_neighbor_grid(contact, device)
+    return w, geom, params, settings, lifecycle, contact, stepping, pop, ox, grid
+
+
+def sor_roofline(ox, geom, device, launches: int = 200) -> dict:
+    """Average duration of one SOR colour-pass launch, measured with HIP events on its stream."""
+    import torch
+
+    from warpbiocell_amd.fields import diffusion as Dn
+
+    kin = ox.species.kinetics
+    stream = torch.cuda.current_stream(device)
+    for c in range(4):
+        Dn.sor_sweep(ox.field, ox.density, kin, ox.settings.omega, c & 1)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    for c in range(launches):
+        Dn.sor_sweep(ox.field, ox.density, kin, ox.settings.omega, c & 1)
+    end.record(stream)
+    end.synchronize()
+    per_launch_s = start.elapsed_time(end) / 1e3 / launches
+    V = geom.node_count
+    bytes_per_launch = 8 * V  # half of 16 B/node per red+black sweep
+    achieved = bytes_per_launch / per_launch_s / 1e9
+    return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
+            "kernel": "sor_color_kernel", "launch_us": round(per_launch_s * 1e6, 3),
+            "algorithmic_bytes_per_launch": bytes_per_launch}
+
+
+def snapshot(pop, ox):
+    return {name: getattr(pop, name).cpu().numpy().copy() for name in
+            ("position", "radius", "cell_state", "cell_type", "age", "oxygen_local", "glucose_local", "velocity",
+             "neighbor_count", "divide_flag", "division_offset", "fate_flags")} | {
+        "rng_state": pop.rng_numpy(), "count": pop.count, "field": ox.numpy()}
+
+
+def cpu_baseline(snap, w, geom, params, settings, lifecycle, contact, stepping) -> dict:
+    """One oracle step (single thread) from the snapshot the GPU's first timed step started at."""
+    from oracle import oracle as O
+
+    pop = O.OraclePopulation.from_numpy(snap["position"][: snap["count"]], snap["radius"][: snap["count"]],
+                                        capacity=len(snap["radius"]), seed=w.seed)
+    for name in ("position", "radius", "cell_state", "cell_type", "age", "oxygen_local", "glucose_local",
+                 "velocity", "neighbor_count", "divide_flag", "division_offset", "fate_flags"):
+        getattr(pop, name)[...] = snap[name]
+    pop.rng_state[...] = snap["rng_state"]
+    a = O.SolveArgs(dx=geom.dx, diffusion=params.diffusion_coefficient, uptake_max=params.uptake_max,
+                    michaelis_k=params.michaelis_k, boundary_value=params.boundary_value)
+    rox = O.OracleOxygen(shape=geom.shape, origin=geom.origin, args=a, omega=settings.omega,
+                         max_sweeps=settings.max_sweeps, tolerance=settings.tolerance,
+                         check_every=settings.check_every, values=snap["field"].copy())
+    n = pop.count
+    t0 = time.perf_counter()
+    _, field = O.oracle_cell_step(pop, lifecycle, contact, stepping, oxygen=rox)
+    dt = time.perf_counter() - t0
+    return {"value": round(n / dt, 1), "unit": "cell-steps/s", "cores": 1, "kind": "port",
+            "sample": f"1 step of {w.name} ({n} cells, {field[0]} SOR sweeps) from the state the GPU's first "
+                      f"timed step started at; oracle C restatement, -O2, one thread; {dt:.2f} s",
+            "sweeps": field[0], "seconds": round(dt, 3)}
+
+
+def main() -> int:
+    ap = argparse.ArgumentParser()
+    ap.add_argument("--gpus", type=int, default=1)
+    ap.add_argument("--steps", type=int, default=100)
+    ap.add_argument("--warmup", type=int, default=5)
+    ap.add_argument("--workload", default="spheroid_100k_161")
+    ap.add_argument("--no-cpu-baseline", action="store_true")
+    ap.add_argument("--no-roofline", action="store_true")
+    args = ap.parse_args()
+    if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
+        print(json.dumps({"error": "this build runs the hot path on one GPU; --gpus above 1 is not supported",
+                          "n_gpus": args.gpus}))
+        return 2
+
+    import torch
+
+    from warpbiocell_amd import _lib
+    from warpbiocell_amd.simulation.simulator import cell_step
+
+    _lib.load()
+    device = torch.device("cuda", 0)
+    w, geom, params, settings, lifecycle, contact, stepping, pop, ox, grid = setup(args.workload, device)
+    log(f"bench: {w.name}: {pop.count} cells, grid {geom.shape}, warmup {args.warmup}, steps {args.steps}")
+    for s in range(args.warmup):
+        r = cell_step(pop, grid, lifecycle, contact, stepping, oxygen=ox)
+        log(f"  warmup {s}: sweeps {r.field_sweeps} residual {r.field_residual:.3e}")
+    torch.cuda.synchronize()
+    snap = None if args.no_cpu_baseline else snapshot(pop, ox)
+
+    reports, cell_steps = [], 0
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    for _ in range(args.steps):
+        cell_steps += pop.count
+        reports.append(cell_step(pop, grid, lifecycle, contact, stepping, oxygen=ox))
+    torch.cuda.synchronize()
+    elapsed = time.perf_counter() - t0
+    sweeps = [r.field_sweeps for r in reports]
+    log(f"  timed: {elapsed:.3f} s, sweeps/step mean {sum(sweeps) / len(sweeps):.1f} "
+        f"(min {min(sweeps)}, max {max(sweeps)}), unconverged steps {sum(not r.field_converged for r in reports)}")
+
+    roof = None if args.no_roofline else sor_roofline(ox, geom, device)
+    if roof:
+        log(f"  sor colour pass: {roof['launch_us']} us, {roof['achieved']} GB/s algorithmic")
+    cpu = None
+    if snap is not None:
+        cpu = cpu_baseline(snap, w, geom, params, settings, lifecycle, contact, stepping)
+        log(f"  cpu baseline: {cpu['value']} cell-steps/s ({cpu['seconds']} s, {cpu['sweeps']} sweeps)")
+
+    result = {
+        "metric": "cell-steps/s @100k cells,161^3 SOR",
+        "value": round(cell_steps / elapsed, 1),
+        "unit": "cell-steps/s",
+        "n_gpus": 1,
+        "steps": args.steps,
+        "warmup": args.warmup,
+        "ms_per_step": round(elapsed / args.steps * 1e3, 3),
+        "higher_is_better": True,
+        "scaling": "weak",
+        "vs_baseline": None,
+        "dtype": "f32",
+        "data": "synthetic",
+        "config": {"workload": w.name, "cells": w.n, "grid": list(geom.shape), "omega": settings.omega,
+                   "tolerance": settings.tolerance, "check_every": settings.check_every,
+                   "mechanics_substeps": stepping.mechanics_substeps,
+                   "sweeps_per_step_mean": round(sum(sweeps) / len(sweeps), 1),
+                   "unconverged_steps": sum(not r.field_converged for r in reports),
+                   "final_residual": reports[-1].field_residual},
+        "roofline": roof,
+        "cpu_baseline": cpu,
+    }
+    print(json.dumps(result), flush=True)
+    return 0
+
+
+if __name__ == "__main__":
+    sys.exit(main())

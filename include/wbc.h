@@ -1,0 +1,148 @@
+/*
+ * wbc.h — C ABI of the MI355X (gfx950) WarpBioCell hot-path library, libwbc_hip.so.
+ *
+ * Conventions
+ *   - Every array argument is a DEVICE pointer owned by the caller (a torch tensor's
+ *     data_ptr()), except where a parameter is documented as pinned host memory.
+ *   - Grid arrays are C order (i, j, k) with k contiguous, shape (nx, ny, nz); positions and
+ *     velocities are (n, 3) float32 rows; the deposit accumulator is (4, nx, ny, nz) int64.
+ *   - Every function except the *_bytes queries returns 0 or a hipError_t code; none throws
+ *     or exits. Work is enqueued on `stream` (hipStream_t, pass torch's current stream).
+ *   - Hot calls allocate nothing: scratch is a caller-owned device workspace sized by the
+ *     matching *_workspace_bytes() query.
+ *   - Scalars that the reference passes as wp.float32 are float here; the host computes them
+ *     in double and rounds once, as Warp's launch does.
+ *
+ * Reference paths are relative to the WarpBioCell source root (src/warpbiocell/).
+ */
+#ifndef WBC_H
+#define WBC_H
+
+#include <stddef.h>
+#include <stdint.h>
+
+#ifdef __cplusplus
+extern "C" {
+#endif
+
+typedef struct ihipStream_t* wbc_stream_t; /* == hipStream_t */
+
+/* ---- per-cell RNG streams ------------------------------------------------------------------
+ * Replaces _init_rng, cells/state.py:25-28, launched over the whole capacity (state.py:110):
+ * rng_state[i] = rand_init(seed, i) for i in [0, capacity). */
+int wbc_init_rng(uint32_t* rng_state, int64_t capacity, int32_t seed, wbc_stream_t stream);
+
+/* ---- prefix sums ---------------------------------------------------------------------------
+ * Replaces warp.utils.array_scan(divide_flag[:n], division_offset[:n], inclusive=True),
+ * cells/lifecycle.py:146. Workspace: wbc_scan_workspace_bytes(n) bytes of device memory. */
+size_t wbc_scan_workspace_bytes(int64_t n);
+int wbc_scan_inclusive_i32(const int32_t* in, int32_t* out, int64_t n, void* workspace, size_t workspace_bytes,
+                           wbc_stream_t stream);
+/* Exclusive variant (bucket offsets of the hash-grid counting sort). */
+int wbc_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, void* workspace, size_t workspace_bytes,
+                           wbc_stream_t stream);
+
+/* ---- lifecycle -----------------------------------------------------------------------------
+ * Replaces the lifecycle_decide launch of decide_lifecycle, cells/lifecycle.py:96-132
+ * (kernel kernels/cell_kernels.py:77-154); argument order and meaning as in the kernel. */
+int wbc_lifecycle_decide(float dt, float death_rate, float anoxic_death_rate, float division_rate,
+                         int32_t inhibition_threshold, float hypoxia_threshold, float death_threshold,
+                         float glucose_threshold, float glucose_death_threshold, int32_t necrosis_requires_glucose,
+                         int32_t phenotype_model, float apoptosis_rate, float necrosis_rate, int32_t* cell_state,
+                         float* age, const int32_t* neighbor_count, const float* oxygen_local,
+                         const float* glucose_local, const int32_t* fate_flags, uint32_t* rng_state,
+                         int32_t* divide_flag, int64_t n, wbc_stream_t stream);
+
+/* Replaces the place_daughters launch of commit_divisions, cells/lifecycle.py:155-178
+ * (kernel kernels/cell_kernels.py:157-200). The caller has checked count + daughters <= capacity
+ * (CapacityError, lifecycle.py:150-154) before calling. */
+int wbc_place_daughters(int32_t count, float placement_factor, const int32_t* divide_flag,
+                        const int32_t* division_offset, float* position, float* radius, int32_t* cell_state,
+                        int32_t* cell_type, float* age, float* oxygen_local, float* glucose_local,
+                        uint32_t* rng_state, float* velocity, int32_t* neighbor_count, int32_t* fate_flags,
+                        int64_t n, wbc_stream_t stream);
+
+/* ---- cell -> grid deposit ------------------------------------------------------------------
+ * Replaces CellDensities.deposit, fields/densities.py:26-43: accumulator.zero_() (use
+ * wbc_fill_zero_i64), deposit_trilinear (kernels/field_kernels.py:78-116) and
+ * fixed_point_to_density (:119-126) fused with combine_densities (:129-142) for the species
+ * weights. per_state (4, nx, ny, nz) and combined (nx, ny, nz) may each be NULL (not both). */
+int wbc_fill_zero_i64(int64_t* p, int64_t count, wbc_stream_t stream);
+int wbc_deposit_trilinear_i64(const float* position, const int32_t* cell_state, int64_t n, float ox, float oy,
+                              float oz, float inv_dx, int32_t nx, int32_t ny, int32_t nz, int64_t* accumulator,
+                              wbc_stream_t stream);
+int wbc_density_finalize(const int64_t* accumulator, int64_t nodes, float inv_voxel_volume, float w0, float w1,
+                         float w2, float w3, float* combined, float* per_state, wbc_stream_t stream);
+/* Replaces CellDensities.combine, fields/densities.py:45-50 (combine_densities). */
+int wbc_combine_densities(const float* per_state, int64_t nodes, float w0, float w1, float w2, float w3, float* out,
+                          wbc_stream_t stream);
+
+/* ---- grid -> cell sampling -----------------------------------------------------------------
+ * Replaces SpeciesField.sample_at_cells, fields/species.py:157-168 (kernel
+ * kernels/field_kernels.py:153-182). */
+int wbc_sample_trilinear(const float* field, int32_t nx, int32_t ny, int32_t nz, const float* position, int64_t n,
+                         float ox, float oy, float oz, float inv_dx, float* out, wbc_stream_t stream);
+
+/* ---- red-black SOR -------------------------------------------------------------------------
+ * One colour pass: replaces one sor_sweep launch, fields/diffusion.py:105-111 (kernel
+ * kernels/field_kernels.py:185-212). `fixed` may be NULL (no pinned nodes); `production` may be
+ * NULL (the species has none: diffusion.py:73 passes zeros). bc_*: 0 Dirichlet, 1 Neumann. */
+int wbc_sor_sweep(int32_t color, float omega, float dx2_over_D, float uptake_max, float michaelis_k, int32_t bc_x,
+                  int32_t bc_y, int32_t bc_z, const int32_t* fixed, const float* density, const float* production,
+                  int32_t has_source, const float* source, float source_k, float* field, int32_t nx, int32_t ny,
+                  int32_t nz, wbc_stream_t stream);
+
+/* Replaces the steady_state_residual launch, fields/diffusion.py:114-119 (kernel
+ * kernels/field_kernels.py:215-243): *residual = max(*residual, max over updated nodes). The
+ * caller zeroes *residual first, as diffusion.py:113 does with residual_buffer.zero_(). */
+int wbc_steady_state_residual(float dx2_over_D, float uptake_max, float michaelis_k, float reference, int32_t bc_x,
+                              int32_t bc_y, int32_t bc_z, const int32_t* fixed, const float* density,
+                              const float* production, int32_t has_source, const float* source, float source_k,
+                              const float* field, int32_t nx, int32_t ny, int32_t nz, float* residual,
+                              wbc_stream_t stream);
+
+/* Replaces solve_steady_state, fields/diffusion.py:79-123, whole loop: chunks of check_every
+ * sweeps, a residual after each, stop below `tolerance` or at max_sweeps. host_residual is
+ * PINNED host memory of at least ceil(max_sweeps / check_every) floats. Outputs mirror
+ * SolveReport (diffusion.py:64-68). Synchronizes with the stream before returning. */
+size_t wbc_solve_workspace_bytes(int32_t max_sweeps, int32_t check_every);
+int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolerance, int32_t check_every, float dx2_over_D,
+                           float uptake_max, float michaelis_k, float reference, int32_t bc_x, int32_t bc_y,
+                           int32_t bc_z, const int32_t* fixed, const float* density, const float* production,
+                           int32_t has_source, const float* source, float source_k, float* field, int32_t nx,
+                           int32_t ny, int32_t nz, void* workspace, size_t workspace_bytes, float* host_residual,
+                           int32_t* sweeps_out, float* residual_out, int32_t* converged_out, wbc_stream_t stream);
+
+/* ---- hash grid -----------------------------------------------------------------------------
+ * Replaces wp.HashGrid(dim, dim, dim) (spatial/neighbors.py:23) and NeighborGrid.build
+ * (:33-37) over position[:n]. The workspace holds the bucket table and the bucket-ordered
+ * copy of the points; wbc_hashgrid_init zeroes it once after allocation. */
+size_t wbc_hashgrid_workspace_bytes(int32_t dim, int64_t capacity);
+int wbc_hashgrid_init(void* workspace, size_t workspace_bytes, int32_t dim, int64_t capacity, wbc_stream_t stream);
+int wbc_hashgrid_build(const float* position, const float* radius, int32_t n, float cell_size, int32_t dim,
+                       int64_t capacity, void* workspace, size_t workspace_bytes, wbc_stream_t stream);
+/* Device pointers (inside the workspace) to the slot ids in bucket order (the order
+ * hash_grid_point_id(grid, tid) walks) and the dim^3 + 1 bucket starts. */
+int wbc_hashgrid_point_ids(const void* workspace, size_t workspace_bytes, int32_t dim, int64_t capacity,
+                           const int32_t** point_id, const int32_t** bucket_start);
+
+/* ---- contact mechanics ---------------------------------------------------------------------
+ * Replaces the contact_velocities launch, cells/mechanics.py:149-161 (kernel
+ * kernels/mechanics_kernels.py:26-57), over the last build. */
+int wbc_contact_velocities(const void* workspace, size_t workspace_bytes, int32_t dim, int64_t capacity, int32_t n,
+                           float cell_size, float query_radius, float rate, float* velocity,
+                           int32_t* neighbor_count, wbc_stream_t stream);
+/* Replaces integrate_positions, cells/mechanics.py:171-176 (kernels/mechanics_kernels.py:60-67). */
+int wbc_integrate_positions(float* position, const float* velocity, int64_t n, float dt, wbc_stream_t stream);
+/* Replaces relax_contacts, cells/mechanics.py:179-189 without a region: `substeps` x (build,
+ * contact gather fused with x += v dt). */
+int wbc_relax_contacts(float* position, const float* radius, int32_t n, float cell_size, int32_t dim,
+                       float query_radius, float rate, float dt, int32_t substeps, float* velocity,
+                       int32_t* neighbor_count, int64_t capacity, void* workspace, size_t workspace_bytes,
+                       wbc_stream_t stream);
+
+#ifdef __cplusplus
+}
+#endif
+
+#endif /* WBC_H */

@@ -1,0 +1,391 @@
+"""Python face of the CPU oracle (``libwbc_oracle.so``). TEST INFRASTRUCTURE ONLY.
+
+Loaded by the test suite, the smoke check and the CPU-baseline leg of the benchmark, never by
+the product. Each function calls the C restatement in ``wbc_oracle.c`` with the same scalar
+conversions the product's host code makes (double -> float32 once, at the call), so the two
+see identical inputs. ``OraclePopulation`` and ``oracle_cell_step`` compose them into the
+reference's step (simulation/simulator.py:58-81) on numpy arrays.
+"""
+
+from __future__ import annotations
+
+import ctypes
+import subprocess
+from ctypes import POINTER, c_float, c_int32, c_int64, c_uint32, c_void_p
+from dataclasses import dataclass, field
+from pathlib import Path
+
+import numpy as np
+
+HERE = Path(__file__).resolve().parent
+LIBRARY = HERE / "libwbc_oracle.so"
+
+_lib = None
+
+
+def build() -> Path:
+    """Compile the oracle with its Makefile (always; cheap)."""
+    subprocess.run(["make", "-s", "-C", str(HERE), "-B", "libwbc_oracle.so"], check=True)
+    return LIBRARY
+
+
+def lib() -> ctypes.CDLL:
+    global _lib
+    if _lib is None:
+        if not LIBRARY.exists():
+            build()
+        _lib = ctypes.CDLL(str(LIBRARY))
+        L = _lib
+        f32, i32, i64, p = c_float, c_int32, c_int64, c_void_p
+        sig = {
+            "orc_pcg": (c_uint32, [c_uint32]),
+            "orc_rand_init": (c_uint32, [i32, i32]),
+            "orc_randf_step": (f32, [p]),
+            "orc_sample_sphere": (None, [p, p]),
+            "orc_expf": (f32, [f32]), "orc_sinf": (f32, [f32]), "orc_cosf": (f32, [f32]), "orc_acosf": (f32, [f32]),
+            "orc_init_rng": (None, [p, i64, i32]),
+            "orc_deposit_trilinear": (None, [p, p, i64, f32, f32, f32, f32, i32, i32, i32, p]),
+            "orc_fixed_point_to_density": (None, [p, i64, f32, p]),
+            "orc_combine_densities": (None, [f32, f32, f32, f32, p, i64, p]),
+            "orc_sample_trilinear": (None, [p, i32, i32, i32, p, i64, f32, f32, f32, f32, p]),
+            "orc_sor_sweep_f32": (None, [i32, f32, f32, f32, f32, i32, i32, i32, p, p, p, i32, p, f32, p, i32, i32, i32]),
+            "orc_residual_f32": (f32, [f32, f32, f32, f32, i32, i32, i32, p, p, p, i32, p, f32, p, i32, i32, i32]),
+            "orc_solve_steady_state_f32": (i32, [f32, i32, f32, i32, f32, f32, f32, f32, i32, i32, i32, p, p, p, i32,
+                                                 p, f32, p, i32, i32, i32, p, p]),
+            "orc_sor_sweep_f64": (None, [i32] + [ctypes.c_double] * 4 + [i32, i32, i32, p, p, p, i32, p,
+                                                                         ctypes.c_double, p, i32, i32, i32]),
+            "orc_residual_f64": (ctypes.c_double, [ctypes.c_double] * 4 + [i32, i32, i32, p, p, p, i32, p,
+                                                                            ctypes.c_double, p, i32, i32, i32]),
+            "orc_solve_steady_state_f64": (i32, [ctypes.c_double, i32, ctypes.c_double, i32] + [ctypes.c_double] * 4 +
+                                           [i32, i32, i32, p, p, p, i32, p, ctypes.c_double, p, i32, i32, i32, p, p]),
+            "orc_bucket_key": (i32, [f32, f32, f32, f32, i32]),
+            "orc_hashgrid_build": (None, [p, i64, f32, i32, p, p]),
+            "orc_contact_velocities": (None, [p, p, i64, f32, i32, p, p, f32, f32, p, p]),
+            "orc_integrate_positions": (None, [p, p, i64, f32]),
+            "orc_contact_substep": (None, [p, p, i64, f32, i32, f32, f32, f32, p, p, p, p]),
+            "orc_scan_inclusive_i32": (None, [p, p, i64]),
+            "orc_lifecycle_decide": (None, [f32, f32, f32, f32, i32, f32, f32, f32, f32, i32, i32, f32, f32,
+                                            p, p, p, p, p, p, p, p, i64]),
+            "orc_place_daughters": (None, [i32, f32, p, p, p, p, p, p, p, p, p, p, p, p, p, i64]),
+            "orc_adjoint_sor_f64": (None, [i32] + [ctypes.c_double] * 4 + [p, p, p, p, i32, i32, i32]),
+        }
+        for name, (res, args) in sig.items():
+            fn = getattr(L, name)
+            fn.restype = res
+            fn.argtypes = args
+    return _lib
+
+
+def _p(a: np.ndarray | None):
+    if a is None:
+        return None
+    assert a.flags["C_CONTIGUOUS"], "oracle arrays must be C-contiguous"
+    return a.ctypes.data
+
+
+# ---- RNG and elementary functions ---------------------------------------------------------------
+
+def init_rng(capacity: int, seed: int) -> np.ndarray:
+    out = np.zeros(capacity, dtype=np.uint32)
+    lib().orc_init_rng(_p(out), capacity, seed)
+    return out
+
+
+def pcg(v: int) -> int:
+    return int(lib().orc_pcg(v & 0xFFFFFFFF))
+
+
+def randf_sequence(state: int, n: int) -> tuple[np.ndarray, int]:
+    s = np.array([state], dtype=np.uint32)
+    out = np.array([lib().orc_randf_step(_p(s)) for _ in range(n)], dtype=np.float32)
+    return out, int(s[0])
+
+
+def sample_sphere(state: int) -> tuple[np.ndarray, int]:
+    s = np.array([state], dtype=np.uint32)
+    out = np.zeros(3, dtype=np.float32)
+    lib().orc_sample_sphere(_p(s), _p(out))
+    return out, int(s[0])
+
+
+def expf(x: float) -> np.float32:
+    return np.float32(lib().orc_expf(x))
+
+
+def sinf(x: float) -> np.float32:
+    return np.float32(lib().orc_sinf(x))
+
+
+def cosf(x: float) -> np.float32:
+    return np.float32(lib().orc_cosf(x))
+
+
+def acosf(x: float) -> np.float32:
+    return np.float32(lib().orc_acosf(x))
+
+
+# ---- deposit / sample -------------------------------------------------------------------------------
+
+def deposit_accumulator(positions, states, origin, dx, shape) -> np.ndarray:
+    pos = np.ascontiguousarray(positions, dtype=np.float32)
+    st = np.ascontiguousarray(states, dtype=np.int32)
+    acc = np.zeros((4, *shape), dtype=np.int64)
+    lib().orc_deposit_trilinear(_p(pos), _p(st), len(pos), origin[0], origin[1], origin[2], 1.0 / dx, *shape, _p(acc))
+    return acc
+
+
+def fixed_point_to_density(acc: np.ndarray, dx: float) -> np.ndarray:
+    out = np.zeros(acc.shape, dtype=np.float32)
+    lib().orc_fixed_point_to_density(_p(acc), acc.size, 1.0 / dx ** 3, _p(out))
+    return out
+
+
+def combine(weights, density: np.ndarray) -> np.ndarray:
+    nodes = density[0].size
+    out = np.zeros(density.shape[1:], dtype=np.float32)
+    lib().orc_combine_densities(*[float(w) for w in weights], _p(np.ascontiguousarray(density)), nodes, _p(out))
+    return out
+
+
+def sample(field: np.ndarray, positions, origin, dx) -> np.ndarray:
+    f = np.ascontiguousarray(field, dtype=np.float32)
+    pos = np.ascontiguousarray(positions, dtype=np.float32)
+    out = np.zeros(len(pos), dtype=np.float32)
+    lib().orc_sample_trilinear(_p(f), *f.shape, _p(pos), len(pos), origin[0], origin[1], origin[2], 1.0 / dx, _p(out))
+    return out
+
+
+# ---- SOR ------------------------------------------------------------------------------------------
+
+@dataclass
+class SolveArgs:
+    """Scalars of one steady-state problem (fields/diffusion.py:94-99)."""
+
+    dx: float
+    diffusion: float
+    uptake_max: float
+    michaelis_k: float
+    boundary_value: float
+    boundary: tuple = (0, 0, 0)
+
+    @property
+    def dx2_over_D(self) -> float:
+        return self.dx ** 2 / self.diffusion
+
+    @property
+    def reference(self) -> float:
+        return self.boundary_value if self.boundary_value > 0.0 else 1.0
+
+
+def _dtype_suffix(dtype):
+    return ("f64", np.float64) if dtype == np.float64 else ("f32", np.float32)
+
+
+def sor_sweep(field: np.ndarray, density: np.ndarray, a: SolveArgs, omega: float, color: int, fixed=None,
+              production=None) -> None:
+    suf, dt = _dtype_suffix(field.dtype)
+    dens = np.ascontiguousarray(density, dtype=dt)
+    prod = None if production is None else np.ascontiguousarray(production, dtype=dt)
+    fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.int32)
+    getattr(lib(), f"orc_sor_sweep_{suf}")(color, omega, a.dx2_over_D, a.uptake_max, a.michaelis_k, *a.boundary,
+                                            _p(fx), _p(dens), _p(prod), 0, None, 1.0, _p(field), *field.shape)
+
+
+def residual(field: np.ndarray, density: np.ndarray, a: SolveArgs, fixed=None, production=None) -> float:
+    suf, dt = _dtype_suffix(field.dtype)
+    dens = np.ascontiguousarray(density, dtype=dt)
+    prod = None if production is None else np.ascontiguousarray(production, dtype=dt)
+    fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.int32)
+    return float(getattr(lib(), f"orc_residual_{suf}")(a.dx2_over_D, a.uptake_max, a.michaelis_k, a.reference,
+                                                      *a.boundary, _p(fx), _p(dens), _p(prod), 0, None, 1.0,
+                                                      _p(field), *field.shape))
+
+
+def solve(field: np.ndarray, density: np.ndarray, a: SolveArgs, omega: float, max_sweeps: int, tolerance: float,
+          check_every: int, fixed=None, production=None) -> tuple[int, float, bool]:
+    """fields/diffusion.py:79-123 in place on ``field`` (float32 or float64)."""
+    suf, dt = _dtype_suffix(field.dtype)
+    dens = np.ascontiguousarray(density, dtype=dt)
+    prod = None if production is None else np.ascontiguousarray(production, dtype=dt)
+    fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.int32)
+    res = np.zeros(1, dtype=dt)
+    conv = np.zeros(1, dtype=np.int32)
+    sweeps = getattr(lib(), f"orc_solve_steady_state_{suf}")(
+        omega, max_sweeps, tolerance, check_every, a.dx2_over_D, a.uptake_max, a.michaelis_k, a.reference, *a.boundary,
+        _p(fx), _p(dens), _p(prod), 0, None, 1.0, _p(field), *field.shape, _p(res), _p(conv))
+    return int(sweeps), float(res[0]), bool(conv[0])
+
+
+def apply_dirichlet(values: np.ndarray, boundary, value: float) -> None:
+    for axis, b in enumerate(boundary):
+        if b == 0:
+            idx = [slice(None)] * 3
+            idx[axis] = 0
+            values[tuple(idx)] = value
+            idx[axis] = -1
+            values[tuple(idx)] = value
+
+
+# ---- hash grid / mechanics ---------------------------------------------------------------------------
+
+def hashgrid_build(positions, cell_size: float, dim: int) -> tuple[np.ndarray, np.ndarray]:
+    pos = np.ascontiguousarray(positions, dtype=np.float32)
+    start = np.zeros(dim ** 3 + 1, dtype=np.int32)
+    ids = np.zeros(max(len(pos), 1), dtype=np.int32)
+    lib().orc_hashgrid_build(_p(pos), len(pos), cell_size, dim, _p(start), _p(ids))
+    return start, ids[: len(pos)]
+
+
+def contact_velocities(positions, radii, cell_size: float, dim: int, query_radius: float, rate: float):
+    pos = np.ascontiguousarray(positions, dtype=np.float32)
+    rad = np.ascontiguousarray(radii, dtype=np.float32)
+    start, ids = hashgrid_build(pos, cell_size, dim)
+    vel = np.zeros_like(pos)
+    cnt = np.zeros(len(pos), dtype=np.int32)
+    lib().orc_contact_velocities(_p(pos), _p(rad), len(pos), cell_size, dim, _p(start), _p(ids), query_radius, rate,
+                                 _p(vel), _p(cnt))
+    return vel, cnt
+
+
+def relax_contacts(positions: np.ndarray, radii: np.ndarray, cell_size: float, dim: int, query_radius: float,
+                   rate: float, dt: float, substeps: int, velocity: np.ndarray, counts: np.ndarray) -> None:
+    """In place on ``positions`` (n, 3) float32."""
+    n = len(positions)
+    start = np.zeros(dim ** 3 + 1, dtype=np.int32)
+    ids = np.zeros(max(n, 1), dtype=np.int32)
+    for _ in range(substeps):
+        lib().orc_contact_substep(_p(positions), _p(radii), n, cell_size, dim, query_radius, rate, dt, _p(velocity),
+                                  _p(counts), _p(start), _p(ids))
+
+
+def scan_inclusive(flags) -> np.ndarray:
+    f = np.ascontiguousarray(flags, dtype=np.int32)
+    out = np.zeros_like(f)
+    lib().orc_scan_inclusive_i32(_p(f), _p(out), len(f))
+    return out
+
+
+# ---- population and step ----------------------------------------------------------------------------
+
+@dataclass
+class OraclePopulation:
+    """numpy SoA with the layout of the reference's CellPopulation (cells/state.py:31-111)."""
+
+    capacity: int
+    count: int
+    seed: int
+    position: np.ndarray
+    radius: np.ndarray
+    cell_state: np.ndarray
+    cell_type: np.ndarray
+    age: np.ndarray
+    oxygen_local: np.ndarray
+    glucose_local: np.ndarray
+    rng_state: np.ndarray
+    velocity: np.ndarray
+    neighbor_count: np.ndarray
+    divide_flag: np.ndarray
+    division_offset: np.ndarray
+    fate_flags: np.ndarray
+    extra: dict = field(default_factory=dict)
+
+    @classmethod
+    def from_numpy(cls, positions, radii, capacity=None, seed=0, states=None) -> OraclePopulation:
+        positions = np.ascontiguousarray(positions, dtype=np.float32)
+        count = len(positions)
+        capacity = count if capacity is None else capacity
+        if states is None:
+            states = np.zeros(count, dtype=np.int32)
+
+        def padded(v, dtype, fill=0):
+            buf = np.full((capacity,) + np.asarray(v).shape[1:], fill, dtype=dtype)
+            buf[:count] = v
+            return buf
+
+        return cls(capacity=capacity, count=count, seed=seed, position=padded(positions, np.float32),
+                   radius=padded(np.asarray(radii, dtype=np.float32), np.float32),
+                   cell_state=padded(np.asarray(states, dtype=np.int32), np.int32, fill=3),
+                   cell_type=np.zeros(capacity, dtype=np.int32), age=np.zeros(capacity, dtype=np.float32),
+                   oxygen_local=np.zeros(capacity, dtype=np.float32), glucose_local=np.zeros(capacity, dtype=np.float32),
+                   rng_state=init_rng(capacity, seed), velocity=np.zeros((capacity, 3), dtype=np.float32),
+                   neighbor_count=np.zeros(capacity, dtype=np.int32), divide_flag=np.zeros(capacity, dtype=np.int32),
+                   division_offset=np.zeros(capacity, dtype=np.int32), fate_flags=np.zeros(capacity, dtype=np.int32))
+
+
+def lifecycle_decide(pop: OraclePopulation, params, dt: float) -> None:
+    n = pop.count
+    lib().orc_lifecycle_decide(dt, params.death_rate, params.anoxic_death_rate, params.division_rate,
+                               int(params.inhibition_threshold), params.hypoxia_threshold, params.death_threshold,
+                               params.glucose_threshold, params.glucose_death_threshold,
+                               1 if params.necrosis_requires_glucose else 0,
+                               1 if params.phenotype_model == "network" else 0, params.apoptosis_rate,
+                               params.necrosis_rate, _p(pop.cell_state), _p(pop.age), _p(pop.neighbor_count),
+                               _p(pop.oxygen_local), _p(pop.glucose_local), _p(pop.fate_flags), _p(pop.rng_state),
+                               _p(pop.divide_flag), n)
+
+
+def commit_divisions(pop: OraclePopulation, params) -> int:
+    n = pop.count
+    if n == 0:
+        return 0
+    lib().orc_scan_inclusive_i32(_p(pop.divide_flag), _p(pop.division_offset), n)
+    nd = int(pop.division_offset[n - 1])
+    if nd == 0:
+        return 0
+    if n + nd > pop.capacity:
+        raise RuntimeError("capacity exceeded")
+    lib().orc_place_daughters(n, params.placement_factor, _p(pop.divide_flag), _p(pop.division_offset),
+                              _p(pop.position), _p(pop.radius), _p(pop.cell_state), _p(pop.cell_type), _p(pop.age),
+                              _p(pop.oxygen_local), _p(pop.glucose_local), _p(pop.rng_state), _p(pop.velocity),
+                              _p(pop.neighbor_count), _p(pop.fate_flags), n)
+    pop.count = n + nd
+    return nd
+
+
+@dataclass
+class OracleOxygen:
+    """Oxygen field state for the oracle step (fields/species.py:190-195 with oxygen weights)."""
+
+    shape: tuple
+    origin: tuple
+    args: SolveArgs
+    omega: float
+    max_sweeps: int
+    tolerance: float
+    check_every: int
+    values: np.ndarray = None
+    density: np.ndarray = None
+    weights: tuple = (1.0, 1.0, 1.0, 0.0)
+
+    def __post_init__(self):
+        if self.values is None:
+            self.values = np.full(self.shape, self.args.boundary_value, dtype=np.float32)
+            apply_dirichlet(self.values, self.args.boundary, self.args.boundary_value)
+
+    def update(self, pop: OraclePopulation) -> tuple[int, float, bool]:
+        acc = deposit_accumulator(pop.position[: pop.count], pop.cell_state[: pop.count], self.origin, self.args.dx,
+                                  self.shape)
+        self.density = combine(self.weights, fixed_point_to_density(acc, self.args.dx))
+        report = solve(self.values, self.density, self.args, self.omega, self.max_sweeps, self.tolerance,
+                       self.check_every)
+        pop.oxygen_local[: pop.count] = sample(self.values, pop.position[: pop.count], self.origin, self.args.dx)
+        return report
+
+
+def oracle_cell_step(pop: OraclePopulation, lifecycle, contact, stepping, dim: int = 128, oxygen=None):
+    """simulation/simulator.py:58-81 on the oracle."""
+    report = (0, 0.0, True)
+    if oxygen is not None:
+        report = oxygen.update(pop)
+    lifecycle_decide(pop, lifecycle, stepping.dt_cells)
+    daughters = commit_divisions(pop, lifecycle)
+    n = pop.count
+    pos = np.ascontiguousarray(pop.position[:n])
+    vel = np.zeros((n, 3), dtype=np.float32)
+    cnt = np.zeros(n, dtype=np.int32)
+    relax_contacts(pos, pop.radius[:n].copy(), contact.query_radius, dim, contact.query_radius, contact.rate,
+                   stepping.dt_mechanics, stepping.mechanics_substeps, vel, cnt)
+    pop.position[:n] = pos
+    pop.velocity[:n] = vel
+    pop.neighbor_count[:n] = cnt
+    return daughters, report

@@ -1,0 +1,98 @@
+"""Independent numpy restatements used to pin the oracle (brute force, dense solves, closed
+forms). Written in float64 from the mathematics, not from the oracle's code."""
+
+from __future__ import annotations
+
+import numpy as np
+
+
+def trilinear_deposit_f64(positions, consuming, origin, dx, shape):
+    """Cells / dx^3 from unit trilinear weights (the reference's deposit_reference)."""
+    u = (np.asarray(positions, dtype=np.float64) - np.asarray(origin)) / dx
+    base = np.floor(u).astype(int)
+    frac = u - base
+    out = np.zeros(shape)
+    for c in range(len(u)):
+        i0, j0, k0 = base[c]
+        if min(i0, j0, k0) < 0 or i0 + 1 >= shape[0] or j0 + 1 >= shape[1] or k0 + 1 >= shape[2]:
+            continue
+        if not consuming[c]:
+            continue
+        fx, fy, fz = frac[c]
+        for di, wx in ((0, 1 - fx), (1, fx)):
+            for dj, wy in ((0, 1 - fy), (1, fy)):
+                for dk, wz in ((0, 1 - fz), (1, fz)):
+                    out[i0 + di, j0 + dj, k0 + dk] += wx * wy * wz
+    return out / dx ** 3
+
+
+def _mirror(idx, n):
+    return 1 if idx < 0 else (n - 2 if idx > n - 1 else idx)
+
+
+def dirichlet_mask(shape, boundary):
+    mask = np.zeros(shape, dtype=bool)
+    for axis, b in enumerate(boundary):
+        if b == 0:
+            idx = [slice(None)] * 3
+            idx[axis] = 0
+            mask[tuple(idx)] = True
+            idx[axis] = -1
+            mask[tuple(idx)] = True
+    return mask
+
+
+def steady_state_dense(values0, density, D, q, K, dx, boundary, picard=40, tol=1e-12):
+    """Discrete steady state by a dense linear solve inside a Picard loop on the uptake."""
+    v = np.asarray(values0, dtype=np.float64).copy()
+    dens = np.asarray(density, dtype=np.float64)
+    shape = v.shape
+    fixed = dirichlet_mask(shape, boundary)
+    unknown = np.argwhere(~fixed)
+    index = -np.ones(shape, dtype=int)
+    for row, (i, j, k) in enumerate(unknown):
+        index[i, j, k] = row
+    m = len(unknown)
+    nx, ny, nz = shape
+    for _ in range(picard):
+        a = np.zeros((m, m))
+        rhs = np.zeros(m)
+        for row, (i, j, k) in enumerate(unknown):
+            c = dens[i, j, k] * q / (K + v[i, j, k])
+            a[row, row] = -(6.0 + dx ** 2 * c / D)
+            for nb in ((_mirror(i - 1, nx), j, k), (_mirror(i + 1, nx), j, k), (i, _mirror(j - 1, ny), k),
+                       (i, _mirror(j + 1, ny), k), (i, j, _mirror(k - 1, nz)), (i, j, _mirror(k + 1, nz))):
+                col = index[nb]
+                if col >= 0:
+                    a[row, col] += 1.0
+                else:
+                    rhs[row] -= v[nb]
+        sol = np.linalg.solve(a, rhs)
+        new = v.copy()
+        new[~fixed] = sol
+        change = np.abs(new - v).max()
+        v = new
+        if change < tol:
+            break
+    return v
+
+
+def contact_brute_force(positions, radii, rate, query_radius, eps=1e-6):
+    x = np.asarray(positions, dtype=np.float64)
+    r = np.asarray(radii, dtype=np.float64)
+    d_vec = x[:, None, :] - x[None, :, :]
+    d = np.linalg.norm(d_vec, axis=2)
+    np.fill_diagonal(d, np.inf)
+    near = d < query_radius
+    overlap = r[:, None] + r[None, :] - d
+    active = near & (overlap > 0.0) & (d > eps)
+    with np.errstate(divide="ignore", invalid="ignore"):
+        coef = np.where(active, rate * overlap / d, 0.0)
+    return np.einsum("ij,ijk->ik", coef, d_vec), near.sum(axis=1).astype(np.int32)
+
+
+def pcg_py(v: int) -> int:
+    """PCG-RXS-M-XS 32-bit hash with Python integers (Jarzynski & Olano 2020, listing pcg_hash)."""
+    state = (v * 747796405 + 2891336453) % 2 ** 32
+    word = (((state >> ((state >> 28) + 4)) ^ state) * 277803737) % 2 ** 32
+    return (word >> 22) ^ word

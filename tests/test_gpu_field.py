@@ -1,0 +1,157 @@
+"""GPU parity of the field path (deposit, density, SOR, residual, solve loop, sampling) against
+the oracle on the same inputs. The HIP kernels and the oracle evaluate the same expressions with
+one rounding per operation, so integer AND float results are compared bit for bit; the error
+against a float64 solve is reported alongside."""
+
+import numpy as np
+import pytest
+import torch
+
+from warpbiocell_amd import workloads as W
+from warpbiocell_amd.cells.initialization import spherical_cluster
+from warpbiocell_amd.cells.state import CellPopulation
+from warpbiocell_amd.fields import diffusion as Dn
+from warpbiocell_amd.fields.densities import CellDensities
+from warpbiocell_amd.fields.diffusion import SolverSettings, UptakeKinetics
+from warpbiocell_amd.fields.oxygen import OxygenField, OxygenParams
+from warpbiocell_amd.fields.scalar_field import Boundary, GridGeometry, ScalarField
+
+pytestmark = pytest.mark.gpu
+
+MIXED = (Boundary.DIRICHLET, Boundary.NEUMANN, Boundary.DIRICHLET)
+
+
+def _cluster_with_states(n, seed, device, outside=0):
+    rng = np.random.default_rng(seed)
+    pos = spherical_cluster(n, 8.0, spacing_factor=0.9, jitter=0.3, seed=seed)
+    if outside:
+        pos[:outside] += np.float32(5000.0)
+    states = rng.integers(0, 4, n).astype(np.int32)
+    return pos, states, CellPopulation.from_numpy(pos, np.full(n, 8.0), capacity=n + 17, device=device, states=states)
+
+
+def test_deposit_accumulator_is_bit_exact_and_order_independent(oracle, device):
+    pos, states, pop = _cluster_with_states(5000, 21, device, outside=37)
+    geom = GridGeometry.centered_cube(600.0, 20.0)
+    dens = CellDensities(geom, device=device)
+    expected = oracle.deposit_accumulator(pos, states, geom.origin, geom.dx, geom.shape)
+    for _ in range(3):
+        dens.deposit(pop)
+        np.testing.assert_array_equal(dens.accumulator.cpu().numpy(), expected)
+    np.testing.assert_array_equal(dens.density.cpu().numpy(), oracle.fixed_point_to_density(expected, geom.dx))
+
+
+def test_fused_density_combine_is_bit_exact(oracle, device):
+    pos, states, pop = _cluster_with_states(3000, 5, device)
+    geom = GridGeometry.centered_cube(500.0, 12.5)
+    dens = CellDensities(geom, device=device)
+    out = torch.zeros(geom.shape, dtype=torch.float32, device=device)
+    w = (1.0, 0.75, 0.5, 0.0)
+    dens.deposit_combined(pop, w, out)
+    per_state = oracle.fixed_point_to_density(oracle.deposit_accumulator(pos, states, geom.origin, geom.dx,
+                                                                         geom.shape), geom.dx)
+    np.testing.assert_array_equal(out.cpu().numpy(), oracle.combine(w, per_state))
+    np.testing.assert_array_equal(dens.density.cpu().numpy(), per_state)  # lazily materialised
+    out2 = torch.zeros_like(out)
+    dens.combine((0.3, 0.0, 1.0, 2.0), out2)
+    np.testing.assert_array_equal(out2.cpu().numpy(), oracle.combine((0.3, 0.0, 1.0, 2.0), per_state))
+
+
+def test_sampling_is_bit_exact_and_clamps(oracle, device):
+    geom = GridGeometry.centered_cube(200.0, 25.0)
+    rng = np.random.default_rng(1)
+    values = rng.uniform(0.0, 50.0, geom.shape).astype(np.float32)
+    pos = rng.uniform(-130.0, 130.0, size=(4000, 3)).astype(np.float32)
+    ox = OxygenField(geom, OxygenParams(), boundary=(Boundary.NEUMANN,) * 3, device=device)
+    ox.field.assign(values)
+    pop = CellPopulation.from_numpy(pos, np.full(4000, 8.0), device=device)
+    ox.sample_at_cells(pop)
+    np.testing.assert_array_equal(pop.oxygen_numpy(), oracle.sample(values, pos, geom.origin, geom.dx))
+
+
+@pytest.mark.parametrize("boundary", [(0, 0, 0), (0, 1, 0), (1, 1, 1)])
+@pytest.mark.parametrize("pinned", [False, True])
+def test_colour_passes_and_residual_are_bit_exact(oracle, device, boundary, pinned):
+    geom = GridGeometry(origin=(0.0, 0.0, 0.0), dx=10.0, shape=(19, 23, 37))
+    rng = np.random.default_rng(7)
+    values = rng.uniform(5.0, 20.0, geom.shape).astype(np.float32)
+    density = rng.uniform(0.0, 2.0e-3, geom.shape).astype(np.float32)
+    production = rng.uniform(0.0, 10.0, geom.shape).astype(np.float32) if pinned else None
+    fixed = (rng.random(geom.shape) < 0.05) if pinned else None
+    kin = UptakeKinetics(2.0e3, 5.0e4, 3.0)
+    f = ScalarField.create(geom, 20.0, boundary=tuple(Boundary(b) for b in boundary), device=device)
+    f.assign(values)
+    if pinned:
+        f.set_fixed(fixed)
+    ref = f.numpy()
+    a = oracle.SolveArgs(dx=10.0, diffusion=2.0e3, uptake_max=5.0e4, michaelis_k=3.0, boundary_value=20.0,
+                         boundary=boundary)
+    d_t = torch.from_numpy(density).to(device)
+    p_t = None if production is None else torch.from_numpy(production).to(device)
+    fx = None if fixed is None else fixed.astype(np.int32)
+    for sweep in range(6):
+        for color in (0, 1):
+            Dn.sor_sweep(f, d_t, kin, 1.7, color, production=p_t)
+            oracle.sor_sweep(ref, density, a, 1.7, color, fixed=fx, production=production)
+        np.testing.assert_array_equal(f.numpy(), ref, err_msg=f"sweep {sweep}")
+    res_buf = torch.zeros(1, dtype=torch.float32, device=device)
+    gpu_res = Dn.steady_state_residual(f, d_t, kin, res_buf, production=p_t)
+    assert np.float32(gpu_res) == np.float32(oracle.residual(ref, density, a, fixed=fx, production=production))
+
+
+def test_solve_loop_matches_the_oracle_report_and_field(oracle, device):
+    """The whole chunked loop: same sweeps, same residual, same field bits (41^3 spheroid)."""
+    w = W.spheroid_1k_41()
+    geom = GridGeometry.centered_cube(w.box_um, w.dx_um)
+    settings = SolverSettings(omega=1.8, tolerance=1e-6, max_sweeps=2000, check_every=10)
+    params = OxygenParams(W.D_UM2_PER_H, W.UPTAKE_MMHG_UM3_PER_H, W.MICHAELIS_K, 40.0)
+    ox = OxygenField(geom, params, settings, device=device)
+    pop = CellPopulation.from_numpy(w.positions, w.radii, device=device)
+    a = oracle.SolveArgs(dx=geom.dx, diffusion=W.D_UM2_PER_H, uptake_max=W.UPTAKE_MMHG_UM3_PER_H,
+                         michaelis_k=W.MICHAELIS_K, boundary_value=40.0)
+    ref = oracle.OracleOxygen(shape=geom.shape, origin=geom.origin, args=a, omega=1.8, max_sweeps=2000,
+                              tolerance=1e-6, check_every=10)
+    opop = oracle.OraclePopulation.from_numpy(w.positions, w.radii)
+    for step in range(3):
+        report = ox.update(pop)
+        expected = ref.update(opop)
+        assert (report.sweeps, np.float32(report.residual), report.converged) == \
+               (expected[0], np.float32(expected[1]), expected[2]), f"step {step}"
+        np.testing.assert_array_equal(ox.numpy(), ref.values)
+        np.testing.assert_array_equal(pop.oxygen_numpy(), opop.oxygen_local[: opop.count])
+    # Error of the FP32 field against an FP64 solve of the same discrete problem.
+    f64 = ref.values.astype(np.float64)
+    oracle.solve(f64, ref.density.astype(np.float64), a, 1.8, 4000, 1e-12, 10)
+    err = np.abs(ox.numpy() - f64).max() / 40.0
+    print(f"41^3 field: max |fp32 - fp64| / O_b = {err:.3e}")
+    assert err < 1e-4
+
+
+def test_max_sweeps_and_pipelined_checks(oracle, device):
+    """Budget exhaustion mid-chunk and convergence inside a queued-ahead chunk."""
+    geom = GridGeometry.centered_cube(300.0, 20.0)
+    kin = UptakeKinetics(7.2e6, 1.4e8, 3.4)
+    a = oracle.SolveArgs(dx=20.0, diffusion=7.2e6, uptake_max=1.4e8, michaelis_k=3.4, boundary_value=38.0)
+    for max_sweeps, tol, check in ((25, 1e-9, 10), (2000, 1e-5, 7), (3, 1e-5, 1)):
+        f = ScalarField.create(geom, 38.0, initial_value=0.0, device=device)
+        ref = f.numpy()
+        density = np.zeros(geom.shape, np.float32)
+        rep = Dn.solve_steady_state(f, torch.from_numpy(density).to(device), kin,
+                                    SolverSettings(1.8, max_sweeps, tol, check))
+        exp = oracle.solve(ref, density, a, 1.8, max_sweeps, tol, check)
+        assert (rep.sweeps, np.float32(rep.residual), rep.converged) == (exp[0], np.float32(exp[1]), exp[2])
+        np.testing.assert_array_equal(f.numpy(), ref)
+
+
+def test_cosh_profile_on_the_gpu(device):
+    L, nx = 200.0, 81
+    geom = GridGeometry(origin=(-L, 0.0, 0.0), dx=2 * L / (nx - 1), shape=(nx, 3, 3))
+    D, kappa, K = 1.0e4, 3.0 / L, 1.0e9
+    params = OxygenParams(diffusion_coefficient=D, uptake_max=kappa ** 2 * D * K, michaelis_k=K, boundary_value=1.0)
+    ox = OxygenField(geom, params, SolverSettings(omega=1.5, tolerance=1e-5, max_sweeps=20000, check_every=10),
+                     boundary=(Boundary.DIRICHLET, Boundary.NEUMANN, Boundary.NEUMANN), device=device)
+    ox.density.fill_(1.0)
+    report = ox.solve_steady_state()
+    x = geom.node_coordinates()[0]
+    assert report.converged
+    np.testing.assert_allclose(ox.numpy()[:, 1, 1], np.cosh(kappa * x) / np.cosh(kappa * L), atol=2e-3)

@@ -1,0 +1,111 @@
+"""GPU parity of the hash-grid build and the contact mechanics: bucket order and neighbour
+counts exact, velocities and positions bit-exact against the oracle; the reference's
+brute-force and analytic checks restated on the GPU path."""
+
+import numpy as np
+import pytest
+
+from tests.helpers import contact_brute_force
+from warpbiocell_amd.cells.initialization import spherical_cluster
+from warpbiocell_amd.cells.mechanics import ContactParams, contact_substep, make_neighbor_grid, relax_contacts
+from warpbiocell_amd.cells.state import CellPopulation
+from warpbiocell_amd.spatial.neighbors import NeighborGrid
+
+pytestmark = pytest.mark.gpu
+
+
+@pytest.mark.parametrize("dim", [5, 128])
+def test_hash_grid_order_is_bit_exact(oracle, device, dim):
+    pos = spherical_cluster(3000, 8.0, spacing_factor=0.85, jitter=0.3, seed=2)
+    pop = CellPopulation.from_numpy(pos, np.full(3000, 8.0), capacity=3100, device=device)
+    grid = NeighborGrid(16.5, dim=dim, device=device)
+    for _ in range(2):  # the counts reset themselves: a second build must agree too
+        grid.build(pop)
+        start, ids = oracle.hashgrid_build(pos, 16.5, dim)
+        np.testing.assert_array_equal(grid.point_ids().cpu().numpy(), ids)
+        np.testing.assert_array_equal(grid.bucket_starts().cpu().numpy(), start)
+
+
+def test_counts_match_brute_force_and_velocities_are_bit_exact(oracle, device):
+    rng = np.random.default_rng(5)
+    pos = spherical_cluster(2000, 8.0, spacing_factor=0.9, jitter=0.3, seed=5)
+    radii = rng.uniform(5.0, 10.0, 2000).astype(np.float32)
+    params = ContactParams(stiffness=1.0, damping=1.0, max_radius=10.0, margin=0.5)
+    pop = CellPopulation.from_numpy(pos, radii, device=device)
+    contact_substep(pop, make_neighbor_grid(params, device), params, dt=0.0)
+    vel, cnt = oracle.contact_velocities(pos, radii, np.float32(params.query_radius), 128, params.query_radius,
+                                         params.rate)
+    _, brute = contact_brute_force(pos, radii, params.rate, params.query_radius)
+    np.testing.assert_array_equal(pop.neighbor_counts_numpy(), brute)
+    np.testing.assert_array_equal(pop.neighbor_counts_numpy(), cnt)
+    np.testing.assert_array_equal(pop.velocity.cpu().numpy(), vel)
+    np.testing.assert_array_equal(pop.positions_numpy(), pos)
+
+
+def test_relaxation_substeps_are_bit_exact(oracle, device):
+    r, n = 8.0, 3000
+    pos = spherical_cluster(n, r, spacing_factor=0.85, jitter=0.2, seed=21)
+    radii = np.random.default_rng(1).uniform(6.0, 9.0, n).astype(np.float32)
+    params = ContactParams(stiffness=10.0, damping=1.0, max_radius=9.0, margin=0.5)
+    pop = CellPopulation.from_numpy(pos, radii, capacity=n + 100, device=device)
+    relax_contacts(pop, make_neighbor_grid(params, device), params, dt=0.01, substeps=40)
+    ref = pos.copy()
+    vel = np.zeros_like(ref)
+    cnt = np.zeros(n, np.int32)
+    oracle.relax_contacts(ref, radii, np.float32(params.query_radius), 128, params.query_radius, params.rate, 0.01, 40,
+                          vel, cnt)
+    np.testing.assert_array_equal(pop.positions_numpy(), ref)
+    np.testing.assert_array_equal(pop.neighbor_counts_numpy(), cnt)
+    assert np.abs(ref - pos).max() > 0.5
+
+
+def test_substep_api_matches_the_fused_loop(device):
+    pos = spherical_cluster(800, 8.0, spacing_factor=0.85, jitter=0.2, seed=7)
+    params = ContactParams(stiffness=10.0, damping=1.0, max_radius=8.0, margin=0.5)
+    a = CellPopulation.from_numpy(pos, np.full(800, 8.0), device=device)
+    b = CellPopulation.from_numpy(pos, np.full(800, 8.0), device=device)
+    ga, gb = make_neighbor_grid(params, device), make_neighbor_grid(params, device)
+    for _ in range(10):
+        contact_substep(a, ga, params, 0.01)
+    relax_contacts(b, gb, params, 0.01, 10)
+    np.testing.assert_array_equal(a.positions_numpy(), b.positions_numpy())
+    np.testing.assert_array_equal(a.velocity.cpu().numpy(), b.velocity.cpu().numpy())
+
+
+def test_grid_only_indexes_active_prefix(device):
+    pos = np.array([[0.0, 0.0, 0.0], [1.0, 0.0, 0.0]], dtype=np.float32)
+    params = ContactParams(stiffness=1.0, damping=1.0, max_radius=8.0)
+    pop = CellPopulation.from_numpy(pos, np.full(2, 8.0), capacity=50, device=device)
+    contact_substep(pop, make_neighbor_grid(params, device), params, dt=0.0)
+    np.testing.assert_array_equal(pop.neighbor_counts_numpy(), [1, 1])
+
+
+def test_isolated_cells_and_single_cell(device):
+    pos = np.array([[0.0, 0.0, 0.0], [100.0, 0.0, 0.0], [0.0, 100.0, 0.0]], dtype=np.float32)
+    params = ContactParams(stiffness=10.0, damping=1.0, max_radius=8.0)
+    pop = CellPopulation.from_numpy(pos, np.full(3, 8.0), device=device)
+    relax_contacts(pop, make_neighbor_grid(params, device), params, dt=0.01, substeps=50)
+    np.testing.assert_array_equal(pop.positions_numpy(), pos)
+    assert np.all(pop.neighbor_counts_numpy() == 0)
+
+
+def test_pair_overlap_decays_at_the_analytic_rate(device):
+    r, overlap0, dt, n = 8.0, 3.0, 0.01, 15
+    pos = np.array([[0.0, 0.0, 0.0], [2 * r - overlap0, 0.0, 0.0]], dtype=np.float32)
+    params = ContactParams(stiffness=10.0, damping=1.0, max_radius=r)
+    pop = CellPopulation.from_numpy(pos, np.full(2, r), device=device)
+    relax_contacts(pop, make_neighbor_grid(params, device), params, dt=dt, substeps=n)
+    x = pop.positions_numpy()
+    assert 2 * r - np.linalg.norm(x[1] - x[0]) == pytest.approx(overlap0 * (1 - 2 * params.rate * dt) ** n, rel=1e-3)
+    assert np.all(x[:, 1:] == 0.0)
+
+
+def test_runs_are_bitwise_reproducible(device):
+    def run():
+        pos = spherical_cluster(3000, 8.0, spacing_factor=0.85, jitter=0.2, seed=21)
+        pop = CellPopulation.from_numpy(pos, np.full(3000, 8.0), device=device)
+        params = ContactParams(stiffness=10.0, damping=1.0, max_radius=8.0, margin=0.5)
+        relax_contacts(pop, make_neighbor_grid(params, device), params, dt=0.01, substeps=100)
+        return pop.positions_numpy()
+
+    np.testing.assert_array_equal(run(), run())

@@ -1,0 +1,88 @@
+"""Host-side logic of the drop-in API that runs without a GPU: parameter validation and geometry
+(the reference's ValueError contract)."""
+
+import numpy as np
+import pytest
+
+from warpbiocell_amd.cells.initialization import spherical_cluster
+from warpbiocell_amd.cells.lifecycle import LifecycleParams
+from warpbiocell_amd.cells.mechanics import ContactParams
+from warpbiocell_amd.fields.diffusion import SolverSettings, UptakeKinetics
+from warpbiocell_amd.fields.oxygen import OxygenParams, species_params
+from warpbiocell_amd.fields.scalar_field import GridGeometry
+from warpbiocell_amd.simulation.simulator import TimeStepping
+
+
+def test_geometry_helpers():
+    geom = GridGeometry.centered_cube(400.0, 20.0)
+    assert geom.shape == (21, 21, 21) and geom.origin == (-200.0, -200.0, -200.0) and geom.upper == (200.0,) * 3
+    inside = geom.contains(np.array([[0.0, 0.0, 0.0], [199.0, -199.0, 0.0], [201.0, 0.0, 0.0]]))
+    np.testing.assert_array_equal(inside, [True, True, False])
+    assert GridGeometry.centered_cube(1600.0, 10.0).shape == (161, 161, 161)
+    assert GridGeometry.centered_cube(800.0, 20.0).shape == (41, 41, 41)
+    with pytest.raises(ValueError):
+        GridGeometry(origin=(0, 0, 0), dx=0.0, shape=(4, 4, 4))
+    with pytest.raises(ValueError):
+        GridGeometry(origin=(0, 0, 0), dx=1.0, shape=(2, 4, 4))
+
+
+def test_solver_and_kinetics_validation():
+    with pytest.raises(ValueError):
+        SolverSettings(omega=2.0)
+    with pytest.raises(ValueError):
+        SolverSettings(check_every=0)
+    with pytest.raises(ValueError):
+        SolverSettings(tolerance=0.0)
+    with pytest.raises(ValueError):
+        UptakeKinetics(0.0, 1.0, 1.0)
+    with pytest.raises(ValueError):
+        UptakeKinetics(1.0, 1.0, 0.0)
+    with pytest.raises(ValueError):
+        OxygenParams(boundary_value=-1.0)
+
+
+def test_oxygen_weights_cover_living_states():
+    sp = species_params(OxygenParams(uptake_max=2.0))
+    assert sp.uptake_weights == (1.0, 1.0, 1.0, 0.0)
+    sp = species_params(OxygenParams(uptake_max=2.0), {"hypoxic": 0.5})
+    assert sp.uptake_weights == (1.0, 1.0, 0.5, 0.0)
+
+
+def test_contact_stability_rule():
+    params = ContactParams(stiffness=10.0, damping=1.0, max_radius=8.0)
+    params.check_substep(0.04)
+    with pytest.raises(ValueError):
+        params.check_substep(0.1)
+    with pytest.raises(ValueError):
+        params.check_substep(0.0)
+    assert params.query_radius == 16.0 and params.rate == 10.0
+    with pytest.raises(ValueError):
+        ContactParams(stiffness=-1.0, damping=1.0, max_radius=8.0)
+
+
+def test_lifecycle_params_validation():
+    with pytest.raises(ValueError):
+        LifecycleParams(division_rate=-1.0)
+    with pytest.raises(ValueError):
+        LifecycleParams(inhibition_threshold=0)
+    with pytest.raises(ValueError):
+        LifecycleParams(placement_factor=0.0)
+    with pytest.raises(ValueError):
+        LifecycleParams(hypoxia_threshold=1.0, death_threshold=2.0)
+    assert LifecycleParams.oxygen_dependent().hypoxia_threshold == 8.0
+
+
+def test_time_stepping():
+    assert TimeStepping(dt_cells=0.25, dt_mechanics=0.025).mechanics_substeps == 10
+    assert TimeStepping(dt_cells=0.1, dt_mechanics=0.025).mechanics_substeps == 4
+    with pytest.raises(ValueError):
+        TimeStepping(dt_cells=0.1, dt_mechanics=0.2)
+
+
+def test_spherical_cluster_is_deterministic_and_compact():
+    a = spherical_cluster(500, 8.0, seed=3)
+    np.testing.assert_array_equal(a, spherical_cluster(500, 8.0, seed=3))
+    assert a.shape == (500, 3) and a.dtype == np.float32
+    d = np.linalg.norm(a[:, None] - a[None], axis=2)
+    np.fill_diagonal(d, np.inf)
+    assert d.min() > 0.5 * 0.9 * 16.0

@@ -1,0 +1,174 @@
+// Per-cell RNG streams, lifecycle decisions and prefix-sum slot allocation for daughters.
+//
+// Replaces: cells/state.py:25-28 (_init_rng), kernels/cell_kernels.py:77-154
+// (lifecycle_decide), cells/lifecycle.py:146 + kernels/cell_kernels.py:157-200
+// (array_scan + place_daughters).
+#include "wbc_scan.hpp"
+
+namespace wbc {
+
+__global__ void __launch_bounds__(256) init_rng_kernel(uint32_t* __restrict__ rng_state, int64_t capacity,
+                                                        int32_t seed)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i < capacity) rng_state[i] = pcg((uint32_t)seed + pcg((uint32_t)(int32_t)i));
+}
+
+__device__ __forceinline__ float oxygen_factor(float o, float hypoxia_threshold, float death_threshold)
+{
+    if (o >= hypoxia_threshold) return 1.0f;
+    if (o <= death_threshold) return 0.0f;
+    return (o - death_threshold) / (hypoxia_threshold - death_threshold);
+}
+
+struct LifecycleArgs {
+    float dt, death_rate, anoxic_death_rate, division_rate;
+    int32_t inhibition_threshold;
+    float hypoxia_threshold, death_threshold, glucose_threshold, glucose_death_threshold;
+    int32_t necrosis_requires_glucose, phenotype_model;
+    float apoptosis_rate, necrosis_rate;
+};
+
+__global__ void __launch_bounds__(256) lifecycle_decide_kernel(
+    LifecycleArgs a, int32_t* __restrict__ cell_state, float* __restrict__ age,
+    const int32_t* __restrict__ neighbor_count, const float* __restrict__ oxygen_local,
+    const float* __restrict__ glucose_local, const int32_t* __restrict__ fate_flags,
+    uint32_t* __restrict__ rng_state, int32_t* __restrict__ divide_flag, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    divide_flag[i] = 0;
+    if (cell_state[i] == kStateDead) return;
+    age[i] = age[i] + a.dt;
+    uint32_t rng = rng_state[i];
+    float o = oxygen_local[i];
+    float g = glucose_local[i];
+    int32_t flags = a.phenotype_model == 1 ? fate_flags[i] : 0;
+
+    // Death first; its draw is always consumed (cell_kernels.py:115-132).
+    bool lethal = o < a.death_threshold;
+    if (a.necrosis_requires_glucose != 0) lethal = lethal && (g < a.glucose_death_threshold);
+    float rate = a.death_rate;
+    if (lethal) rate = fmaxf(a.death_rate, a.anoxic_death_rate);
+    if (flags & kFateNecrosis) rate = fmaxf(rate, a.necrosis_rate);
+    if (flags & kFateApoptosis) rate = fmaxf(rate, a.apoptosis_rate);
+    float p_death = 1.0f - exp_f32(-rate * a.dt);
+    if (randf(rng) < p_death) {
+        cell_state[i] = kStateDead;
+        rng_state[i] = rng;
+        return;
+    }
+
+    bool crowded = neighbor_count[i] >= a.inhibition_threshold;
+    bool may_divide = !crowded;
+    float factor = 1.0f;
+    if (a.phenotype_model == 1)
+        may_divide = may_divide && ((flags & kFateProliferation) != 0) && ((flags & kFateGrowthArrest) == 0);
+    else
+        factor = oxygen_factor(o, a.hypoxia_threshold, a.death_threshold) *
+                 oxygen_factor(g, a.glucose_threshold, a.glucose_death_threshold);
+
+    if (o < a.hypoxia_threshold)
+        cell_state[i] = kStateHypoxic;
+    else if (may_divide)
+        cell_state[i] = kStateProliferative;
+    else
+        cell_state[i] = kStateQuiescent;
+
+    if (may_divide) {
+        float p_divide = 1.0f - exp_f32(-a.division_rate * factor * a.dt);
+        if (randf(rng) < p_divide) divide_flag[i] = 1;
+    }
+    rng_state[i] = rng;
+}
+
+// The k-th dividing cell in slot order fills slot count + k (cell_kernels.py:179-180).
+__global__ void __launch_bounds__(256) place_daughters_kernel(
+    int32_t count, float placement_factor, const int32_t* __restrict__ divide_flag,
+    const int32_t* __restrict__ division_offset, float* __restrict__ position, float* __restrict__ radius,
+    int32_t* __restrict__ cell_state, int32_t* __restrict__ cell_type, float* __restrict__ age,
+    float* __restrict__ oxygen_local, float* __restrict__ glucose_local, uint32_t* __restrict__ rng_state,
+    float* __restrict__ velocity, int32_t* __restrict__ neighbor_count, int32_t* __restrict__ fate_flags, int64_t n)
+{
+    int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n || divide_flag[i] == 0) return;
+    int64_t j = (int64_t)count + division_offset[i] - 1;
+    uint32_t rng = rng_state[i];
+    float d[3];
+    sample_unit_sphere_surface(rng, d[0], d[1], d[2]);
+    rng_state[i] = rng;
+    float s = 0.5f * placement_factor * radius[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        float h = d[a] * s;
+        float x = position[3 * i + a];
+        position[3 * i + a] = x - h;
+        position[3 * j + a] = x + h;
+        velocity[3 * j + a] = 0.0f;
+    }
+    radius[j] = radius[i];
+    cell_state[j] = cell_state[i];
+    cell_type[j] = cell_type[i];
+    age[j] = 0.0f;
+    oxygen_local[j] = oxygen_local[i];
+    glucose_local[j] = glucose_local[i];
+    neighbor_count[j] = 0;
+    fate_flags[j] = fate_flags[i];
+    // rng_state[j] was seeded for every slot when the population was created; left untouched.
+}
+
+}  // namespace wbc
+
+using namespace wbc;
+
+WBC_API int wbc_init_rng(uint32_t* rng_state, int64_t capacity, int32_t seed, hipStream_t stream)
+{
+    if (capacity < 0 || capacity > INT32_MAX) return (int)hipErrorInvalidValue;
+    if (capacity == 0) return 0;
+    if (!rng_state) return (int)hipErrorInvalidValue;
+    init_rng_kernel<<<grid_for(capacity, 256), 256, 0, stream>>>(rng_state, capacity, seed);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+WBC_API int wbc_lifecycle_decide(float dt, float death_rate, float anoxic_death_rate, float division_rate,
+                                 int32_t inhibition_threshold, float hypoxia_threshold, float death_threshold,
+                                 float glucose_threshold, float glucose_death_threshold,
+                                 int32_t necrosis_requires_glucose, int32_t phenotype_model, float apoptosis_rate,
+                                 float necrosis_rate, int32_t* cell_state, float* age,
+                                 const int32_t* neighbor_count, const float* oxygen_local,
+                                 const float* glucose_local, const int32_t* fate_flags, uint32_t* rng_state,
+                                 int32_t* divide_flag, int64_t n, hipStream_t stream)
+{
+    if (n < 0 || n > INT32_MAX) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!cell_state || !age || !neighbor_count || !oxygen_local || !glucose_local || !fate_flags || !rng_state ||
+        !divide_flag)
+        return (int)hipErrorInvalidValue;
+    LifecycleArgs a{dt, death_rate, anoxic_death_rate, division_rate, inhibition_threshold, hypoxia_threshold,
+                    death_threshold, glucose_threshold, glucose_death_threshold, necrosis_requires_glucose,
+                    phenotype_model, apoptosis_rate, necrosis_rate};
+    lifecycle_decide_kernel<<<grid_for(n, 256), 256, 0, stream>>>(a, cell_state, age, neighbor_count, oxygen_local,
+                                                                   glucose_local, fate_flags, rng_state,
+                                                                   divide_flag, n);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+WBC_API int wbc_place_daughters(int32_t count, float placement_factor, const int32_t* divide_flag,
+                                const int32_t* division_offset, float* position, float* radius,
+                                int32_t* cell_state, int32_t* cell_type, float* age, float* oxygen_local,
+                                float* glucose_local, uint32_t* rng_state, float* velocity,
+                                int32_t* neighbor_count, int32_t* fate_flags, int64_t n, hipStream_t stream)
+{
+    if (n < 0 || n > INT32_MAX || count < 0) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!divide_flag || !division_offset || !position || !radius || !cell_state || !cell_type || !age ||
+        !oxygen_local || !glucose_local || !rng_state || !velocity || !neighbor_count || !fate_flags)
+        return (int)hipErrorInvalidValue;
+    place_daughters_kernel<<<grid_for(n, 256), 256, 0, stream>>>(
+        count, placement_factor, divide_flag, division_offset, position, radius, cell_state, cell_type, age,
+        oxygen_local, glucose_local, rng_state, velocity, neighbor_count, fate_flags, n);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}

@@ -1,0 +1,464 @@
+// Cell <-> grid transfer and the red-black SOR reaction-diffusion solve.
+//
+// Replaces kernels/field_kernels.py (deposit_trilinear :78-116, fixed_point_to_density
+// :119-126, combine_densities :129-142, sample_trilinear :153-182, sor_sweep :185-212,
+// steady_state_residual :215-243) and the solve loop of fields/diffusion.py:79-123.
+//
+// Grid arrays are C order (i, j, k) with k contiguous, exactly the reference's Warp 3-D
+// arrays; the deposit accumulator is (state, i, j, k) int64 like CellDensities.
+#include "wbc_field.hpp"
+
+namespace wbc {
+
+// --------------------------------------------------------------------------------------------
+// Deposit: 8 int64 atomics per cell; integer addition is associative, so the accumulator does
+// not depend on the order in which the atomics land.
+// --------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) deposit_trilinear_kernel(const float* __restrict__ position,
+                                                                 const int32_t* __restrict__ cell_state, int64_t n,
+                                                                 float ox, float oy, float oz, float inv_dx,
+                                                                 int32_t nx, int32_t ny, int32_t nz,
+                                                                 unsigned long long* __restrict__ acc)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    int32_t s = cell_state[c];
+    if (s < 0 || s >= 4) return;
+    float ux = (position[3 * c + 0] - ox) * inv_dx;
+    float uy = (position[3 * c + 1] - oy) * inv_dx;
+    float uz = (position[3 * c + 2] - oz) * inv_dx;
+    int32_t i0 = (int32_t)floorf(ux), j0 = (int32_t)floorf(uy), k0 = (int32_t)floorf(uz);
+    if (i0 < 0 || j0 < 0 || k0 < 0) return;
+    if (i0 + 1 >= nx || j0 + 1 >= ny || k0 + 1 >= nz) return;
+    float fx = ux - (float)i0, fy = uy - (float)j0, fz = uz - (float)k0;
+    const int64_t plane = (int64_t)ny * nz;
+    unsigned long long* base = acc + (int64_t)s * nx * plane + i0 * plane + (int64_t)j0 * nz + k0;
+#pragma unroll
+    for (int di = 0; di < 2; ++di) {
+        float wx = di == 0 ? 1.0f - fx : fx;
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            float wy = dj == 0 ? 1.0f - fy : fy;
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) {
+                float wz = dk == 0 ? 1.0f - fz : fz;
+                float w = wx * wy * wz;
+                long long q = (long long)(int32_t)(w * kFixedOne + 0.5f);
+                atomicAdd(base + di * plane + dj * nz + dk, (unsigned long long)q);
+            }
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) zero_u64_kernel(unsigned long long* __restrict__ p, int64_t count)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (; t < count; t += stride) p[t] = 0ull;
+}
+
+// density_s = float(acc_s) / 2^24 * inv_voxel_volume; combined = w0 d0 + w1 d1 + w2 d2 + w3 d3.
+__global__ void __launch_bounds__(256) density_finalize_kernel(const long long* __restrict__ acc, int64_t nodes,
+                                                                float inv_vol, float w0, float w1, float w2,
+                                                                float w3, float* __restrict__ combined,
+                                                                float* __restrict__ per_state)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nodes) return;
+    float d0 = (float)acc[t] / kFixedOne * inv_vol;
+    float d1 = (float)acc[nodes + t] / kFixedOne * inv_vol;
+    float d2 = (float)acc[2 * nodes + t] / kFixedOne * inv_vol;
+    float d3 = (float)acc[3 * nodes + t] / kFixedOne * inv_vol;
+    if (per_state) {
+        per_state[t] = d0;
+        per_state[nodes + t] = d1;
+        per_state[2 * nodes + t] = d2;
+        per_state[3 * nodes + t] = d3;
+    }
+    if (combined) combined[t] = w0 * d0 + w1 * d1 + w2 * d2 + w3 * d3;
+}
+
+__global__ void __launch_bounds__(256) combine_kernel(const float* __restrict__ per_state, int64_t nodes, float w0,
+                                                       float w1, float w2, float w3, float* __restrict__ out)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= nodes) return;
+    out[t] = w0 * per_state[t] + w1 * per_state[nodes + t] + w2 * per_state[2 * nodes + t] +
+             w3 * per_state[3 * nodes + t];
+}
+
+// --------------------------------------------------------------------------------------------
+// Sampling: clamp to the grid, then x-, y- and z-lerps in the reference's order.
+// --------------------------------------------------------------------------------------------
+__global__ void __launch_bounds__(256) sample_trilinear_kernel(const float* __restrict__ field, int32_t nx,
+                                                                int32_t ny, int32_t nz,
+                                                                const float* __restrict__ position, int64_t n,
+                                                                float ox, float oy, float oz, float inv_dx,
+                                                                float* __restrict__ out)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    float ux = (position[3 * c + 0] - ox) * inv_dx;
+    float uy = (position[3 * c + 1] - oy) * inv_dx;
+    float uz = (position[3 * c + 2] - oz) * inv_dx;
+    ux = fminf(fmaxf(ux, 0.0f), (float)(nx - 1));
+    uy = fminf(fmaxf(uy, 0.0f), (float)(ny - 1));
+    uz = fminf(fmaxf(uz, 0.0f), (float)(nz - 1));
+    int32_t i0 = min((int32_t)floorf(ux), nx - 2);
+    int32_t j0 = min((int32_t)floorf(uy), ny - 2);
+    int32_t k0 = min((int32_t)floorf(uz), nz - 2);
+    float fx = ux - (float)i0, fy = uy - (float)j0, fz = uz - (float)k0;
+    const int64_t plane = (int64_t)ny * nz;
+    const float* f = field + i0 * plane + (int64_t)j0 * nz + k0;
+    float c00 = f[0] * (1.0f - fx) + f[plane] * fx;
+    float c10 = f[nz] * (1.0f - fx) + f[plane + nz] * fx;
+    float c01 = f[1] * (1.0f - fx) + f[plane + 1] * fx;
+    float c11 = f[nz + 1] * (1.0f - fx) + f[plane + nz + 1] * fx;
+    float c0 = c00 * (1.0f - fy) + c10 * fy;
+    float c1 = c01 * (1.0f - fy) + c11 * fy;
+    out[c] = c0 * (1.0f - fz) + c1 * fz;
+}
+
+// --------------------------------------------------------------------------------------------
+// Red-black SOR, one colour per launch. Thread (m, j, i) owns node k = 2m + ((color+i+j)&1),
+// so every lane does useful work and neighbouring lanes touch neighbouring memory.
+// --------------------------------------------------------------------------------------------
+__device__ __forceinline__ float nb_sum(const float* __restrict__ f, int32_t nx, int32_t ny, int32_t nz, int32_t i,
+                                        int32_t j, int32_t k, int64_t t, int64_t plane)
+{
+    int64_t im = i > 0 ? t - plane : t + plane;        // mirror f[-1] = f[1]
+    int64_t ip = i < nx - 1 ? t + plane : t - plane;   // mirror f[n] = f[n-2]
+    int64_t jm = j > 0 ? t - nz : t + nz;
+    int64_t jp = j < ny - 1 ? t + nz : t - nz;
+    int64_t km = k > 0 ? t - 1 : t + 1;
+    int64_t kp = k < nz - 1 ? t + 1 : t - 1;
+    return f[im] + f[ip] + f[jm] + f[jp] + f[km] + f[kp];
+}
+
+__device__ __forceinline__ bool on_dirichlet_face(int32_t i, int32_t j, int32_t k, const SorParams& p)
+{
+    if (p.bc_x == 0 && (i == 0 || i == p.nx - 1)) return true;
+    if (p.bc_y == 0 && (j == 0 || j == p.ny - 1)) return true;
+    if (p.bc_z == 0 && (k == 0 || k == p.nz - 1)) return true;
+    return false;
+}
+
+__device__ __forceinline__ float production_at(const SorParams& p, int64_t t)
+{
+    if (!p.production) return 0.0f;
+    float sf = 1.0f;
+    if (p.has_source) {
+        float v = p.source[t];
+        sf = v / (p.source_k + v);
+    }
+    return p.production[t] * sf;
+}
+
+__global__ void __launch_bounds__(256) sor_color_kernel(SorParams p, int32_t color, float* __restrict__ field,
+                                                        const int32_t* __restrict__ done)
+{
+    if (done && *done) return;
+    const int32_t i = blockIdx.z;
+    const int32_t j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int32_t m = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= p.ny) return;
+    const int32_t k = 2 * m + ((color + i + j) & 1);
+    if (k >= p.nz) return;
+    if (on_dirichlet_face(i, j, k, p)) return;
+    const int64_t plane = (int64_t)p.ny * p.nz;
+    const int64_t t = i * plane + (int64_t)j * p.nz + k;
+    if (p.fixed && p.fixed[t] != 0) return;
+    float o = field[t];
+    float c = p.density[t] * p.uptake_max / (p.michaelis_k + o);
+    float prod = production_at(p, t);
+    float gs = (nb_sum(field, p.nx, p.ny, p.nz, i, j, k, t, plane) + p.dx2_over_D * prod) /
+               (6.0f + p.dx2_over_D * c);
+    float v = (1.0f - p.omega) * o + p.omega * gs;
+    field[t] = fmaxf(v, 0.0f);
+}
+
+// Max-norm relative residual over updated nodes; one atomic per block on the float's bits
+// (every candidate is >= 0, so unsigned order equals float order). `out` must start at 0.
+__global__ void __launch_bounds__(256) residual_kernel(SorParams p, const float* __restrict__ field,
+                                                       float* __restrict__ out, const int32_t* __restrict__ done)
+{
+    __shared__ float wave_max_lds[4];
+    if (done && *done) return;
+    const int32_t i = blockIdx.z;
+    const int32_t j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    float q = 0.0f;
+    if (j < p.ny && k < p.nz && !on_dirichlet_face(i, j, k, p)) {
+        const int64_t plane = (int64_t)p.ny * p.nz;
+        const int64_t t = i * plane + (int64_t)j * p.nz + k;
+        if (!(p.fixed && p.fixed[t] != 0)) {
+            float o = field[t];
+            float c = p.density[t] * p.uptake_max / (p.michaelis_k + o);
+            float prod = production_at(p, t);
+            float r = nb_sum(field, p.nx, p.ny, p.nz, i, j, k, t, plane) + p.dx2_over_D * prod -
+                      (6.0f + p.dx2_over_D * c) * o;
+            q = fabsf(r) / fmaxf(6.0f * o, p.reference);
+        }
+    }
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    q = wave_max(q);
+    if ((tid & (kWave - 1)) == 0) wave_max_lds[tid / kWave] = q;
+    __syncthreads();
+    if (tid == 0) {
+        float m = fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
+        if (m > 0.0f) atomicMax((unsigned int*)out, __float_as_uint(m));
+    }
+}
+
+// After a chunk: latch convergence on the device so later, speculatively queued sweeps no-op.
+__global__ void check_converged_kernel(const float* __restrict__ residual, int32_t chunk, float tolerance,
+                                       int32_t* __restrict__ state)
+{
+    if (state[0] == 0 && residual[chunk] < tolerance) {
+        state[0] = 1;
+        state[1] = chunk;
+    }
+}
+
+__global__ void __launch_bounds__(256) zero_i32_kernel(int32_t* __restrict__ p, int64_t count)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) p[t] = 0;
+}
+
+hipError_t launch_sor_color(const SorParams& p, int32_t color, float* field, const int32_t* done, hipStream_t s)
+{
+    dim3 block(64, 4, 1);
+    dim3 grid(grid_for((p.nz + 1) / 2, 64), grid_for(p.ny, 4), p.nx);
+    sor_color_kernel<<<grid, block, 0, s>>>(p, color, field, done);
+    return hipGetLastError();
+}
+
+hipError_t launch_residual(const SorParams& p, const float* field, float* out, const int32_t* done, hipStream_t s)
+{
+    dim3 block(64, 4, 1);
+    dim3 grid(grid_for(p.nz, 64), grid_for(p.ny, 4), p.nx);
+    residual_kernel<<<grid, block, 0, s>>>(p, field, out, done);
+    return hipGetLastError();
+}
+
+}  // namespace wbc
+
+using namespace wbc;
+
+static bool grid_ok(int32_t nx, int32_t ny, int32_t nz)
+{
+    return nx >= 3 && ny >= 3 && nz >= 3 && (int64_t)nx * ny * nz < ((int64_t)1 << 31) && nx < 65536;
+}
+
+WBC_API int wbc_fill_zero_i64(int64_t* p, int64_t count, hipStream_t stream)
+{
+    if (count < 0) return (int)hipErrorInvalidValue;
+    if (count == 0) return 0;
+    if (!p) return (int)hipErrorInvalidValue;
+    int64_t blocks = grid_for(count, 256);
+    if (blocks > 8192) blocks = 8192;
+    zero_u64_kernel<<<(unsigned)blocks, 256, 0, stream>>>((unsigned long long*)p, count);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+WBC_API int wbc_deposit_trilinear_i64(const float* position, const int32_t* cell_state, int64_t n, float ox,
+                                      float oy, float oz, float inv_dx, int32_t nx, int32_t ny, int32_t nz,
+                                      int64_t* accumulator, hipStream_t stream)
+{
+    if (n < 0 || n > INT32_MAX || !grid_ok(nx, ny, nz)) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!position || !cell_state || !accumulator) return (int)hipErrorInvalidValue;
+    deposit_trilinear_kernel<<<grid_for(n, 256), 256, 0, stream>>>(position, cell_state, n, ox, oy, oz, inv_dx, nx,
+                                                                    ny, nz, (unsigned long long*)accumulator);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+WBC_API int wbc_density_finalize(const int64_t* accumulator, int64_t nodes, float inv_voxel_volume, float w0,
+                                 float w1, float w2, float w3, float* combined, float* per_state,
+                                 hipStream_t stream)
+{
+    if (nodes <= 0 || nodes >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;
+    if (!accumulator || (!combined && !per_state)) return (int)hipErrorInvalidValue;
+    density_finalize_kernel<<<grid_for(nodes, 256), 256, 0, stream>>>((const long long*)accumulator, nodes,
+                                                                       inv_voxel_volume, w0, w1, w2, w3, combined,
+                                                                       per_state);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+WBC_API int wbc_combine_densities(const float* per_state, int64_t nodes, float w0, float w1, float w2, float w3,
+                                  float* out, hipStream_t stream)
+{
+    if (nodes <= 0 || nodes >= ((int64_t)1 << 31) || !per_state || !out) return (int)hipErrorInvalidValue;
+    combine_kernel<<<grid_for(nodes, 256), 256, 0, stream>>>(per_state, nodes, w0, w1, w2, w3, out);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+WBC_API int wbc_sample_trilinear(const float* field, int32_t nx, int32_t ny, int32_t nz, const float* position,
+                                 int64_t n, float ox, float oy, float oz, float inv_dx, float* out,
+                                 hipStream_t stream)
+{
+    if (n < 0 || n > INT32_MAX || !grid_ok(nx, ny, nz)) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!field || !position || !out) return (int)hipErrorInvalidValue;
+    sample_trilinear_kernel<<<grid_for(n, 256), 256, 0, stream>>>(field, nx, ny, nz, position, n, ox, oy, oz,
+                                                                   inv_dx, out);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+static int make_params(SorParams& p, float omega, float dx2_over_D, float uptake_max, float michaelis_k,
+                       float reference, int32_t bc_x, int32_t bc_y, int32_t bc_z, const int32_t* fixed,
+                       const float* density, const float* production, int32_t has_source, const float* source,
+                       float source_k, int32_t nx, int32_t ny, int32_t nz)
+{
+    if (!grid_ok(nx, ny, nz) || !density) return (int)hipErrorInvalidValue;
+    if (has_source && (!source || !production)) return (int)hipErrorInvalidValue;
+    p.omega = omega;
+    p.dx2_over_D = dx2_over_D;
+    p.uptake_max = uptake_max;
+    p.michaelis_k = michaelis_k;
+    p.reference = reference;
+    p.bc_x = bc_x;
+    p.bc_y = bc_y;
+    p.bc_z = bc_z;
+    p.fixed = fixed;
+    p.density = density;
+    p.production = production;
+    p.has_source = has_source;
+    p.source = source;
+    p.source_k = source_k;
+    p.nx = nx;
+    p.ny = ny;
+    p.nz = nz;
+    return 0;
+}
+
+WBC_API int wbc_sor_sweep(int32_t color, float omega, float dx2_over_D, float uptake_max, float michaelis_k,
+                          int32_t bc_x, int32_t bc_y, int32_t bc_z, const int32_t* fixed, const float* density,
+                          const float* production, int32_t has_source, const float* source, float source_k,
+                          float* field, int32_t nx, int32_t ny, int32_t nz, hipStream_t stream)
+{
+    SorParams p;
+    int rc = make_params(p, omega, dx2_over_D, uptake_max, michaelis_k, 0.0f, bc_x, bc_y, bc_z, fixed, density,
+                         production, has_source, source, source_k, nx, ny, nz);
+    if (rc) return rc;
+    if (!field || (color != 0 && color != 1)) return (int)hipErrorInvalidValue;
+    return (int)launch_sor_color(p, color, field, nullptr, stream);
+}
+
+WBC_API int wbc_steady_state_residual(float dx2_over_D, float uptake_max, float michaelis_k, float reference,
+                                      int32_t bc_x, int32_t bc_y, int32_t bc_z, const int32_t* fixed,
+                                      const float* density, const float* production, int32_t has_source,
+                                      const float* source, float source_k, const float* field, int32_t nx,
+                                      int32_t ny, int32_t nz, float* residual, hipStream_t stream)
+{
+    SorParams p;
+    int rc = make_params(p, 1.0f, dx2_over_D, uptake_max, michaelis_k, reference, bc_x, bc_y, bc_z, fixed, density,
+                         production, has_source, source, source_k, nx, ny, nz);
+    if (rc) return rc;
+    if (!field || !residual) return (int)hipErrorInvalidValue;
+    return (int)launch_residual(p, field, residual, nullptr, stream);
+}
+
+// ----------------------------------------------------------------------------------------------
+// The steady-state solve loop (fields/diffusion.py:79-123) with the host checks pipelined:
+// chunk c+1 is queued before the host reads chunk c's residual, and a device-side latch makes
+// the queued sweeps no-ops once a chunk converged, so results equal the synchronous loop's.
+// ----------------------------------------------------------------------------------------------
+namespace {
+constexpr int kEventPool = 8;
+constexpr int kLag = 2;  // chunks queued ahead of the host check (must stay below kEventPool)
+struct EventPool {
+    int device = -1;
+    hipEvent_t ev[kEventPool];
+};
+EventPool g_pools[16];
+
+hipError_t get_events(hipEvent_t** out)
+{
+    int dev = 0;
+    WBC_TRY_ERR(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 16) return hipErrorInvalidDevice;
+    EventPool& pool = g_pools[dev];
+    if (pool.device != dev) {
+        for (int e = 0; e < kEventPool; ++e) WBC_TRY_ERR(hipEventCreateWithFlags(&pool.ev[e], hipEventDisableTiming));
+        pool.device = dev;
+    }
+    *out = pool.ev;
+    return hipSuccess;
+}
+}  // namespace
+
+WBC_API size_t wbc_solve_workspace_bytes(int32_t max_sweeps, int32_t check_every)
+{
+    if (max_sweeps < 1 || check_every < 1) return 0;
+    int64_t chunks = (max_sweeps + check_every - 1) / check_every;
+    return (size_t)((4 + chunks) * 4 + 255) / 256 * 256;
+}
+
+WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolerance, int32_t check_every,
+                                   float dx2_over_D, float uptake_max, float michaelis_k, float reference,
+                                   int32_t bc_x, int32_t bc_y, int32_t bc_z, const int32_t* fixed,
+                                   const float* density, const float* production, int32_t has_source,
+                                   const float* source, float source_k, float* field, int32_t nx, int32_t ny,
+                                   int32_t nz, void* workspace, size_t workspace_bytes, float* host_residual,
+                                   int32_t* sweeps_out, float* residual_out, int32_t* converged_out,
+                                   hipStream_t stream)
+{
+    SorParams p;
+    int rc = make_params(p, omega, dx2_over_D, uptake_max, michaelis_k, reference, bc_x, bc_y, bc_z, fixed, density,
+                         production, has_source, source, source_k, nx, ny, nz);
+    if (rc) return rc;
+    if (!field || !workspace || !host_residual || !sweeps_out || !residual_out || !converged_out)
+        return (int)hipErrorInvalidValue;
+    size_t need = wbc_solve_workspace_bytes(max_sweeps, check_every);
+    if (need == 0 || workspace_bytes < need) return (int)hipErrorInvalidValue;
+    const int32_t chunks = (max_sweeps + check_every - 1) / check_every;
+    int32_t* state = (int32_t*)workspace;      // [0] converged latch, [1] converged chunk
+    float* slots = (float*)workspace + 4;      // one residual per chunk
+    hipEvent_t* ev;
+    WBC_TRY(get_events(&ev));
+    zero_i32_kernel<<<grid_for(4 + chunks, 256), 256, 0, stream>>>(state, 4 + chunks);
+    WBC_CHECK_LAUNCH();
+
+    int32_t queued = 0, checked = 0, sweeps = 0;
+    float residual = __builtin_huge_valf();
+    bool converged = false;
+    while (true) {
+        while (queued < chunks && queued - checked <= kLag) {
+            int32_t chunk = check_every;
+            if (max_sweeps - queued * check_every < chunk) chunk = max_sweeps - queued * check_every;
+            for (int32_t s = 0; s < chunk; ++s) {
+                WBC_TRY(launch_sor_color(p, 0, field, state, stream));
+                WBC_TRY(launch_sor_color(p, 1, field, state, stream));
+            }
+            WBC_TRY(launch_residual(p, field, slots + queued, state, stream));
+            check_converged_kernel<<<1, 1, 0, stream>>>(slots, queued, tolerance, state);
+            WBC_CHECK_LAUNCH();
+            WBC_TRY(hipMemcpyAsync(host_residual + queued, slots + queued, sizeof(float), hipMemcpyDeviceToHost,
+                                   stream));
+            WBC_TRY(hipEventRecord(ev[queued % kEventPool], stream));
+            ++queued;
+        }
+        if (checked == queued) break;
+        WBC_TRY(hipEventSynchronize(ev[checked % kEventPool]));
+        residual = host_residual[checked];
+        int32_t chunk = check_every;
+        if (max_sweeps - checked * check_every < chunk) chunk = max_sweeps - checked * check_every;
+        sweeps += chunk;
+        ++checked;
+        if (residual < tolerance) {
+            converged = true;
+            break;
+        }
+    }
+    *sweeps_out = sweeps;
+    *residual_out = residual;
+    *converged_out = converged ? 1 : 0;
+    return 0;
+}

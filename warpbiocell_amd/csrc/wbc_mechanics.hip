@@ -1,0 +1,305 @@
+// Hash-grid neighbour build and overdamped contact mechanics.
+//
+// Replaces spatial/neighbors.py:33-37 (wp.HashGrid.build over the active prefix),
+// kernels/mechanics_kernels.py:26-57 (contact_velocities), :60-67 (integrate_positions) and the
+// substep driver cells/mechanics.py:142-189.
+//
+// Build = stable counting sort of the active slots by bucket:
+//   1. key per point, atomic count per bucket;
+//   2. exclusive scan of the counts -> bucket start (start[dim^3] = n);
+//   3. scatter: slot = start + (atomic decrement of the count) - 1, which returns every count to
+//      zero for the next build;
+//   4. restore slot order inside each bucket (rank = number of smaller ids in the bucket) and
+//      write a bucket-ordered copy of (x, y, z, r) so the contact gather reads contiguous float4
+//      runs.
+// The bucket of p is floor(p / cell_size) per axis, each reduced modulo dim, linearised with z
+// fastest; a query walks x, then y, then z buckets, and each bucket in slot order. That fixes
+// the FP32 summation order of the gather, so the result does not depend on thread timing.
+#include "wbc_scan.hpp"
+
+namespace wbc {
+
+struct HashGridLayout {
+    size_t cnt, start, partial, key, tmp, id, xyzr, total;
+};
+
+static HashGridLayout hashgrid_layout(int32_t dim, int64_t capacity)
+{
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const int64_t nb = (int64_t)dim * dim * dim;
+    HashGridLayout L;
+    size_t off = 0;
+    L.cnt = off;     off += up(nb * 4);
+    L.start = off;   off += up((nb + 1) * 4);
+    L.partial = off; off += up(scan_workspace_bytes(nb));
+    L.key = off;     off += up(capacity * 4);
+    L.tmp = off;     off += up(capacity * 4);
+    L.id = off;      off += up(capacity * 4);
+    L.xyzr = off;    off += up(capacity * 16);
+    L.total = off;
+    return L;
+}
+
+__device__ __forceinline__ int32_t wrap_dim(int32_t c, int32_t dim)
+{
+    int32_t m = c % dim;
+    return m < 0 ? m + dim : m;
+}
+
+__device__ __forceinline__ int32_t cell_coord(float p, float inv_cell) { return (int32_t)floorf(p * inv_cell); }
+
+__global__ void __launch_bounds__(256) hg_key_count(const float* __restrict__ position, int32_t n, float inv_cell,
+                                                     int32_t dim, int32_t* __restrict__ key,
+                                                     int32_t* __restrict__ cnt)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t cx = wrap_dim(cell_coord(position[3 * i + 0], inv_cell), dim);
+    int32_t cy = wrap_dim(cell_coord(position[3 * i + 1], inv_cell), dim);
+    int32_t cz = wrap_dim(cell_coord(position[3 * i + 2], inv_cell), dim);
+    int32_t b = (cx * dim + cy) * dim + cz;
+    key[i] = b;
+    atomicAdd(cnt + b, 1);
+}
+
+__global__ void __launch_bounds__(256) hg_scatter(const int32_t* __restrict__ key, int32_t n,
+                                                   const int32_t* __restrict__ start, int32_t* __restrict__ cnt,
+                                                   int32_t* __restrict__ tmp)
+{
+    int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i >= n) return;
+    int32_t b = key[i];
+    int32_t left = atomicSub(cnt + b, 1);
+    tmp[start[b] + left - 1] = i;
+}
+
+__global__ void __launch_bounds__(256) hg_order(const int32_t* __restrict__ key, int32_t n,
+                                                 const int32_t* __restrict__ start,
+                                                 const int32_t* __restrict__ tmp, const float* __restrict__ position,
+                                                 const float* __restrict__ radius, int32_t* __restrict__ id,
+                                                 float4* __restrict__ xyzr)
+{
+    int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    int32_t me = tmp[t];
+    int32_t b = key[me];
+    int32_t s = start[b], e = start[b + 1];
+    int32_t rank = 0;
+    for (int32_t u = s; u < e; ++u) rank += tmp[u] < me ? 1 : 0;
+    id[s + rank] = me;
+    xyzr[s + rank] = make_float4(position[3 * me + 0], position[3 * me + 1], position[3 * me + 2], radius[me]);
+}
+
+__global__ void __launch_bounds__(256) hg_set_total(int32_t* __restrict__ start, int64_t nb, int32_t n)
+{
+    start[nb] = n;
+}
+
+// Gather of cell t (bucket order): neighbour count within query_radius and the repulsion
+// velocity; optionally x += v * dt into the slot-ordered position array (safe because the
+// gather reads only the bucket-ordered copy).
+template <bool kIntegrate>
+__global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict__ start,
+                                                       const int32_t* __restrict__ id,
+                                                       const float4* __restrict__ xyzr, int32_t n, float inv_cell,
+                                                       int32_t dim, float query_radius, float rate, float dt,
+                                                       float* __restrict__ velocity,
+                                                       int32_t* __restrict__ neighbor_count,
+                                                       float* __restrict__ position)
+{
+    int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int32_t i = id[t];
+    const float4 P = xyzr[t];
+    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
+    int32_t count = 0;
+    const int32_t x0 = cell_coord(P.x - query_radius, inv_cell), x1 = cell_coord(P.x + query_radius, inv_cell);
+    const int32_t y0 = cell_coord(P.y - query_radius, inv_cell), y1 = cell_coord(P.y + query_radius, inv_cell);
+    const int32_t z0 = cell_coord(P.z - query_radius, inv_cell), z1 = cell_coord(P.z + query_radius, inv_cell);
+    const int32_t wz0 = wrap_dim(z0, dim);
+    const bool z_contiguous = wz0 + (z1 - z0) < dim;  // the z buckets are consecutive keys
+    for (int32_t cx = x0; cx <= x1; ++cx) {
+        for (int32_t cy = y0; cy <= y1; ++cy) {
+            const int32_t row = (wrap_dim(cx, dim) * dim + wrap_dim(cy, dim)) * dim;
+            for (int32_t cz = z0; cz <= z1; ++cz) {
+                int32_t s, e;
+                if (z_contiguous) {
+                    s = start[row + wz0];
+                    e = start[row + wz0 + (z1 - z0) + 1];
+                    cz = z1;  // one contiguous run covers every z bucket of this row
+                } else {
+                    int32_t b = row + wrap_dim(cz, dim);
+                    s = start[b];
+                    e = start[b + 1];
+                }
+                for (int32_t u = s; u < e; ++u) {
+                    if (u == t) continue;
+                    const float4 Q = xyzr[u];
+                    float dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+                    float d = sqrtf(dx * dx + dy * dy + dz * dz);
+                    if (d < query_radius) {
+                        count += 1;
+                        float overlap = P.w + Q.w - d;
+                        if (overlap > 0.0f && d > kCoincidentEps) {
+                            float coef = rate * overlap / d;
+                            vx = vx + coef * dx;
+                            vy = vy + coef * dy;
+                            vz = vz + coef * dz;
+                        }
+                    }
+                }
+            }
+        }
+    }
+    velocity[3 * i + 0] = vx;
+    velocity[3 * i + 1] = vy;
+    velocity[3 * i + 2] = vz;
+    neighbor_count[i] = count;
+    if (kIntegrate) {
+        position[3 * i + 0] = P.x + vx * dt;
+        position[3 * i + 1] = P.y + vy * dt;
+        position[3 * i + 2] = P.z + vz * dt;
+    }
+}
+
+__global__ void __launch_bounds__(256) integrate_kernel(float* __restrict__ position,
+                                                         const float* __restrict__ velocity, int64_t m, float dt)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < m) position[t] = position[t] + velocity[t] * dt;
+}
+
+__global__ void __launch_bounds__(256) zero_i32_kernel_mech(int32_t* __restrict__ p, int64_t count)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < count) p[t] = 0;
+}
+
+static hipError_t hashgrid_build(const float* position, const float* radius, int32_t n, float inv_cell,
+                                 int32_t dim, char* ws, const HashGridLayout& L, hipStream_t s)
+{
+    const int64_t nb = (int64_t)dim * dim * dim;
+    int32_t* cnt = (int32_t*)(ws + L.cnt);
+    int32_t* start = (int32_t*)(ws + L.start);
+    int32_t* key = (int32_t*)(ws + L.key);
+    int32_t* tmp = (int32_t*)(ws + L.tmp);
+    hg_key_count<<<grid_for(n, 256), 256, 0, s>>>(position, n, inv_cell, dim, key, cnt);
+    WBC_TRY_ERR(hipGetLastError());
+    WBC_TRY_ERR(scan_i32(cnt, start, nb, (int32_t*)(ws + L.partial), true, s));
+    hg_set_total<<<1, 1, 0, s>>>(start, nb, n);
+    hg_scatter<<<grid_for(n, 256), 256, 0, s>>>(key, n, start, cnt, tmp);
+    hg_order<<<grid_for(n, 256), 256, 0, s>>>(key, n, start, tmp, position, radius, (int32_t*)(ws + L.id),
+                                              (float4*)(ws + L.xyzr));
+    return hipGetLastError();
+}
+
+}  // namespace wbc
+
+using namespace wbc;
+
+static bool hg_args_ok(int32_t dim, int64_t capacity)
+{
+    return dim >= 3 && dim <= 1024 && (int64_t)dim * dim * dim < ((int64_t)1 << 31) - 1 && capacity >= 0 &&
+           capacity <= INT32_MAX;
+}
+
+WBC_API size_t wbc_hashgrid_workspace_bytes(int32_t dim, int64_t capacity)
+{
+    if (!hg_args_ok(dim, capacity)) return 0;
+    return hashgrid_layout(dim, capacity).total;
+}
+
+// Zero the bucket counts once after allocation; every build leaves them at zero again.
+WBC_API int wbc_hashgrid_init(void* workspace, size_t workspace_bytes, int32_t dim, int64_t capacity,
+                              hipStream_t stream)
+{
+    if (!hg_args_ok(dim, capacity) || !workspace) return (int)hipErrorInvalidValue;
+    HashGridLayout L = hashgrid_layout(dim, capacity);
+    if (workspace_bytes < L.total) return (int)hipErrorInvalidValue;
+    int64_t nb = (int64_t)dim * dim * dim;
+    zero_i32_kernel_mech<<<grid_for(nb, 256), 256, 0, stream>>>((int32_t*)((char*)workspace + L.cnt), nb);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+WBC_API int wbc_hashgrid_build(const float* position, const float* radius, int32_t n, float cell_size, int32_t dim,
+                               int64_t capacity, void* workspace, size_t workspace_bytes, hipStream_t stream)
+{
+    if (!hg_args_ok(dim, capacity) || n < 0 || n > capacity || !(cell_size > 0.0f)) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!position || !radius || !workspace) return (int)hipErrorInvalidValue;
+    HashGridLayout L = hashgrid_layout(dim, capacity);
+    if (workspace_bytes < L.total) return (int)hipErrorInvalidValue;
+    return (int)hashgrid_build(position, radius, n, 1.0f / cell_size, dim, (char*)workspace, L, stream);
+}
+
+// Slot ids in bucket order after a build (the reference's hash_grid_point_id order).
+WBC_API int wbc_hashgrid_point_ids(const void* workspace, size_t workspace_bytes, int32_t dim, int64_t capacity,
+                                   const int32_t** point_id, const int32_t** bucket_start)
+{
+    if (!hg_args_ok(dim, capacity) || !workspace || !point_id || !bucket_start) return (int)hipErrorInvalidValue;
+    HashGridLayout L = hashgrid_layout(dim, capacity);
+    if (workspace_bytes < L.total) return (int)hipErrorInvalidValue;
+    *point_id = (const int32_t*)((const char*)workspace + L.id);
+    *bucket_start = (const int32_t*)((const char*)workspace + L.start);
+    return 0;
+}
+
+static int launch_contact(bool integrate, const void* workspace, size_t workspace_bytes, int32_t dim,
+                          int64_t capacity, int32_t n, float cell_size, float query_radius, float rate, float dt,
+                          float* velocity, int32_t* neighbor_count, float* position, hipStream_t stream)
+{
+    if (!hg_args_ok(dim, capacity) || n < 0 || n > capacity || !(cell_size > 0.0f)) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!workspace || !velocity || !neighbor_count || (integrate && !position)) return (int)hipErrorInvalidValue;
+    HashGridLayout L = hashgrid_layout(dim, capacity);
+    if (workspace_bytes < L.total) return (int)hipErrorInvalidValue;
+    const char* ws = (const char*)workspace;
+    const int32_t* start = (const int32_t*)(ws + L.start);
+    const int32_t* id = (const int32_t*)(ws + L.id);
+    const float4* xyzr = (const float4*)(ws + L.xyzr);
+    float inv_cell = 1.0f / cell_size;
+    if (integrate)
+        contact_kernel<true><<<grid_for(n, 256), 256, 0, stream>>>(start, id, xyzr, n, inv_cell, dim, query_radius,
+                                                                   rate, dt, velocity, neighbor_count, position);
+    else
+        contact_kernel<false><<<grid_for(n, 256), 256, 0, stream>>>(start, id, xyzr, n, inv_cell, dim, query_radius,
+                                                                    rate, dt, velocity, neighbor_count, position);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+WBC_API int wbc_contact_velocities(const void* workspace, size_t workspace_bytes, int32_t dim, int64_t capacity,
+                                   int32_t n, float cell_size, float query_radius, float rate, float* velocity,
+                                   int32_t* neighbor_count, hipStream_t stream)
+{
+    return launch_contact(false, workspace, workspace_bytes, dim, capacity, n, cell_size, query_radius, rate, 0.0f,
+                          velocity, neighbor_count, nullptr, stream);
+}
+
+WBC_API int wbc_integrate_positions(float* position, const float* velocity, int64_t n, float dt, hipStream_t stream)
+{
+    if (n < 0 || n > INT32_MAX) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!position || !velocity) return (int)hipErrorInvalidValue;
+    integrate_kernel<<<grid_for(3 * n, 256), 256, 0, stream>>>(position, velocity, 3 * n, dt);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+// `substeps` x (build -> contact gather -> x += v dt), cells/mechanics.py:142-189 without a region.
+WBC_API int wbc_relax_contacts(float* position, const float* radius, int32_t n, float cell_size, int32_t dim,
+                               float query_radius, float rate, float dt, int32_t substeps, float* velocity,
+                               int32_t* neighbor_count, int64_t capacity, void* workspace, size_t workspace_bytes,
+                               hipStream_t stream)
+{
+    if (substeps < 0) return (int)hipErrorInvalidValue;
+    for (int32_t s = 0; s < substeps; ++s) {
+        int rc = wbc_hashgrid_build(position, radius, n, cell_size, dim, capacity, workspace, workspace_bytes, stream);
+        if (rc) return rc;
+        rc = launch_contact(true, workspace, workspace_bytes, dim, capacity, n, cell_size, query_radius, rate, dt,
+                            velocity, neighbor_count, position, stream);
+        if (rc) return rc;
+    }
+    return 0;
+}

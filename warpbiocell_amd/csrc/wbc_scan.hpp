@@ -1,0 +1,17 @@
+// Device-wide int32 prefix sums shared by the division slot allocation and the hash grid.
+#pragma once
+#include "wbc_common.hpp"
+
+namespace wbc {
+
+constexpr int kScanBlock = 256;
+constexpr int kScanItems = 16;
+constexpr int kScanTile = kScanBlock * kScanItems;
+
+// Bytes of the tile-sum buffer a scan of n values needs (tiles + 1 ints, 256-byte aligned).
+size_t scan_workspace_bytes(int64_t n);
+
+// out = inclusive (or exclusive) prefix sum of in[0, n); partial[tiles] receives the total.
+hipError_t scan_i32(const int* in, int* out, int64_t n, int* partial, bool exclusive, hipStream_t stream);
+
+}  // namespace wbc
