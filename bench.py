@@ -7,9 +7,9 @@ decisions and slot allocation -> 4 contact substeps (hash-grid build, gather, in
 Inputs are synthetic (``warpbiocell_amd.workloads.spheroid_100k_161``) and resident on the GPU
 before the timed region. One JSON line goes to stdout; progress goes to stderr.
 
-Roofline: the SOR sweep kernel, timed live with HIP events on the stream it runs on, priced at
-its algorithmic bytes (16 B per node per red+black sweep: the field read by each colour pass,
-the updated colour written, the density read) against MI355X's 8 TB/s HBM peak.
+Roofline: the fused SOR sweep kernel, timed live with HIP events on the stream it runs on,
+priced at its algorithmic bytes (12 B per node per red+black sweep: field read, density read,
+field written) against MI355X's 8 TB/s HBM peak.
 
 CPU baseline: the oracle (C restatement of the reference, one thread) runs ONE step of the same
 workload from the exact state the GPU's first timed step started from.
@@ -58,28 +58,40 @@ def setup(workload_name: str, device):
 
 
 def sor_roofline(ox, geom, device, launches: int = 200) -> dict:
-    """Average duration of one SOR colour-pass launch, measured with HIP events on its stream."""
+    """Average duration of one fused SOR sweep launch, measured with HIP events on its stream.
+
+    Algorithmic bytes of one red+black sweep of the fused kernel: the field read once, the
+    density read once, the field written once, 12 B per node (two separate colour passes would
+    read the field twice: 16 B per node)."""
     import torch
 
-    from warpbiocell_amd.fields import diffusion as Dn
+    from warpbiocell_amd import _lib
+    from warpbiocell_amd.fields.diffusion import solver_scalars
 
     kin = ox.species.kinetics
+    dx2_over_D, _ = solver_scalars(ox.field, kin)
     stream = torch.cuda.current_stream(device)
+    bufs = [ox.field.values.clone(), torch.empty_like(ox.field.values)]
+
+    def sweep(c):
+        _lib.call("wbc_sor_sweep_fused", ox.settings.omega, dx2_over_D, kin.uptake_max, kin.michaelis_k, None,
+                  ox.density.data_ptr(), None, 0, None, 1.0, bufs[c & 1].data_ptr(), bufs[(c + 1) & 1].data_ptr(),
+                  *geom.shape, stream.cuda_stream)
+
     for c in range(4):
-        Dn.sor_sweep(ox.field, ox.density, kin, ox.settings.omega, c & 1)
+        sweep(c)
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
     for c in range(launches):
-        Dn.sor_sweep(ox.field, ox.density, kin, ox.settings.omega, c & 1)
+        sweep(c)
     end.record(stream)
     end.synchronize()
     per_launch_s = start.elapsed_time(end) / 1e3 / launches
-    V = geom.node_count
-    bytes_per_launch = 8 * V  # half of 16 B/node per red+black sweep
+    bytes_per_launch = 12 * geom.node_count
     achieved = bytes_per_launch / per_launch_s / 1e9
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-            "kernel": "sor_color_kernel", "launch_us": round(per_launch_s * 1e6, 3),
+            "kernel": "sor_fused_kernel", "launch_us": round(per_launch_s * 1e6, 3),
             "algorithmic_bytes_per_launch": bytes_per_launch}
 
 
@@ -123,6 +135,8 @@ def main() -> int:
     ap.add_argument("--workload", default="spheroid_100k_161")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--copy-calibration", action="store_true",
+                    help="only run a plain 256 MiB device copy 5 times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
         print(json.dumps({"error": "this build runs the hot path on one GPU; --gpus above 1 is not supported",
@@ -136,11 +150,23 @@ def main() -> int:
 
     _lib.load()
     device = torch.device("cuda", 0)
+    if args.copy_calibration:
+        src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device=device)
+        dst = torch.empty_like(src)
+        for _ in range(5):
+            dst.copy_(src)
+        torch.cuda.synchronize()
+        log("copy calibration: 5 x 256 MiB device-to-device copies")
+        return 0
     w, geom, params, settings, lifecycle, contact, stepping, pop, ox, grid = setup(args.workload, device)
     log(f"bench: {w.name}: {pop.count} cells, grid {geom.shape}, warmup {args.warmup}, steps {args.steps}")
     for s in range(args.warmup):
         r = cell_step(pop, grid, lifecycle, contact, stepping, oxygen=ox)
         log(f"  warmup {s}: sweeps {r.field_sweeps} residual {r.field_residual:.3e}")
+    from warpbiocell_amd.autotune import tune_sor
+
+    tune = tune_sor(ox.field, ox.density, ox.species.kinetics, settings.omega)
+    log(f"  autotune: tile {tune['shape']} of {tune['us_per_sweep']} (us per sweep)")
     torch.cuda.synchronize()
     snap = None if args.no_cpu_baseline else snapshot(pop, ox)
 
@@ -158,7 +184,7 @@ def main() -> int:
 
     roof = None if args.no_roofline else sor_roofline(ox, geom, device)
     if roof:
-        log(f"  sor colour pass: {roof['launch_us']} us, {roof['achieved']} GB/s algorithmic")
+        log(f"  sor fused sweep: {roof['launch_us']} us, {roof['achieved']} GB/s algorithmic")
     cpu = None
     if snap is not None:
         cpu = cpu_baseline(snap, w, geom, params, settings, lifecycle, contact, stepping)
@@ -184,6 +210,7 @@ def main() -> int:
                    "unconverged_steps": sum(not r.field_converged for r in reports),
                    "final_residual": reports[-1].field_residual},
         "roofline": roof,
+        "sor_autotune": tune,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

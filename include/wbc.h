@@ -92,6 +92,27 @@ int wbc_sor_sweep(int32_t color, float omega, float dx2_over_D, float uptake_max
                   int32_t has_source, const float* source, float source_k, float* field, int32_t nx, int32_t ny,
                   int32_t nz, wbc_stream_t stream);
 
+/* One full red+black sweep in one launch, src -> dst (two distinct buffers), on LDS tiles:
+ * bit-identical to the two in-place colour passes of wbc_sor_sweep (fields/diffusion.py:105-111)
+ * for grids whose faces are all Dirichlet (bc 0). */
+int wbc_sor_sweep_fused(float omega, float dx2_over_D, float uptake_max, float michaelis_k, const int32_t* fixed,
+                        const float* density, const float* production, int32_t has_source, const float* source,
+                        float source_k, const float* src, float* dst, int32_t nx, int32_t ny, int32_t nz,
+                        wbc_stream_t stream);
+
+/* Tile-shape table of the fused sweep (60 nodes along k by TJ x TI): the autotuner times every
+ * entry on the caller's grid and selects one; the solve loop uses the selected entry.
+ * wbc_sor_sweep_fused_cfg runs entry `cfg` (-1: the selected one). No reference counterpart:
+ * Warp picks launch shapes itself. */
+int wbc_sor_fused_config_count(void);
+int wbc_sor_fused_config_shape(int32_t cfg, int32_t* tk, int32_t* tj, int32_t* ti);
+int wbc_sor_fused_set_config(int32_t cfg);
+int wbc_sor_fused_get_config(void);
+int wbc_sor_sweep_fused_cfg(int32_t cfg, float omega, float dx2_over_D, float uptake_max, float michaelis_k,
+                            const int32_t* fixed, const float* density, const float* production, int32_t has_source,
+                            const float* source, float source_k, const float* src, float* dst, int32_t nx,
+                            int32_t ny, int32_t nz, wbc_stream_t stream);
+
 /* Replaces the steady_state_residual launch, fields/diffusion.py:114-119 (kernel
  * kernels/field_kernels.py:215-243): *residual = max(*residual, max over updated nodes). The
  * caller zeroes *residual first, as diffusion.py:113 does with residual_buffer.zero_(). */
@@ -104,8 +125,11 @@ int wbc_steady_state_residual(float dx2_over_D, float uptake_max, float michaeli
 /* Replaces solve_steady_state, fields/diffusion.py:79-123, whole loop: chunks of check_every
  * sweeps, a residual after each, stop below `tolerance` or at max_sweeps. host_residual is
  * PINNED host memory of at least ceil(max_sweeps / check_every) floats. Outputs mirror
- * SolveReport (diffusion.py:64-68). Synchronizes with the stream before returning. */
-size_t wbc_solve_workspace_bytes(int32_t max_sweeps, int32_t check_every);
+ * SolveReport (diffusion.py:64-68). All-Dirichlet grids run fused sweeps ping-ponging through
+ * a second field buffer in the workspace; the result always ends in `field`. The host reads
+ * chunk c's residual while chunk c+1 runs; a device-side latch turns sweeps queued after
+ * convergence into no-ops. Returns after the stream has produced the report. */
+size_t wbc_solve_workspace_bytes(int32_t max_sweeps, int32_t check_every, int32_t nx, int32_t ny, int32_t nz);
 int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolerance, int32_t check_every, float dx2_over_D,
                            float uptake_max, float michaelis_k, float reference, int32_t bc_x, int32_t bc_y,
                            int32_t bc_z, const int32_t* fixed, const float* density, const float* production,

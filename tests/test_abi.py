@@ -46,7 +46,7 @@ def test_workspace_queries_run_on_the_host():
     hg = _lib.query("wbc_hashgrid_workspace_bytes", 128, 100_000)
     assert hg >= 2 * 4 * 128 ** 3 + 28 * 100_000
     assert _lib.query("wbc_hashgrid_workspace_bytes", 2, 10) == 0  # rejected
-    assert _lib.query("wbc_solve_workspace_bytes", 2000, 10) >= 4 * 204
+    assert _lib.query("wbc_solve_workspace_bytes", 2000, 10, 41, 41, 41) >= 4 * (204 + 41 ** 3)
 
 
 def test_library_links_the_hip_runtime_by_soname():

@@ -155,3 +155,47 @@ def test_cosh_profile_on_the_gpu(device):
     x = geom.node_coordinates()[0]
     assert report.converged
     np.testing.assert_allclose(ox.numpy()[:, 1, 1], np.cosh(kappa * x) / np.cosh(kappa * L), atol=2e-3)
+
+
+@pytest.mark.parametrize("shape", [(19, 23, 37), (9, 9, 33), (17, 10, 66), (3, 3, 3), (41, 41, 41)])
+@pytest.mark.parametrize("pinned", [False, True])
+def test_fused_sweep_equals_two_colour_passes(oracle, device, shape, pinned):
+    """One fused launch per sweep (src -> dst) is bit-identical to the two in-place colour passes."""
+    from warpbiocell_amd import _lib
+
+    rng = np.random.default_rng(sum(shape) + pinned)
+    values = rng.uniform(0.0, 40.0, shape).astype(np.float32)
+    density = np.where(rng.random(shape) < 0.5, rng.uniform(0.0, 2e-3, shape), 0.0).astype(np.float32)
+    production = rng.uniform(0.0, 10.0, shape).astype(np.float32) if pinned else None
+    fixed = (rng.random(shape) < 0.05).astype(np.int32) if pinned else None
+    a = oracle.SolveArgs(dx=10.0, diffusion=2.0e3, uptake_max=5.0e4, michaelis_k=3.0, boundary_value=20.0)
+    bufs = [torch.from_numpy(values).to(device), torch.zeros(shape, dtype=torch.float32, device=device)]
+    d_t = torch.from_numpy(density).to(device)
+    p_t = None if production is None else torch.from_numpy(production).to(device)
+    f_t = None if fixed is None else torch.from_numpy(fixed).to(device)
+    ref = values.copy()
+    for sweep in range(5):
+        _lib.call("wbc_sor_sweep_fused", 1.8, a.dx2_over_D, a.uptake_max, a.michaelis_k, _lib.ptr(f_t), d_t.data_ptr(),
+                  _lib.ptr(p_t), 0, None, 1.0, bufs[sweep % 2].data_ptr(), bufs[(sweep + 1) % 2].data_ptr(), *shape,
+                  _lib.stream_of(device))
+        for color in (0, 1):
+            oracle.sor_sweep(ref, density, a, 1.8, color, fixed=fixed, production=production)
+        np.testing.assert_array_equal(bufs[(sweep + 1) % 2].cpu().numpy(), ref, err_msg=f"sweep {sweep}")
+
+
+def test_depleted_core_solve_is_bit_exact(oracle, device):
+    """Strong uptake drives the core to (near) zero: exercises the tiny-value division paths."""
+    geom = GridGeometry.centered_cube(300.0, 20.0)
+    params = OxygenParams(uptake_max=1.0e12)
+    a = oracle.SolveArgs(dx=20.0, diffusion=params.diffusion_coefficient, uptake_max=1.0e12,
+                         michaelis_k=params.michaelis_k, boundary_value=params.boundary_value)
+    ox = OxygenField(geom, params, SolverSettings(omega=1.9, tolerance=1e-5, check_every=10), device=device)
+    density = np.zeros(geom.shape, np.float32)
+    density[3:-3, 3:-3, 3:-3] = 1e-3
+    ox.density.copy_(torch.from_numpy(density))
+    ref = ox.numpy()
+    report = ox.solve_steady_state()
+    expected = oracle.solve(ref, density, a, 1.9, 2000, 1e-5, 10)
+    assert (report.sweeps, report.converged) == (expected[0], expected[2])
+    np.testing.assert_array_equal(ox.numpy(), ref)
+    assert ref.min() >= 0.0 and ref[8, 8, 8] < 1e-3

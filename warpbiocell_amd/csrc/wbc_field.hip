@@ -177,49 +177,6 @@ __global__ void __launch_bounds__(256) sor_color_kernel(SorParams p, int32_t col
     field[t] = fmaxf(v, 0.0f);
 }
 
-// Max-norm relative residual over updated nodes; one atomic per block on the float's bits
-// (every candidate is >= 0, so unsigned order equals float order). `out` must start at 0.
-__global__ void __launch_bounds__(256) residual_kernel(SorParams p, const float* __restrict__ field,
-                                                       float* __restrict__ out, const int32_t* __restrict__ done)
-{
-    __shared__ float wave_max_lds[4];
-    if (done && *done) return;
-    const int32_t i = blockIdx.z;
-    const int32_t j = blockIdx.y * blockDim.y + threadIdx.y;
-    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
-    float q = 0.0f;
-    if (j < p.ny && k < p.nz && !on_dirichlet_face(i, j, k, p)) {
-        const int64_t plane = (int64_t)p.ny * p.nz;
-        const int64_t t = i * plane + (int64_t)j * p.nz + k;
-        if (!(p.fixed && p.fixed[t] != 0)) {
-            float o = field[t];
-            float c = p.density[t] * p.uptake_max / (p.michaelis_k + o);
-            float prod = production_at(p, t);
-            float r = nb_sum(field, p.nx, p.ny, p.nz, i, j, k, t, plane) + p.dx2_over_D * prod -
-                      (6.0f + p.dx2_over_D * c) * o;
-            q = fabsf(r) / fmaxf(6.0f * o, p.reference);
-        }
-    }
-    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
-    q = wave_max(q);
-    if ((tid & (kWave - 1)) == 0) wave_max_lds[tid / kWave] = q;
-    __syncthreads();
-    if (tid == 0) {
-        float m = fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
-        if (m > 0.0f) atomicMax((unsigned int*)out, __float_as_uint(m));
-    }
-}
-
-// After a chunk: latch convergence on the device so later, speculatively queued sweeps no-op.
-__global__ void check_converged_kernel(const float* __restrict__ residual, int32_t chunk, float tolerance,
-                                       int32_t* __restrict__ state)
-{
-    if (state[0] == 0 && residual[chunk] < tolerance) {
-        state[0] = 1;
-        state[1] = chunk;
-    }
-}
-
 __global__ void __launch_bounds__(256) zero_i32_kernel(int32_t* __restrict__ p, int64_t count)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
@@ -231,14 +188,6 @@ hipError_t launch_sor_color(const SorParams& p, int32_t color, float* field, con
     dim3 block(64, 4, 1);
     dim3 grid(grid_for((p.nz + 1) / 2, 64), grid_for(p.ny, 4), p.nx);
     sor_color_kernel<<<grid, block, 0, s>>>(p, color, field, done);
-    return hipGetLastError();
-}
-
-hipError_t launch_residual(const SorParams& p, const float* field, float* out, const int32_t* done, hipStream_t s)
-{
-    dim3 block(64, 4, 1);
-    dim3 grid(grid_for(p.nz, 64), grid_for(p.ny, 4), p.nx);
-    residual_kernel<<<grid, block, 0, s>>>(p, field, out, done);
     return hipGetLastError();
 }
 
@@ -362,7 +311,7 @@ WBC_API int wbc_steady_state_residual(float dx2_over_D, float uptake_max, float 
                          production, has_source, source, source_k, nx, ny, nz);
     if (rc) return rc;
     if (!field || !residual) return (int)hipErrorInvalidValue;
-    return (int)launch_residual(p, field, residual, nullptr, stream);
+    return (int)launch_residual_atomic(p, field, residual, stream);
 }
 
 // ----------------------------------------------------------------------------------------------
@@ -394,11 +343,36 @@ hipError_t get_events(hipEvent_t** out)
 }
 }  // namespace
 
-WBC_API size_t wbc_solve_workspace_bytes(int32_t max_sweeps, int32_t check_every)
+namespace {
+struct SolveLayout {
+    size_t state, slots, partial, scratch, total;
+};
+
+SolveLayout solve_layout(int32_t chunks, int32_t partials, int64_t nodes)
 {
-    if (max_sweeps < 1 || check_every < 1) return 0;
-    int64_t chunks = (max_sweeps + check_every - 1) / check_every;
-    return (size_t)((4 + chunks) * 4 + 255) / 256 * 256;
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    SolveLayout L;
+    size_t off = 0;
+    L.state = off;   off += up(4 * sizeof(int32_t));
+    L.slots = off;   off += up((size_t)chunks * sizeof(float));
+    L.partial = off; off += up((size_t)partials * sizeof(float));
+    L.scratch = off; off += up((size_t)nodes * sizeof(float));
+    L.total = off;
+    return L;
+}
+}  // namespace
+
+// Device workspace of wbc_solve_steady_state: convergence latch, one residual per chunk, the
+// residual partials and a second field buffer for the fused (ping-pong) sweeps.
+WBC_API size_t wbc_solve_workspace_bytes(int32_t max_sweeps, int32_t check_every, int32_t nx, int32_t ny, int32_t nz)
+{
+    if (max_sweeps < 1 || check_every < 1 || !grid_ok(nx, ny, nz)) return 0;
+    SorParams p{};
+    p.nx = nx;
+    p.ny = ny;
+    p.nz = nz;
+    int32_t chunks = (max_sweeps + check_every - 1) / check_every;
+    return solve_layout(chunks, residual_partials(p), (int64_t)nx * ny * nz).total;
 }
 
 WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolerance, int32_t check_every,
@@ -416,30 +390,44 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
     if (rc) return rc;
     if (!field || !workspace || !host_residual || !sweeps_out || !residual_out || !converged_out)
         return (int)hipErrorInvalidValue;
-    size_t need = wbc_solve_workspace_bytes(max_sweeps, check_every);
+    size_t need = wbc_solve_workspace_bytes(max_sweeps, check_every, nx, ny, nz);
     if (need == 0 || workspace_bytes < need) return (int)hipErrorInvalidValue;
     const int32_t chunks = (max_sweeps + check_every - 1) / check_every;
-    int32_t* state = (int32_t*)workspace;      // [0] converged latch, [1] converged chunk
-    float* slots = (float*)workspace + 4;      // one residual per chunk
+    const int32_t partials = residual_partials(p);
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    const SolveLayout L = solve_layout(chunks, partials, nodes);
+    char* ws = (char*)workspace;
+    int32_t* state = (int32_t*)(ws + L.state);  // [0] converged latch, [1] converged chunk
+    float* slots = (float*)(ws + L.slots);      // one residual per chunk
+    float* partial = (float*)(ws + L.partial);
+    float* buf[2] = {field, (float*)(ws + L.scratch)};
+    const bool fused = fused_supported(p);
     hipEvent_t* ev;
     WBC_TRY(get_events(&ev));
-    zero_i32_kernel<<<grid_for(4 + chunks, 256), 256, 0, stream>>>(state, 4 + chunks);
+    zero_i32_kernel<<<1, 256, 0, stream>>>(state, 4);
     WBC_CHECK_LAUNCH();
 
-    int32_t queued = 0, checked = 0, sweeps = 0;
+    auto chunk_len = [&](int32_t c) {
+        return max_sweeps - c * check_every < check_every ? max_sweeps - c * check_every : check_every;
+    };
+    int32_t queued = 0, checked = 0, sweeps = 0, cur = 0;  // buf[cur] holds the field after the queued sweeps
     float residual = __builtin_huge_valf();
     bool converged = false;
+    int32_t result_buf = 0;
     while (true) {
         while (queued < chunks && queued - checked <= kLag) {
-            int32_t chunk = check_every;
-            if (max_sweeps - queued * check_every < chunk) chunk = max_sweeps - queued * check_every;
-            for (int32_t s = 0; s < chunk; ++s) {
-                WBC_TRY(launch_sor_color(p, 0, field, state, stream));
-                WBC_TRY(launch_sor_color(p, 1, field, state, stream));
+            const int32_t len = chunk_len(queued);
+            for (int32_t s = 0; s < len; ++s) {
+                if (fused) {
+                    WBC_TRY(launch_sor_fused(p, buf[cur], buf[cur ^ 1], state, stream));
+                    cur ^= 1;
+                } else {
+                    WBC_TRY(launch_sor_color(p, 0, field, state, stream));
+                    WBC_TRY(launch_sor_color(p, 1, field, state, stream));
+                }
             }
-            WBC_TRY(launch_residual(p, field, slots + queued, state, stream));
-            check_converged_kernel<<<1, 1, 0, stream>>>(slots, queued, tolerance, state);
-            WBC_CHECK_LAUNCH();
+            WBC_TRY(launch_residual_partial(p, buf[cur], partial, state, stream));
+            WBC_TRY(launch_residual_finish(partial, partials, slots, queued, tolerance, state, stream));
             WBC_TRY(hipMemcpyAsync(host_residual + queued, slots + queued, sizeof(float), hipMemcpyDeviceToHost,
                                    stream));
             WBC_TRY(hipEventRecord(ev[queued % kEventPool], stream));
@@ -448,15 +436,17 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
         if (checked == queued) break;
         WBC_TRY(hipEventSynchronize(ev[checked % kEventPool]));
         residual = host_residual[checked];
-        int32_t chunk = check_every;
-        if (max_sweeps - checked * check_every < chunk) chunk = max_sweeps - checked * check_every;
-        sweeps += chunk;
+        sweeps += chunk_len(checked);
         ++checked;
         if (residual < tolerance) {
             converged = true;
             break;
         }
     }
+    // The field after the last executed chunk is in buf[sweeps % 2] (fused path); chunks queued
+    // after convergence did nothing.
+    if (fused) result_buf = sweeps & 1;
+    if (result_buf != 0) WBC_TRY(launch_copy_f32(buf[1], field, nodes, stream));
     *sweeps_out = sweeps;
     *residual_out = residual;
     *converged_out = converged ? 1 : 0;

@@ -1,0 +1,425 @@
+// One full red-black SOR sweep per launch: both colour passes fused on LDS tiles.
+//
+// Equivalent, bit for bit, to the reference's two in-place colour launches per sweep
+// (fields/diffusion.py:105-111, kernel kernels/field_kernels.py:185-212): a red node reads only
+// black neighbours (all still at their pre-sweep values), a black node reads only red
+// neighbours (all already updated). Each workgroup loads its tile of the PRE-sweep field with a
+// halo of 2 into LDS, updates the red nodes of the tile plus a 1-node ring in place in LDS, then
+// the black nodes of the tile, and writes the tile's final values to a second buffer. Reading
+// `src` and writing `dst` (ping-pong) keeps every load at pre-sweep values, so neighbouring
+// tiles never race; the ring's red values are computed redundantly by both neighbours, from the
+// same operands, so they agree exactly.
+//
+// HBM traffic per sweep is one read of the field, one of the density and one write of the
+// field (12 B per node) against 24 B per node for two strided colour passes; the halo re-reads
+// are served by L2 because consecutive workgroups of one XCD get neighbouring tiles.
+//
+// Scope: Dirichlet faces (every BASELINE configuration); grids with a Neumann face use the
+// colour-pass kernels, whose mirror rule needs the ring mirrored after the red pass.
+#include "wbc_field.hpp"
+
+namespace wbc {
+
+// Tile: 60 nodes along k (64 with the halo of 2, one wave per LDS row), TJ x TI along j and i.
+constexpr int kTK = 60, kLK = 64;
+
+template <int TJ, int TI>
+struct SorTile {
+    static constexpr int LJ = TJ + 4, LI = TI + 4;
+    static constexpr int kRows = LJ * LI;                 // LDS rows of 64 floats
+    static constexpr int kRingRows = (TJ + 2) * (TI + 2);  // red pass: tile + ring 1
+    static constexpr int kRows2 = TJ * TI;                 // black pass: tile
+};
+
+// Workgroup -> tile with neighbouring tiles on the same XCD (workgroups are dispatched to the 8
+// XCDs round-robin by id): XCD x gets a contiguous range of tile ids.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int q = nb / 8, r = nb % 8;
+    const int xcd = b % 8, slot = b / 8;
+    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
+__device__ __forceinline__ bool updated_node(const SorParams& p, int i, int j, int k, int64_t t)
+{
+    if (i <= 0 || j <= 0 || k <= 0 || i >= p.nx - 1 || j >= p.ny - 1 || k >= p.nz - 1) return false;
+    return !(p.fixed && p.fixed[t] != 0);
+}
+
+// x / 6 correctly rounded: q = RN(x * RN(1/6)) corrected once with the exact remainder
+// (Brisebarre, Muller & Raina, "Accelerating correctly rounded floating-point division when the
+// divisor is known in advance", IEEE Trans. Computers 53(8), 2004). Checked against IEEE
+// division for every non-negative finite binary32 x: identical for all |x| >= 2^-120, so
+// smaller, infinite and NaN inputs take the division.
+__device__ __forceinline__ float div6(float x)
+{
+    const float ax = fabsf(x);
+    if (!(ax >= 0x1p-120f && ax <= 3.0e38f)) return x / 6.0f;
+    const float r6 = 1.0f / 6.0f;
+    const float q0 = x * r6;
+    const float r = fmaf(-q0, 6.0f, x);
+    return fmaf(r, r6, q0);
+}
+
+// One node update from LDS neighbours (e: LDS index; row strides kLK and kLK*LJ); `dens` is the
+// node's density, loaded ahead by the caller.
+template <int LJ>
+__device__ __forceinline__ float sor_update(const SorParams& p, const float* __restrict__ s, int e, int64_t t,
+                                            float dens, bool q_finite)
+{
+    constexpr int SJ = kLK, SI = kLK * LJ;
+    const float o = s[e];
+    const float nb = s[e - SI] + s[e + SI] + s[e - SJ] + s[e + SJ] + s[e - 1] + s[e + 1];
+    float x = nb;
+    if (p.production) {
+        float sf = 1.0f;
+        if (p.has_source) {
+            float v = p.source[t];
+            sf = v / (p.source_k + v);
+        }
+        x = nb + p.dx2_over_D * (p.production[t] * sf);
+    }
+    const float kpo = p.michaelis_k + o;
+    float gs;
+    // density == 0 makes c == +-0 whenever K + o > 0 and uptake_max is finite, and 6 + h*c is then
+    // 6 exactly: the quotient is x / 6.
+    if (dens == 0.0f && kpo > 0.0f && q_finite) {
+        gs = div6(x);
+    } else {
+        const float c = dens * p.uptake_max / kpo;
+        gs = x / (6.0f + p.dx2_over_D * c);
+    }
+    const float v = (1.0f - p.omega) * o + p.omega * gs;
+    return fmaxf(v, 0.0f);
+}
+
+template <int TJ, int TI>
+__global__ void __launch_bounds__(256) sor_fused_kernel(SorParams p, const float* __restrict__ src,
+                                                        float* __restrict__ dst, const int32_t* __restrict__ done,
+                                                        int tiles_k, int tiles_j, int q_finite)
+{
+    using T = SorTile<TJ, TI>;
+    constexpr int kLoadIters = (T::kRows + 3) / 4;      // LDS rows per wave
+    constexpr int kRedIters = (T::kRingRows + 7) / 8;   // half-wave rows, red pass
+    constexpr int kBlackIters = (T::kRows2 + 7) / 8;    // half-wave rows, black pass
+    __shared__ float s[T::kRows * kLK];
+    if (done && *done) return;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int tk = tile % tiles_k;
+    const int tj = (tile / tiles_k) % tiles_j;
+    const int ti = tile / (tiles_k * tiles_j);
+    const int k0 = tk * kTK, j0 = tj * TJ, i0 = ti * TI;
+    const int64_t plane = (int64_t)p.ny * p.nz;
+    const int lane = threadIdx.x & (kWave - 1);
+    const int wave = threadIdx.x / kWave;
+    const int half = lane >> 5, q = lane & 31;
+
+    // Issue every global load of the workgroup first (field rows with a halo of 2, then the
+    // densities of this lane's red and black nodes), consume afterwards: the loads overlap.
+    float v[kLoadIters];
+    {
+        const int gk = k0 - 2 + lane;
+        const bool k_in = gk >= 0 && gk < p.nz;
+#pragma unroll
+        for (int m = 0; m < kLoadIters; ++m) {
+            const int row = wave + 4 * m;
+            const int gj = j0 - 2 + row % T::LJ, gi = i0 - 2 + row / T::LJ;
+            v[m] = 0.0f;
+            if (row < T::kRows && k_in && gi >= 0 && gi < p.nx && gj >= 0 && gj < p.ny)
+                v[m] = src[gi * plane + (int64_t)gj * p.nz + gk];
+        }
+    }
+    float dred[kRedIters], dblack[kBlackIters];
+#pragma unroll
+    for (int m = 0; m < kRedIters; ++m) {
+        const int rr = wave * 2 + half + 8 * m;
+        const int gi = i0 - 1 + rr / (TJ + 2), gj = j0 - 1 + rr % (TJ + 2);
+        const int gk = k0 - 1 + 2 * q + ((gi + gj + k0 - 1) & 1);
+        dred[m] = 0.0f;
+        if (rr < T::kRingRows && q < (kTK + 2) / 2 && gi > 0 && gj > 0 && gk > 0 && gi < p.nx - 1 &&
+            gj < p.ny - 1 && gk < p.nz - 1)
+            dred[m] = p.density[gi * plane + (int64_t)gj * p.nz + gk];
+    }
+#pragma unroll
+    for (int m = 0; m < kBlackIters; ++m) {
+        const int rr = wave * 2 + half + 8 * m;
+        const int gi = i0 + rr / TJ, gj = j0 + rr % TJ, gk0 = k0 + 2 * q;
+        const int gk = gk0 + (((gi + gj + gk0) & 1) ? 0 : 1);
+        dblack[m] = 0.0f;
+        if (rr < T::kRows2 && q < kTK / 2 && gi > 0 && gj > 0 && gk > 0 && gi < p.nx - 1 && gj < p.ny - 1 &&
+            gk < p.nz - 1)
+            dblack[m] = p.density[gi * plane + (int64_t)gj * p.nz + gk];
+    }
+#pragma unroll
+    for (int m = 0; m < kLoadIters; ++m) {
+        const int row = wave + 4 * m;
+        if (row < T::kRows) s[row * kLK + lane] = v[m];
+    }
+    __syncthreads();
+
+    // Red pass ((i+j+k) even) over the tile and its 1-node ring, in place in LDS. Each half-wave
+    // takes one row of 31 pairs (k0-1 .. k0+60).
+#pragma unroll
+    for (int m = 0; m < kRedIters; ++m) {
+        const int rr = wave * 2 + half + 8 * m;
+        if (rr >= T::kRingRows || q >= (kTK + 2) / 2) continue;
+        const int jj = rr % (TJ + 2), ii = rr / (TJ + 2);
+        const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
+        const int gk = k0 - 1 + 2 * q + ((gi + gj + k0 - 1) & 1);
+        const int64_t t = gi * plane + (int64_t)gj * p.nz + gk;
+        if (!updated_node(p, gi, gj, gk, t)) continue;
+        const int e = ((ii + 1) * T::LJ + (jj + 1)) * kLK + (gk - k0 + 2);
+        s[e] = sor_update<T::LJ>(p, s, e, t, dred[m], q_finite != 0);
+    }
+    __syncthreads();
+
+    // Black pass over the tile (rows of 30 pairs); write both nodes of each pair.
+#pragma unroll
+    for (int m = 0; m < kBlackIters; ++m) {
+        const int rr = wave * 2 + half + 8 * m;
+        if (rr >= T::kRows2 || q >= kTK / 2) continue;
+        const int jj = rr % TJ, ii = rr / TJ;
+        const int gi = i0 + ii, gj = j0 + jj, gk0 = k0 + 2 * q;
+        if (gi >= p.nx || gj >= p.ny || gk0 >= p.nz) continue;
+        const int base = ((ii + 2) * T::LJ + (jj + 2)) * kLK + (2 * q + 2);
+        const int64_t t0 = gi * plane + (int64_t)gj * p.nz + gk0;
+        const int black = ((gi + gj + gk0) & 1) ? 0 : 1;  // which of the two nodes is black
+        float out0 = s[base], out1 = s[base + 1];
+        const int gkb = gk0 + black;
+        if (gkb < p.nz && updated_node(p, gi, gj, gkb, t0 + black)) {
+            const float u = sor_update<T::LJ>(p, s, base + black, t0 + black, dblack[m], q_finite != 0);
+            if (black) out1 = u; else out0 = u;
+        }
+        const bool two = gk0 + 1 < p.nz;
+        if (two && (t0 & 1) == 0) {
+            *reinterpret_cast<float2*>(dst + t0) = make_float2(out0, out1);
+        } else {
+            dst[t0] = out0;
+            if (two) dst[t0 + 1] = out1;
+        }
+    }
+}
+
+// Residual with per-block maxima: block (64 x 4) covers a k-row segment of 4 rows and strides
+// over i. kAtomic = false stores one partial per block (reduced by residual_finish_kernel, no
+// same-address atomics); kAtomic = true folds the block maximum into out[0] with an unsigned
+// atomic max (every candidate is >= 0, so unsigned order is float order).
+template <bool kAtomic>
+__global__ void __launch_bounds__(256) residual_partial_kernel(SorParams p, const float* __restrict__ field,
+                                                               float* __restrict__ partial,
+                                                               const int32_t* __restrict__ done, int q_finite)
+{
+    __shared__ float wave_max_lds[4];
+    if (done && *done) return;
+    const int32_t j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int32_t k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t plane = (int64_t)p.ny * p.nz;
+    float m = 0.0f;
+    if (j < p.ny && k < p.nz) {
+        for (int32_t i = blockIdx.z; i < p.nx; i += gridDim.z) {
+            if (p.bc_x == 0 && (i == 0 || i == p.nx - 1)) continue;
+            if (p.bc_y == 0 && (j == 0 || j == p.ny - 1)) continue;
+            if (p.bc_z == 0 && (k == 0 || k == p.nz - 1)) continue;
+            const int64_t t = i * plane + (int64_t)j * p.nz + k;
+            if (p.fixed && p.fixed[t] != 0) continue;
+            const float o = field[t];
+            const float dens = p.density[t];
+            const float kpo = p.michaelis_k + o;
+            float c = 0.0f;
+            if (!(dens == 0.0f && kpo > 0.0f && q_finite != 0)) c = dens * p.uptake_max / kpo;
+            float prod = 0.0f;
+            if (p.production) {
+                float sf = 1.0f;
+                if (p.has_source) {
+                    float v = p.source[t];
+                    sf = v / (p.source_k + v);
+                }
+                prod = p.production[t] * sf;
+            }
+            const int64_t im = i > 0 ? t - plane : t + plane, ip = i < p.nx - 1 ? t + plane : t - plane;
+            const int64_t jm = j > 0 ? t - p.nz : t + p.nz, jp = j < p.ny - 1 ? t + p.nz : t - p.nz;
+            const int64_t km = k > 0 ? t - 1 : t + 1, kp = k < p.nz - 1 ? t + 1 : t - 1;
+            const float nb = field[im] + field[ip] + field[jm] + field[jp] + field[km] + field[kp];
+            const float r = nb + p.dx2_over_D * prod - (6.0f + p.dx2_over_D * c) * o;
+            m = fmaxf(m, fabsf(r) / fmaxf(6.0f * o, p.reference));
+        }
+    }
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    m = wave_max(m);
+    if ((tid & (kWave - 1)) == 0) wave_max_lds[tid / kWave] = m;
+    __syncthreads();
+    if (tid == 0) {
+        const float bm = fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
+        if (kAtomic) {
+            if (bm > 0.0f) atomicMax((unsigned int*)partial, __float_as_uint(bm));
+        } else {
+            partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] = bm;
+        }
+    }
+}
+
+// Reduce the partials into residual[chunk] and latch convergence (one block).
+__global__ void __launch_bounds__(256) residual_finish_kernel(const float* __restrict__ partial, int count,
+                                                              float* __restrict__ residual, int32_t chunk,
+                                                              float tolerance, int32_t* __restrict__ state)
+{
+    __shared__ float wave_max_lds[4];
+    if (state[0] != 0) return;
+    float m = 0.0f;
+    for (int b = threadIdx.x; b < count; b += 256) m = fmaxf(m, partial[b]);
+    m = wave_max(m);
+    if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
+        residual[chunk] = m;
+        if (m < tolerance) {
+            state[0] = 1;
+            state[1] = chunk;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(256) copy_f32_kernel(const float* __restrict__ a, float* __restrict__ b, int64_t n)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < n) b[t] = a[t];
+}
+
+// ---- launch helpers ---------------------------------------------------------------------------
+// Tile shapes (TJ x TI; 60 along k) the autotuner chooses from.
+struct FusedConfig {
+    int tj, ti;
+};
+constexpr FusedConfig kFusedConfigs[] = {{4, 4}, {4, 8}, {8, 4}, {8, 8}, {16, 4}, {4, 16}, {2, 16}, {16, 2}};
+constexpr int kFusedConfigCount = sizeof(kFusedConfigs) / sizeof(kFusedConfigs[0]);
+int g_fused_config = 3;
+constexpr int kResidualIStride = 8;
+
+template <int TJ, int TI>
+static hipError_t launch_fused_tile(const SorParams& p, const float* src, float* dst, const int32_t* done, int qf,
+                                    hipStream_t s)
+{
+    const int tk = (p.nz + kTK - 1) / kTK, tj = (p.ny + TJ - 1) / TJ, ti = (p.nx + TI - 1) / TI;
+    sor_fused_kernel<TJ, TI><<<tk * tj * ti, 256, 0, s>>>(p, src, dst, done, tk, tj, qf);
+    return hipGetLastError();
+}
+
+dim3 residual_grid(const SorParams& p)
+{
+    return dim3(grid_for(p.nz, 64), grid_for(p.ny, 4), (unsigned)(p.nx < kResidualIStride ? p.nx : kResidualIStride));
+}
+
+int residual_partials(const SorParams& p)
+{
+    dim3 g = residual_grid(p);
+    return (int)(g.x * g.y * g.z);
+}
+
+static int q_finite_flag(const SorParams& p) { return __builtin_isfinite(p.uptake_max) ? 1 : 0; }
+
+hipError_t launch_sor_fused_cfg(int cfg, const SorParams& p, const float* src, float* dst, const int32_t* done,
+                                hipStream_t s)
+{
+    const int qf = q_finite_flag(p);
+    switch (cfg) {
+    case 0: return launch_fused_tile<4, 4>(p, src, dst, done, qf, s);
+    case 1: return launch_fused_tile<4, 8>(p, src, dst, done, qf, s);
+    case 2: return launch_fused_tile<8, 4>(p, src, dst, done, qf, s);
+    case 3: return launch_fused_tile<8, 8>(p, src, dst, done, qf, s);
+    case 4: return launch_fused_tile<16, 4>(p, src, dst, done, qf, s);
+    case 5: return launch_fused_tile<4, 16>(p, src, dst, done, qf, s);
+    case 6: return launch_fused_tile<2, 16>(p, src, dst, done, qf, s);
+    case 7: return launch_fused_tile<16, 2>(p, src, dst, done, qf, s);
+    default: return hipErrorInvalidValue;
+    }
+}
+
+hipError_t launch_sor_fused(const SorParams& p, const float* src, float* dst, const int32_t* done, hipStream_t s)
+{
+    return launch_sor_fused_cfg(g_fused_config, p, src, dst, done, s);
+}
+
+hipError_t launch_residual_partial(const SorParams& p, const float* field, float* partial, const int32_t* done,
+                                   hipStream_t s)
+{
+    residual_partial_kernel<false><<<residual_grid(p), dim3(64, 4, 1), 0, s>>>(p, field, partial, done,
+                                                                               q_finite_flag(p));
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_atomic(const SorParams& p, const float* field, float* out, hipStream_t s)
+{
+    residual_partial_kernel<true><<<residual_grid(p), dim3(64, 4, 1), 0, s>>>(p, field, out, nullptr,
+                                                                              q_finite_flag(p));
+    return hipGetLastError();
+}
+
+hipError_t launch_residual_finish(const float* partial, int count, float* residual, int32_t chunk, float tol,
+                                  int32_t* state, hipStream_t s)
+{
+    residual_finish_kernel<<<1, 256, 0, s>>>(partial, count, residual, chunk, tol, state);
+    return hipGetLastError();
+}
+
+hipError_t launch_copy_f32(const float* a, float* b, int64_t n, hipStream_t s)
+{
+    copy_f32_kernel<<<grid_for(n, 256), 256, 0, s>>>(a, b, n);
+    return hipGetLastError();
+}
+
+bool fused_supported(const SorParams& p) { return p.bc_x == 0 && p.bc_y == 0 && p.bc_z == 0; }
+
+}  // namespace wbc
+
+using namespace wbc;
+
+// Tile-shape table of the fused sweep: count, shape of entry i, and the entry the solver uses.
+WBC_API int wbc_sor_fused_config_count(void) { return kFusedConfigCount; }
+
+WBC_API int wbc_sor_fused_config_shape(int32_t cfg, int32_t* tk, int32_t* tj, int32_t* ti)
+{
+    if (cfg < 0 || cfg >= kFusedConfigCount || !tk || !tj || !ti) return (int)hipErrorInvalidValue;
+    *tk = kTK;
+    *tj = kFusedConfigs[cfg].tj;
+    *ti = kFusedConfigs[cfg].ti;
+    return 0;
+}
+
+WBC_API int wbc_sor_fused_set_config(int32_t cfg)
+{
+    if (cfg < 0 || cfg >= kFusedConfigCount) return (int)hipErrorInvalidValue;
+    g_fused_config = cfg;
+    return 0;
+}
+
+WBC_API int wbc_sor_fused_get_config(void) { return g_fused_config; }
+
+// One fused red+black sweep with tile shape `cfg` (-1: the selected one), src -> dst.
+WBC_API int wbc_sor_sweep_fused_cfg(int32_t cfg, float omega, float dx2_over_D, float uptake_max, float michaelis_k,
+                                    const int32_t* fixed, const float* density, const float* production,
+                                    int32_t has_source, const float* source, float source_k, const float* src,
+                                    float* dst, int32_t nx, int32_t ny, int32_t nz, hipStream_t stream)
+{
+    if (nx < 3 || ny < 3 || nz < 3 || (int64_t)nx * ny * nz >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;
+    if (!density || !src || !dst || src == dst) return (int)hipErrorInvalidValue;
+    if (has_source && (!source || !production)) return (int)hipErrorInvalidValue;
+    if (cfg < -1 || cfg >= kFusedConfigCount) return (int)hipErrorInvalidValue;
+    SorParams p{omega, dx2_over_D, uptake_max, michaelis_k, 1.0f, 0, 0, 0, fixed, density, production,
+                has_source, source, source_k, nx, ny, nz};
+    return (int)launch_sor_fused_cfg(cfg < 0 ? g_fused_config : cfg, p, src, dst, nullptr, stream);
+}
+
+// One fused red+black sweep, src -> dst (distinct buffers, Dirichlet faces only).
+WBC_API int wbc_sor_sweep_fused(float omega, float dx2_over_D, float uptake_max, float michaelis_k,
+                                const int32_t* fixed, const float* density, const float* production,
+                                int32_t has_source, const float* source, float source_k, const float* src,
+                                float* dst, int32_t nx, int32_t ny, int32_t nz, hipStream_t stream)
+{
+    if (nx < 3 || ny < 3 || nz < 3 || (int64_t)nx * ny * nz >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;
+    if (!density || !src || !dst || src == dst) return (int)hipErrorInvalidValue;
+    if (has_source && (!source || !production)) return (int)hipErrorInvalidValue;
+    SorParams p{omega, dx2_over_D, uptake_max, michaelis_k, 1.0f, 0, 0, 0, fixed, density, production,
+                has_source, source, source_k, nx, ny, nz};
+    return (int)launch_sor_fused(p, src, dst, nullptr, stream);
+}

@@ -115,7 +115,7 @@ class ScalarField:
         key = ("solve", max_sweeps, check_every)
         ws = self._scratch.get(key)
         if ws is None:
-            nbytes = _lib.query("wbc_solve_workspace_bytes", int(max_sweeps), int(check_every))
+            nbytes = _lib.query("wbc_solve_workspace_bytes", int(max_sweeps), int(check_every), *self.geometry.shape)
             chunks = -(-int(max_sweeps) // int(check_every))
             ws = (torch.empty(nbytes, dtype=torch.uint8, device=self.device),
                   torch.zeros(chunks, dtype=torch.float32, pin_memory=True))
