@@ -100,12 +100,13 @@ int wbc_sor_sweep_fused(float omega, float dx2_over_D, float uptake_max, float m
                         float source_k, const float* src, float* dst, int32_t nx, int32_t ny, int32_t nz,
                         wbc_stream_t stream);
 
-/* Tile-shape table of the fused sweep (60 nodes along k by TJ x TI): the autotuner times every
+/* Shape table of the fused sweep, entries (kind, TJ, TI): kind 0 is a 3-D tile of 60 x TJ x TI
+ * nodes (k, j, i); kind 1 a 60 x TJ tile marching TI planes along i. The autotuner times every
  * entry on the caller's grid and selects one; the solve loop uses the selected entry.
  * wbc_sor_sweep_fused_cfg runs entry `cfg` (-1: the selected one). No reference counterpart:
  * Warp picks launch shapes itself. */
 int wbc_sor_fused_config_count(void);
-int wbc_sor_fused_config_shape(int32_t cfg, int32_t* tk, int32_t* tj, int32_t* ti);
+int wbc_sor_fused_config_shape(int32_t cfg, int32_t* kind, int32_t* tj, int32_t* ti);
 int wbc_sor_fused_set_config(int32_t cfg);
 int wbc_sor_fused_get_config(void);
 int wbc_sor_sweep_fused_cfg(int32_t cfg, float omega, float dx2_over_D, float uptake_max, float michaelis_k,

@@ -157,7 +157,7 @@ def test_cosh_profile_on_the_gpu(device):
     np.testing.assert_allclose(ox.numpy()[:, 1, 1], np.cosh(kappa * x) / np.cosh(kappa * L), atol=2e-3)
 
 
-@pytest.mark.parametrize("shape", [(19, 23, 37), (9, 9, 33), (17, 10, 66), (3, 3, 3), (41, 41, 41)])
+@pytest.mark.parametrize("shape", [(19, 23, 37), (9, 9, 33), (17, 10, 66), (3, 3, 3), (41, 41, 41), (35, 30, 130)])
 @pytest.mark.parametrize("pinned", [False, True])
 def test_fused_sweep_equals_two_colour_passes(oracle, device, shape, pinned):
     """One fused launch per sweep (src -> dst) is bit-identical to the two in-place colour passes."""
@@ -173,14 +173,20 @@ def test_fused_sweep_equals_two_colour_passes(oracle, device, shape, pinned):
     d_t = torch.from_numpy(density).to(device)
     p_t = None if production is None else torch.from_numpy(production).to(device)
     f_t = None if fixed is None else torch.from_numpy(fixed).to(device)
-    ref = values.copy()
-    for sweep in range(5):
-        _lib.call("wbc_sor_sweep_fused", 1.8, a.dx2_over_D, a.uptake_max, a.michaelis_k, _lib.ptr(f_t), d_t.data_ptr(),
-                  _lib.ptr(p_t), 0, None, 1.0, bufs[sweep % 2].data_ptr(), bufs[(sweep + 1) % 2].data_ptr(), *shape,
-                  _lib.stream_of(device))
+    refs = [values.copy()]
+    for sweep in range(4):
+        r = refs[-1].copy()
         for color in (0, 1):
-            oracle.sor_sweep(ref, density, a, 1.8, color, fixed=fixed, production=production)
-        np.testing.assert_array_equal(bufs[(sweep + 1) % 2].cpu().numpy(), ref, err_msg=f"sweep {sweep}")
+            oracle.sor_sweep(r, density, a, 1.8, color, fixed=fixed, production=production)
+        refs.append(r)
+    for cfg in range(_lib.query("wbc_sor_fused_config_count")):
+        bufs[0].copy_(torch.from_numpy(values))
+        for sweep in range(4):
+            _lib.call("wbc_sor_sweep_fused_cfg", cfg, 1.8, a.dx2_over_D, a.uptake_max, a.michaelis_k, _lib.ptr(f_t),
+                      d_t.data_ptr(), _lib.ptr(p_t), 0, None, 1.0, bufs[sweep % 2].data_ptr(),
+                      bufs[(sweep + 1) % 2].data_ptr(), *shape, _lib.stream_of(device))
+            np.testing.assert_array_equal(bufs[(sweep + 1) % 2].cpu().numpy(), refs[sweep + 1],
+                                          err_msg=f"config {cfg}, sweep {sweep}")
 
 
 def test_depleted_core_solve_is_bit_exact(oracle, device):

@@ -25,7 +25,7 @@ from pathlib import Path
 PACKAGE_DIR = Path(__file__).resolve().parent
 CSRC = PACKAGE_DIR / "csrc"
 LIBRARY = PACKAGE_DIR / "libwbc_hip.so"
-SOURCES = ("wbc_scan.hip", "wbc_cells.hip", "wbc_field.hip", "wbc_sor_fused.hip", "wbc_mechanics.hip",
+SOURCES = ("wbc_scan.hip", "wbc_cells.hip", "wbc_field.hip", "wbc_sor_fused.hip", "wbc_sor_march.hip", "wbc_mechanics.hip",
            "wbc_batched.hip", "wbc_adjoint.hip")
 ARCH = "gfx950"
 

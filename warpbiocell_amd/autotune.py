@@ -15,12 +15,13 @@ from warpbiocell_amd import _lib
 _cache: dict = {}
 
 
-def config_shapes() -> list[tuple[int, int, int]]:
+def config_shapes() -> list[str]:
+    """Names of the shape table: 'tile60xTJxTI' (3-D tile) or 'march60xTJ/CI' (marching i)."""
     out = []
     for c in range(_lib.query("wbc_sor_fused_config_count")):
-        tk, tj, ti = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
-        _lib.call("wbc_sor_fused_config_shape", c, ctypes.byref(tk), ctypes.byref(tj), ctypes.byref(ti))
-        out.append((tk.value, tj.value, ti.value))
+        kind, tj, ti = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _lib.call("wbc_sor_fused_config_shape", c, ctypes.byref(kind), ctypes.byref(tj), ctypes.byref(ti))
+        out.append(f"tile60x{tj.value}x{ti.value}" if kind.value == 0 else f"march60x{tj.value}/{ti.value}")
     return out
 
 
@@ -57,6 +58,6 @@ def tune_sor(field, density: torch.Tensor, kinetics, omega: float, sweeps: int =
         times = [time_config(c, field, density, kinetics, omega, sweeps) for c in range(len(shapes))]
         best = min(range(len(shapes)), key=lambda c: times[c])
         _cache[key] = {"choice": best, "shape": shapes[best],
-                       "us_per_sweep": {f"{s[0]}x{s[1]}x{s[2]}": round(t, 3) for s, t in zip(shapes, times)}}
+                       "us_per_sweep": {s: round(t, 3) for s, t in zip(shapes, times)}}
     _lib.call("wbc_sor_fused_set_config", _cache[key]["choice"])
     return _cache[key]

@@ -17,6 +17,30 @@ struct SorParams {
     int32_t nx, ny, nz;
 };
 
+// x / 6 correctly rounded: q = RN(x * RN(1/6)) corrected once with the exact remainder
+// (Brisebarre, Muller & Raina, "Accelerating correctly rounded floating-point division when the
+// divisor is known in advance", IEEE Trans. Computers 53(8), 2004). Checked against IEEE
+// division for every non-negative finite binary32 x: identical for all |x| >= 2^-120 (and by
+// symmetry for negative x), so smaller, infinite and NaN inputs take the division.
+__device__ __forceinline__ float div6(float x)
+{
+    const float ax = fabsf(x);
+    if (!(ax >= 0x1p-120f && ax <= 3.0e38f)) return x / 6.0f;
+    const float r6 = 1.0f / 6.0f;
+    const float q0 = x * r6;
+    const float r = fmaf(-q0, 6.0f, x);
+    return fmaf(r, r6, q0);
+}
+
+// Workgroup -> tile with neighbouring tiles on the same XCD: workgroups are dispatched to the 8
+// XCDs round-robin by id, so XCD x receives the contiguous tile range this mapping gives it.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    const int q = nb / 8, r = nb % 8;
+    const int xcd = b % 8, slot = b / 8;
+    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
+}
+
 hipError_t launch_sor_color(const SorParams& p, int32_t color, float* field, const int32_t* done, hipStream_t s);
 
 // Fused red+black sweep src -> dst on LDS tiles (Dirichlet faces only: fused_supported).
@@ -32,5 +56,10 @@ hipError_t launch_residual_finish(const float* partial, int count, float* residu
                                   int32_t* state, hipStream_t s);
 hipError_t launch_residual_atomic(const SorParams& p, const float* field, float* out, hipStream_t s);
 hipError_t launch_copy_f32(const float* a, float* b, int64_t n, hipStream_t s);
+
+// 2-D-tiled variant that marches along i (wbc_sor_march.hip); cfg indexes its own shape table.
+constexpr int kMarchConfigCount = 6;
+hipError_t launch_sor_march_cfg(int cfg, const SorParams& p, const float* src, float* dst, const int32_t* done,
+                                int qf, hipStream_t s);
 
 }  // namespace wbc

@@ -31,40 +31,11 @@ struct SorTile {
     static constexpr int kRows2 = TJ * TI;                 // black pass: tile
 };
 
-// Workgroup -> tile with neighbouring tiles on the same XCD (workgroups are dispatched to the 8
-// XCDs round-robin by id): XCD x gets a contiguous range of tile ids.
-__device__ __forceinline__ int xcd_tile(int b, int nb)
-{
-    const int q = nb / 8, r = nb % 8;
-    const int xcd = b % 8, slot = b / 8;
-    return xcd < r ? xcd * (q + 1) + slot : r * (q + 1) + (xcd - r) * q + slot;
-}
-
-__device__ __forceinline__ bool updated_node(const SorParams& p, int i, int j, int k, int64_t t)
-{
-    if (i <= 0 || j <= 0 || k <= 0 || i >= p.nx - 1 || j >= p.ny - 1 || k >= p.nz - 1) return false;
-    return !(p.fixed && p.fixed[t] != 0);
-}
-
-// x / 6 correctly rounded: q = RN(x * RN(1/6)) corrected once with the exact remainder
-// (Brisebarre, Muller & Raina, "Accelerating correctly rounded floating-point division when the
-// divisor is known in advance", IEEE Trans. Computers 53(8), 2004). Checked against IEEE
-// division for every non-negative finite binary32 x: identical for all |x| >= 2^-120, so
-// smaller, infinite and NaN inputs take the division.
-__device__ __forceinline__ float div6(float x)
-{
-    const float ax = fabsf(x);
-    if (!(ax >= 0x1p-120f && ax <= 3.0e38f)) return x / 6.0f;
-    const float r6 = 1.0f / 6.0f;
-    const float q0 = x * r6;
-    const float r = fmaf(-q0, 6.0f, x);
-    return fmaf(r, r6, q0);
-}
 
 // One node update from LDS neighbours (e: LDS index; row strides kLK and kLK*LJ); `dens` is the
-// node's density, loaded ahead by the caller.
+// node's density, loaded ahead by the caller; t is the node's linear index.
 template <int LJ>
-__device__ __forceinline__ float sor_update(const SorParams& p, const float* __restrict__ s, int e, int64_t t,
+__device__ __forceinline__ float sor_update(const SorParams& p, const float* __restrict__ s, int e, int t,
                                             float dens, bool q_finite)
 {
     constexpr int SJ = kLK, SI = kLK * LJ;
@@ -93,6 +64,7 @@ __device__ __forceinline__ float sor_update(const SorParams& p, const float* __r
     return fmaxf(v, 0.0f);
 }
 
+// Grids are below 2^31 nodes, so node indices are int32 throughout.
 template <int TJ, int TI>
 __global__ void __launch_bounds__(256) sor_fused_kernel(SorParams p, const float* __restrict__ src,
                                                         float* __restrict__ dst, const int32_t* __restrict__ done,
@@ -109,24 +81,37 @@ __global__ void __launch_bounds__(256) sor_fused_kernel(SorParams p, const float
     const int tj = (tile / tiles_k) % tiles_j;
     const int ti = tile / (tiles_k * tiles_j);
     const int k0 = tk * kTK, j0 = tj * TJ, i0 = ti * TI;
-    const int64_t plane = (int64_t)p.ny * p.nz;
+    const int nx = p.nx, ny = p.ny, nz = p.nz;
+    const int plane = ny * nz;
     const int lane = threadIdx.x & (kWave - 1);
     const int wave = threadIdx.x / kWave;
     const int half = lane >> 5, q = lane & 31;
+    const bool qf = q_finite != 0;
+
+    // Lane constants: the k of this lane's red / black node for either row parity, and whether it
+    // is an updated (interior) k.
+    int gk_red[2], gk_blk[2];
+    bool ok_red[2], ok_blk[2];
+#pragma unroll
+    for (int par = 0; par < 2; ++par) {
+        gk_red[par] = k0 - 1 + 2 * q + par;
+        ok_red[par] = q < (kTK + 2) / 2 && gk_red[par] > 0 && gk_red[par] < nz - 1;
+        gk_blk[par] = k0 + 2 * q + par;
+        ok_blk[par] = q < kTK / 2 && gk_blk[par] > 0 && gk_blk[par] < nz - 1;
+    }
 
     // Issue every global load of the workgroup first (field rows with a halo of 2, then the
     // densities of this lane's red and black nodes), consume afterwards: the loads overlap.
     float v[kLoadIters];
     {
         const int gk = k0 - 2 + lane;
-        const bool k_in = gk >= 0 && gk < p.nz;
+        const bool k_in = gk >= 0 && gk < nz;
 #pragma unroll
         for (int m = 0; m < kLoadIters; ++m) {
             const int row = wave + 4 * m;
             const int gj = j0 - 2 + row % T::LJ, gi = i0 - 2 + row / T::LJ;
             v[m] = 0.0f;
-            if (row < T::kRows && k_in && gi >= 0 && gi < p.nx && gj >= 0 && gj < p.ny)
-                v[m] = src[gi * plane + (int64_t)gj * p.nz + gk];
+            if (row < T::kRows && k_in && gi >= 0 && gi < nx && gj >= 0 && gj < ny) v[m] = src[gi * plane + gj * nz + gk];
         }
     }
     float dred[kRedIters], dblack[kBlackIters];
@@ -134,21 +119,19 @@ __global__ void __launch_bounds__(256) sor_fused_kernel(SorParams p, const float
     for (int m = 0; m < kRedIters; ++m) {
         const int rr = wave * 2 + half + 8 * m;
         const int gi = i0 - 1 + rr / (TJ + 2), gj = j0 - 1 + rr % (TJ + 2);
-        const int gk = k0 - 1 + 2 * q + ((gi + gj + k0 - 1) & 1);
+        const int par = (gi + gj + k0 - 1) & 1;
         dred[m] = 0.0f;
-        if (rr < T::kRingRows && q < (kTK + 2) / 2 && gi > 0 && gj > 0 && gk > 0 && gi < p.nx - 1 &&
-            gj < p.ny - 1 && gk < p.nz - 1)
-            dred[m] = p.density[gi * plane + (int64_t)gj * p.nz + gk];
+        if (rr < T::kRingRows && gi > 0 && gj > 0 && gi < nx - 1 && gj < ny - 1 && ok_red[par])
+            dred[m] = p.density[gi * plane + gj * nz + gk_red[par]];
     }
 #pragma unroll
     for (int m = 0; m < kBlackIters; ++m) {
         const int rr = wave * 2 + half + 8 * m;
-        const int gi = i0 + rr / TJ, gj = j0 + rr % TJ, gk0 = k0 + 2 * q;
-        const int gk = gk0 + (((gi + gj + gk0) & 1) ? 0 : 1);
+        const int gi = i0 + rr / TJ, gj = j0 + rr % TJ;
+        const int par = ((gi + gj + k0) & 1) ^ 1;  // black: (i+j+k) odd
         dblack[m] = 0.0f;
-        if (rr < T::kRows2 && q < kTK / 2 && gi > 0 && gj > 0 && gk > 0 && gi < p.nx - 1 && gj < p.ny - 1 &&
-            gk < p.nz - 1)
-            dblack[m] = p.density[gi * plane + (int64_t)gj * p.nz + gk];
+        if (rr < T::kRows2 && gi > 0 && gj > 0 && gi < nx - 1 && gj < ny - 1 && ok_blk[par])
+            dblack[m] = p.density[gi * plane + gj * nz + gk_blk[par]];
     }
 #pragma unroll
     for (int m = 0; m < kLoadIters; ++m) {
@@ -162,14 +145,15 @@ __global__ void __launch_bounds__(256) sor_fused_kernel(SorParams p, const float
 #pragma unroll
     for (int m = 0; m < kRedIters; ++m) {
         const int rr = wave * 2 + half + 8 * m;
-        if (rr >= T::kRingRows || q >= (kTK + 2) / 2) continue;
         const int jj = rr % (TJ + 2), ii = rr / (TJ + 2);
         const int gi = i0 - 1 + ii, gj = j0 - 1 + jj;
-        const int gk = k0 - 1 + 2 * q + ((gi + gj + k0 - 1) & 1);
-        const int64_t t = gi * plane + (int64_t)gj * p.nz + gk;
-        if (!updated_node(p, gi, gj, gk, t)) continue;
+        const int par = (gi + gj + k0 - 1) & 1;
+        if (!(rr < T::kRingRows && gi > 0 && gj > 0 && gi < nx - 1 && gj < ny - 1 && ok_red[par])) continue;
+        const int gk = gk_red[par];
+        const int t = gi * plane + gj * nz + gk;
+        if (p.fixed && p.fixed[t] != 0) continue;
         const int e = ((ii + 1) * T::LJ + (jj + 1)) * kLK + (gk - k0 + 2);
-        s[e] = sor_update<T::LJ>(p, s, e, t, dred[m], q_finite != 0);
+        s[e] = sor_update<T::LJ>(p, s, e, t, dred[m], qf);
     }
     __syncthreads();
 
@@ -177,20 +161,18 @@ __global__ void __launch_bounds__(256) sor_fused_kernel(SorParams p, const float
 #pragma unroll
     for (int m = 0; m < kBlackIters; ++m) {
         const int rr = wave * 2 + half + 8 * m;
-        if (rr >= T::kRows2 || q >= kTK / 2) continue;
         const int jj = rr % TJ, ii = rr / TJ;
         const int gi = i0 + ii, gj = j0 + jj, gk0 = k0 + 2 * q;
-        if (gi >= p.nx || gj >= p.ny || gk0 >= p.nz) continue;
+        if (!(rr < T::kRows2 && q < kTK / 2 && gi < nx && gj < ny && gk0 < nz)) continue;
         const int base = ((ii + 2) * T::LJ + (jj + 2)) * kLK + (2 * q + 2);
-        const int64_t t0 = gi * plane + (int64_t)gj * p.nz + gk0;
-        const int black = ((gi + gj + gk0) & 1) ? 0 : 1;  // which of the two nodes is black
+        const int t0 = gi * plane + gj * nz + gk0;
+        const int par = ((gi + gj + k0) & 1) ^ 1;  // offset of the black node in the pair
         float out0 = s[base], out1 = s[base + 1];
-        const int gkb = gk0 + black;
-        if (gkb < p.nz && updated_node(p, gi, gj, gkb, t0 + black)) {
-            const float u = sor_update<T::LJ>(p, s, base + black, t0 + black, dblack[m], q_finite != 0);
-            if (black) out1 = u; else out0 = u;
+        if (gi > 0 && gj > 0 && gi < nx - 1 && gj < ny - 1 && ok_blk[par] && !(p.fixed && p.fixed[t0 + par] != 0)) {
+            const float u = sor_update<T::LJ>(p, s, base + par, t0 + par, dblack[m], qf);
+            if (par) out1 = u; else out0 = u;
         }
-        const bool two = gk0 + 1 < p.nz;
+        const bool two = gk0 + 1 < nz;
         if (two && (t0 & 1) == 0) {
             *reinterpret_cast<float2*>(dst + t0) = make_float2(out0, out1);
         } else {
@@ -288,10 +270,13 @@ __global__ void __launch_bounds__(256) copy_f32_kernel(const float* __restrict__
 
 // ---- launch helpers ---------------------------------------------------------------------------
 // Tile shapes (TJ x TI; 60 along k) the autotuner chooses from.
+// kind 0: 3-D tile 60 x TJ x TI (this file); kind 1: 60 x TJ tile marching CI planes along i.
 struct FusedConfig {
-    int tj, ti;
+    int kind, tj, ti;
 };
-constexpr FusedConfig kFusedConfigs[] = {{4, 4}, {4, 8}, {8, 4}, {8, 8}, {16, 4}, {4, 16}, {2, 16}, {16, 2}};
+constexpr FusedConfig kFusedConfigs[] = {{0, 4, 4}, {0, 4, 8}, {0, 8, 4}, {0, 8, 8}, {0, 16, 4}, {0, 4, 16},
+                                         {0, 2, 16}, {0, 16, 2}, {1, 12, 8}, {1, 12, 16}, {1, 12, 24},
+                                         {1, 28, 8}, {1, 28, 16}, {1, 12, 4}};
 constexpr int kFusedConfigCount = sizeof(kFusedConfigs) / sizeof(kFusedConfigs[0]);
 int g_fused_config = 3;
 constexpr int kResidualIStride = 8;
@@ -331,7 +316,9 @@ hipError_t launch_sor_fused_cfg(int cfg, const SorParams& p, const float* src, f
     case 5: return launch_fused_tile<4, 16>(p, src, dst, done, qf, s);
     case 6: return launch_fused_tile<2, 16>(p, src, dst, done, qf, s);
     case 7: return launch_fused_tile<16, 2>(p, src, dst, done, qf, s);
-    default: return hipErrorInvalidValue;
+    default:
+        if (cfg >= 8 && cfg < 8 + kMarchConfigCount) return launch_sor_march_cfg(cfg - 8, p, src, dst, done, qf, s);
+        return hipErrorInvalidValue;
     }
 }
 
@@ -377,10 +364,10 @@ using namespace wbc;
 // Tile-shape table of the fused sweep: count, shape of entry i, and the entry the solver uses.
 WBC_API int wbc_sor_fused_config_count(void) { return kFusedConfigCount; }
 
-WBC_API int wbc_sor_fused_config_shape(int32_t cfg, int32_t* tk, int32_t* tj, int32_t* ti)
+WBC_API int wbc_sor_fused_config_shape(int32_t cfg, int32_t* kind, int32_t* tj, int32_t* ti)
 {
-    if (cfg < 0 || cfg >= kFusedConfigCount || !tk || !tj || !ti) return (int)hipErrorInvalidValue;
-    *tk = kTK;
+    if (cfg < 0 || cfg >= kFusedConfigCount || !kind || !tj || !ti) return (int)hipErrorInvalidValue;
+    *kind = kFusedConfigs[cfg].kind;
     *tj = kFusedConfigs[cfg].tj;
     *ti = kFusedConfigs[cfg].ti;
     return 0;
