@@ -166,6 +166,29 @@ int wbc_relax_contacts(float* position, const float* radius, int32_t n, float ce
                        int32_t* neighbor_count, int64_t capacity, void* workspace, size_t workspace_bytes,
                        wbc_stream_t stream);
 
+/* ---- adjoint of the steady-state solve -----------------------------------------------------
+ * No reference counterpart (the reference does not differentiate its solve). Per-cell rates are
+ * deposited as int64 fixed point of w * rate_c * q_inv_ref (the deposit_trilinear support rule),
+ * converted to float with wbc_fixed_to_float; wbc_sample_adjoint scatters dL/dS_c with the
+ * sampling weights of kernels/field_kernels.py:153-182 (float atomics into g, zeroed by the
+ * caller); wbc_adjoint_solve solves A mu = g (A = diag(6 + h Q K/(K+O)^2) - neighbours, Dirichlet
+ * faces, no pinned nodes) by red-black SOR and returns dL/d rate per cell plus
+ * sums[0] = sum mu (P - Q O/(K+O)) (dL/dD = -(h/D) sums[0]) and sums[1] = dL/dO_b. */
+int wbc_deposit_rates_i64(const float* position, const float* rate, int64_t n, float q_inv_ref, float ox, float oy,
+                          float oz, float inv_dx, int32_t nx, int32_t ny, int32_t nz, int64_t* accumulator,
+                          wbc_stream_t stream);
+int wbc_fixed_to_float(const int64_t* accumulator, int64_t nodes, float inv_voxel_volume, float* out,
+                       wbc_stream_t stream);
+int wbc_sample_adjoint(const float* grad_out, const float* position, int64_t n, float ox, float oy, float oz,
+                       float inv_dx, int32_t nx, int32_t ny, int32_t nz, float* g, wbc_stream_t stream);
+size_t wbc_adjoint_workspace_bytes(int32_t nx, int32_t ny, int32_t nz);
+int wbc_adjoint_solve(float omega, int32_t max_sweeps, float tolerance, int32_t check_every, float h, float q_ref,
+                      float michaelis_k, const int32_t* fixed, const float* O, const float* R, const float* production,
+                      const float* g, float* mu, const float* position, int64_t n, float ox, float oy, float oz,
+                      float dx, int32_t nx, int32_t ny, int32_t nz, float* grad_rate, double* sums,
+                      int32_t* sweeps_out, float* residual_out, void* workspace, size_t workspace_bytes,
+                      wbc_stream_t stream);
+
 #ifdef __cplusplus
 }
 #endif

@@ -389,3 +389,89 @@ def oracle_cell_step(pop: OraclePopulation, lifecycle, contact, stepping, dim: i
     pop.velocity[:n] = vel
     pop.neighbor_count[:n] = cnt
     return daughters, report
+
+
+# ---- FP64 adjoint oracle of the steady-state solve -------------------------------------------------
+# The discrete problem of the SOR path with per-cell rates, restated in float64 with exact
+# trilinear weights: F_t(O) = sum_nb O_nb - 6 O_t - h Q_t O_t/(K + O_t) on interior nodes, O = O_b on
+# the faces, Q_t = sum_c w_ct q_c / dx^3. Its adjoint A mu = g, A = diag(6 + h Q K/(K+O)^2) - nb, is
+# solved by orc_adjoint_sor_f64; the gradients are assembled here.
+
+def _deposit_weights(positions, origin, dx, shape):
+    """(cell index, flat node index, weight) of the deposit rule (cells outside the support dropped)."""
+    u = (np.asarray(positions, dtype=np.float64) - np.asarray(origin)) / dx
+    base = np.floor(u).astype(np.int64)
+    inside = np.all(base >= 0, axis=1) & (base[:, 0] + 1 < shape[0]) & (base[:, 1] + 1 < shape[1]) & \
+        (base[:, 2] + 1 < shape[2])
+    return _corner_weights(u, base, inside, shape)
+
+
+def _sample_weights(positions, origin, dx, shape):
+    """The clamped sampling rule of sample_trilinear."""
+    u = (np.asarray(positions, dtype=np.float64) - np.asarray(origin)) / dx
+    u = np.clip(u, 0.0, np.array(shape, dtype=np.float64) - 1.0)
+    base = np.minimum(np.floor(u).astype(np.int64), np.array(shape) - 2)
+    return _corner_weights(u, base, np.ones(len(u), dtype=bool), shape)
+
+
+def _corner_weights(u, base, inside, shape):
+    frac = u - base
+    cells, nodes, weights = [], [], []
+    idx = np.nonzero(inside)[0]
+    for di in (0, 1):
+        for dj in (0, 1):
+            for dk in (0, 1):
+                w = (frac[idx, 0] if di else 1 - frac[idx, 0]) * (frac[idx, 1] if dj else 1 - frac[idx, 1]) * \
+                    (frac[idx, 2] if dk else 1 - frac[idx, 2])
+                flat = ((base[idx, 0] + di) * shape[1] + base[idx, 1] + dj) * shape[2] + base[idx, 2] + dk
+                cells.append(idx)
+                nodes.append(flat)
+                weights.append(w)
+    return np.concatenate(cells), np.concatenate(nodes), np.concatenate(weights)
+
+
+def forward_f64(positions, rates, diffusion, boundary_value, michaelis_k, origin, dx, shape, sweeps=20000,
+                tolerance=1e-12, omega=1.8, start=None):
+    """Converged float64 field and the samples at the cells."""
+    c, t, w = _deposit_weights(positions, origin, dx, shape)
+    Q = np.zeros(int(np.prod(shape)))
+    np.add.at(Q, t, w * np.asarray(rates, dtype=np.float64)[c])
+    Q = (Q / dx ** 3).reshape(shape)
+    field = np.full(shape, float(boundary_value)) if start is None else start.astype(np.float64).copy()
+    apply_dirichlet(field, (0, 0, 0), float(boundary_value))
+    a = SolveArgs(dx=dx, diffusion=diffusion, uptake_max=1.0, michaelis_k=michaelis_k, boundary_value=boundary_value)
+    report = solve(field, Q, a, omega, sweeps, tolerance, 10)
+    sc, st, sw = _sample_weights(positions, origin, dx, shape)
+    S = np.zeros(len(positions))
+    np.add.at(S, sc, sw * field.reshape(-1)[st])
+    return field, Q, S, report
+
+
+def adjoint_f64(positions, grad_S, field, Q, diffusion, michaelis_k, origin, dx, shape, sweeps=4000, omega=1.8):
+    """(dL/dq_c, dL/dD, dL/dO_b) at the float64 solution ``field`` for dL/dS = grad_S."""
+    sc, st, sw = _sample_weights(positions, origin, dx, shape)
+    g = np.zeros(int(np.prod(shape)))
+    np.add.at(g, st, sw * np.asarray(grad_S, dtype=np.float64)[sc])
+    g = g.reshape(shape)
+    h = dx ** 2 / diffusion
+    mu = np.zeros(shape)
+    lib().orc_adjoint_sor_f64(sweeps, omega, h, 1.0, michaelis_k, _p(np.ascontiguousarray(Q)),
+                              _p(np.ascontiguousarray(field)), _p(np.ascontiguousarray(g)), _p(mu), *shape)
+    inner = (slice(1, -1),) * 3
+    ratio = field / (michaelis_k + field)
+    dc, dt, dw = _deposit_weights(positions, origin, dx, shape)
+    grad_q = np.zeros(len(positions))
+    np.add.at(grad_q, dc, dw * (mu * ratio).reshape(-1)[dt])
+    grad_q *= -h / dx ** 3
+    grad_D = -(h / diffusion) * float(np.sum(mu[inner] * (-Q[inner] * ratio[inner])))
+    face = np.ones(shape, dtype=bool)
+    face[inner] = False
+    nd = np.zeros(shape)
+    nd[1, :, :] += 1
+    nd[-2, :, :] += 1
+    nd[:, 1, :] += 1
+    nd[:, -2, :] += 1
+    nd[:, :, 1] += 1
+    nd[:, :, -2] += 1
+    grad_Ob = float(g[face].sum() + np.sum(mu[inner] * nd[inner]))
+    return grad_q, grad_D, grad_Ob, mu

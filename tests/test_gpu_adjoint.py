@@ -1,0 +1,103 @@
+"""GPU adjoint of the steady-state solve: against the FP64 adjoint oracle (itself pinned to
+central differences on the CPU), through torch.autograd, and against FP32 central differences
+(step 1e-3 relative) at the full 100k-cell, 129^3 size."""
+
+import time
+
+import numpy as np
+import pytest
+import torch
+
+from warpbiocell_amd import workloads as W
+from warpbiocell_amd.adjoint import AdjointSettings, OxygenAdjointProblem, sampled_oxygen
+from warpbiocell_amd.fields.diffusion import SolverSettings
+from warpbiocell_amd.fields.scalar_field import GridGeometry
+
+pytestmark = pytest.mark.gpu
+
+
+def _moderate(seed=3, n=3000):
+    rng = np.random.default_rng(seed)
+    pos = W.ball_uniform(n, 150.0, rng)
+    rates = (W.UPTAKE_MMHG_UM3_PER_H * np.exp(0.3 * rng.standard_normal(n))).astype(np.float32)
+    targets = rng.uniform(0.0, 40.0, n).astype(np.float32)
+    return pos, rates, targets, GridGeometry.centered_cube(600.0, 12.5)
+
+
+def _rel(a, b):
+    return float(np.linalg.norm(np.asarray(a, np.float64) - b) / np.linalg.norm(b))
+
+
+def test_gradients_match_the_fp64_adjoint(oracle, device):
+    pos, rates, targets, geom = _moderate()
+    D, Ob, K = W.D_UM2_PER_H, 40.0, 1.0
+    prob = OxygenAdjointProblem(geom, torch.from_numpy(pos), K, W.UPTAKE_MMHG_UM3_PER_H,
+                                SolverSettings(omega=1.8, tolerance=1e-6, max_sweeps=20000, check_every=10),
+                                AdjointSettings(tolerance=1e-7), device=device)
+    S = prob.forward(torch.from_numpy(rates), D, Ob)
+    gq, gD, gOb = prob.backward(2.0 * (S - torch.from_numpy(targets).to(device)))
+    f64, Q64, S64, rep = oracle.forward_f64(pos, rates.astype(np.float64), D, Ob, K, geom.origin, geom.dx, geom.shape)
+    assert rep[2]
+    rq, rD, rOb, _ = oracle.adjoint_f64(pos, 2.0 * (S64 - targets), f64, Q64, D, K, geom.origin, geom.dx, geom.shape,
+                                        sweeps=6000)
+    errs = {"S": _rel(S.cpu().numpy(), S64), "dq": _rel(gq.cpu().numpy(), rq), "dD": abs(gD - rD) / abs(rD),
+            "dOb": abs(gOb - rOb) / abs(rOb)}
+    print("relative error vs FP64 adjoint:", {k: f"{v:.2e}" for k, v in errs.items()},
+          "forward sweeps", prob.last_forward.sweeps, "adjoint sweeps", prob.last_backward[0])
+    assert errs["S"] < 1e-4 and errs["dq"] < 2e-3 and errs["dD"] < 2e-3 and errs["dOb"] < 2e-3
+
+
+def test_autograd_function_routes_the_adjoint(device):
+    pos, rates, targets, geom = _moderate(n=800)
+    prob = OxygenAdjointProblem(geom, torch.from_numpy(pos), 1.0, W.UPTAKE_MMHG_UM3_PER_H, device=device)
+    q = torch.from_numpy(rates).to(device).requires_grad_(True)
+    D = torch.tensor(W.D_UM2_PER_H, dtype=torch.float64, requires_grad=True)
+    Ob = torch.tensor(40.0, dtype=torch.float64, requires_grad=True)
+    S = sampled_oxygen(q, D, Ob, prob)
+    loss = ((S - torch.from_numpy(targets).to(device)) ** 2).sum()
+    loss.backward()
+    gq, gD, gOb = prob.backward(2.0 * (S.detach() - torch.from_numpy(targets).to(device)))
+    np.testing.assert_allclose(q.grad.cpu().numpy(), gq.cpu().numpy(), rtol=1e-5, atol=1e-30)
+    assert D.grad.item() == pytest.approx(gD, rel=1e-6) and Ob.grad.item() == pytest.approx(gOb, rel=1e-6)
+
+
+def test_full_size_gradients_against_central_differences(device):
+    w, rates, targets = W.adjoint_100k_129()
+    geom = GridGeometry.centered_cube(w.box_um, w.dx_um)
+    D, Ob = W.D_UM2_PER_H, w.boundary_mmHg
+    prob = OxygenAdjointProblem(geom, torch.from_numpy(w.positions), W.MICHAELIS_K, W.UPTAKE_MMHG_UM3_PER_H,
+                                SolverSettings(omega=1.8, tolerance=1e-6, max_sweeps=20000, check_every=10),
+                                device=device)
+    T = torch.from_numpy(targets).to(device)
+    q = torch.from_numpy(rates).to(device)
+
+    def loss(qq, d, ob):
+        S = prob.forward(qq, d, ob)
+        return float(((S.double() - T.double()) ** 2).sum()), S
+
+    L0, S = loss(q, D, Ob)
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    L0, S = loss(q, D, Ob)
+    torch.cuda.synchronize()
+    t_fwd = time.perf_counter() - t0
+    t0 = time.perf_counter()
+    gq, gD, gOb = prob.backward(2.0 * (S - T))
+    torch.cuda.synchronize()
+    t_bwd = time.perf_counter() - t0
+    errs = {}
+    for name, grad, fn, base in (("D", gD, lambda e: loss(q, D + e, Ob)[0], D),
+                                 ("O_b", gOb, lambda e: loss(q, D, Ob + e)[0], Ob)):
+        eps = 1e-3 * base
+        fd = (fn(eps) - fn(-eps)) / (2 * eps)
+        errs[name] = abs(fd - grad) / abs(fd)
+    # Directional central difference over all 100k rates: direction v = +-1, step 1e-3 relative.
+    v = torch.from_numpy(np.random.default_rng(5).choice([-1.0, 1.0], w.n).astype(np.float32)).to(device)
+    dq = 1e-3 * q * v
+    fd = (loss(q + dq, D, Ob)[0] - loss(q - dq, D, Ob)[0]) / 2.0
+    adj = float((gq.double() * dq.double()).sum())
+    errs["rates (directional)"] = abs(fd - adj) / abs(fd)
+    print(f"129^3 adjoint: forward {t_fwd * 1e3:.1f} ms ({prob.last_forward.sweeps} sweeps), backward "
+          f"{t_bwd * 1e3:.1f} ms ({prob.last_backward[0]} sweeps), ratio {t_bwd / t_fwd:.2f}; FD relative errors "
+          + ", ".join(f"{k} {e:.2e}" for k, e in errs.items()))
+    assert all(e < 2e-2 for e in errs.values()), errs

@@ -53,6 +53,12 @@ SIGNATURES: dict[str, tuple] = {
     "wbc_contact_velocities": (I32, [P, c_size_t, I32, I64, I32, F32, F32, F32, P, P, P]),
     "wbc_integrate_positions": (I32, [P, P, I64, F32, P]),
     "wbc_relax_contacts": (I32, [P, P, I32, F32, I32, F32, F32, F32, I32, P, P, I64, P, c_size_t, P]),
+    "wbc_deposit_rates_i64": (I32, [P, P, I64, F32, F32, F32, F32, F32, I32, I32, I32, P, P]),
+    "wbc_fixed_to_float": (I32, [P, I64, F32, P, P]),
+    "wbc_sample_adjoint": (I32, [P, P, I64, F32, F32, F32, F32, I32, I32, I32, P, P]),
+    "wbc_adjoint_workspace_bytes": (c_size_t, [I32, I32, I32]),
+    "wbc_adjoint_solve": (I32, [F32, I32, F32, I32, F32, F32, F32, P, P, P, P, P, P, P, I64, F32, F32, F32, F32,
+                                I32, I32, I32, P, P, P, P, P, c_size_t, P]),
 }
 
 _lib = None

@@ -1,0 +1,435 @@
+// Adjoint of the steady-state oxygen solve (no reference counterpart: the reference has no
+// differentiation; this is the implicit adjoint of the discrete problem its SOR solves).
+//
+// Forward problem on updated nodes t (kernels/field_kernels.py:209-212 at the fixed point):
+//   F_t(O) = sum_nb O_nb + h P_t - 6 O_t - h Q_t O_t / (K + O_t) = 0,     h = dx^2 / D,
+// with Q_t = sum_c w_ct q_c / dx^3 (per-cell uptake rates deposited trilinearly; the reference's
+// per-state form is Q = uptake_max * density) and O = O_b on Dirichlet faces.
+// Loss L(S(O)) with S the trilinear sample at the cells (kernels/field_kernels.py:153-182).
+// With g = dL/dO (sample adjoint: scatter of dL/dS_c with the sampling weights) and
+//   A = -dF/dO = diag(6 + a_t) - (neighbours),   a_t = h Q_t K / (K + O_t)^2,
+// which is symmetric for Dirichlet faces, solve A mu = g; then for any parameter theta
+//   dL/dtheta = dL/dtheta|explicit + mu^T dF/dtheta:
+//   dL/dq_c = -(h / dx^3) sum_t w_ct mu_t O_t / (K + O_t)           (per-cell gather)
+//   dL/dD   = -(h / D) sum_t mu_t (P_t - Q_t O_t / (K + O_t))       (dh/dD = -h/D)
+//   dL/dO_b = sum_{Dirichlet b} g_b + sum_t mu_t * (Dirichlet neighbours of t)
+// Only the converged forward field enters: nothing of the forward iteration is stored.
+// The max(., 0) clamp of the forward update is taken as inactive at the fixed point (MM uptake
+// vanishes as O -> 0, so the discrete solution is non-negative without it).
+#include "wbc_field.hpp"
+
+namespace wbc {
+
+// acc[t] += int64(w * (q_c / q_ref) * 2^24 + 0.5) over the 8 nodes of each cell inside the
+// trilinear support (deposit_trilinear's rule); q_ref scales the rates to O(1) fixed point.
+__global__ void __launch_bounds__(256) deposit_rates_kernel(const float* __restrict__ position,
+                                                             const float* __restrict__ rate, int64_t n, float q_inv_ref,
+                                                             float ox, float oy, float oz, float inv_dx, int32_t nx,
+                                                             int32_t ny, int32_t nz,
+                                                             unsigned long long* __restrict__ acc)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const float scale = rate[c] * q_inv_ref;
+    float ux = (position[3 * c + 0] - ox) * inv_dx;
+    float uy = (position[3 * c + 1] - oy) * inv_dx;
+    float uz = (position[3 * c + 2] - oz) * inv_dx;
+    int32_t i0 = (int32_t)floorf(ux), j0 = (int32_t)floorf(uy), k0 = (int32_t)floorf(uz);
+    if (i0 < 0 || j0 < 0 || k0 < 0 || i0 + 1 >= nx || j0 + 1 >= ny || k0 + 1 >= nz) return;
+    float fx = ux - (float)i0, fy = uy - (float)j0, fz = uz - (float)k0;
+    const int64_t plane = (int64_t)ny * nz;
+    unsigned long long* base = acc + i0 * plane + (int64_t)j0 * nz + k0;
+#pragma unroll
+    for (int di = 0; di < 2; ++di)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) {
+                float w = (di ? fx : 1.0f - fx) * (dj ? fy : 1.0f - fy) * (dk ? fz : 1.0f - fz);
+                long long qv = (long long)(w * scale * kFixedOne + 0.5f);
+                atomicAdd(base + di * plane + dj * nz + dk, (unsigned long long)qv);
+            }
+}
+
+__global__ void __launch_bounds__(256) fixed_to_float_kernel(const long long* __restrict__ acc, int64_t nodes,
+                                                             float inv_vol, float* __restrict__ out)
+{
+    int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t < nodes) out[t] = (float)acc[t] / kFixedOne * inv_vol;
+}
+
+// Clamped trilinear weights exactly as sample_trilinear (kernels/field_kernels.py:162-182).
+struct Tri {
+    int64_t base;
+    float fx, fy, fz;
+};
+
+__device__ __forceinline__ Tri sample_weights(const float* position, int64_t c, float ox, float oy, float oz,
+                                              float inv_dx, int nx, int ny, int nz)
+{
+    float ux = (position[3 * c + 0] - ox) * inv_dx;
+    float uy = (position[3 * c + 1] - oy) * inv_dx;
+    float uz = (position[3 * c + 2] - oz) * inv_dx;
+    ux = fminf(fmaxf(ux, 0.0f), (float)(nx - 1));
+    uy = fminf(fmaxf(uy, 0.0f), (float)(ny - 1));
+    uz = fminf(fmaxf(uz, 0.0f), (float)(nz - 1));
+    int i0 = min((int)floorf(ux), nx - 2), j0 = min((int)floorf(uy), ny - 2), k0 = min((int)floorf(uz), nz - 2);
+    Tri w;
+    w.fx = ux - (float)i0;
+    w.fy = uy - (float)j0;
+    w.fz = uz - (float)k0;
+    w.base = (int64_t)i0 * ny * nz + (int64_t)j0 * nz + k0;
+    return w;
+}
+
+// g[t] += dL/dS_c * (sampling weight of t at cell c).
+__global__ void __launch_bounds__(256) sample_adjoint_kernel(const float* __restrict__ grad_out,
+                                                              const float* __restrict__ position, int64_t n, float ox,
+                                                              float oy, float oz, float inv_dx, int nx, int ny, int nz,
+                                                              float* __restrict__ g)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const float go = grad_out[c];
+    if (go == 0.0f) return;
+    Tri w = sample_weights(position, c, ox, oy, oz, inv_dx, nx, ny, nz);
+    const int64_t plane = (int64_t)ny * nz;
+#pragma unroll
+    for (int di = 0; di < 2; ++di)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) {
+                float wt = (di ? w.fx : 1.0f - w.fx) * (dj ? w.fy : 1.0f - w.fy) * (dk ? w.fz : 1.0f - w.fz);
+                atomicAdd(g + w.base + di * plane + dj * nz + dk, go * wt);
+            }
+}
+
+struct AdjointParams {
+    float omega, h, q_ref, michaelis_k;
+    const int32_t* fixed;
+    const float* O;       // converged forward field
+    const float* R;       // rate density / q_ref (Q = q_ref * R)
+    int nx, ny, nz;
+};
+
+__device__ __forceinline__ bool adj_updated(const AdjointParams& a, int i, int j, int k, int64_t t)
+{
+    if (i <= 0 || j <= 0 || k <= 0 || i >= a.nx - 1 || j >= a.ny - 1 || k >= a.nz - 1) return false;
+    return !(a.fixed && a.fixed[t] != 0);
+}
+
+// inv_diag[t] = 1 / (6 + h Q_t K / (K + O_t)^2) and rhs[t] = g[t] * inv_diag[t] on updated nodes,
+// 0 elsewhere; mu starts at 0 (Dirichlet and pinned nodes keep mu = 0).
+__global__ void __launch_bounds__(256) adjoint_setup_kernel(AdjointParams a, const float* __restrict__ g,
+                                                            float* __restrict__ inv_diag, float* __restrict__ rhs,
+                                                            float* __restrict__ mu)
+{
+    const int i = blockIdx.z, j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
+    if (j >= a.ny || k >= a.nz) return;
+    const int64_t t = (int64_t)i * a.ny * a.nz + (int64_t)j * a.nz + k;
+    mu[t] = 0.0f;
+    if (!adj_updated(a, i, j, k, t)) {
+        inv_diag[t] = 0.0f;
+        rhs[t] = 0.0f;
+        return;
+    }
+    const float kpo = a.michaelis_k + a.O[t];
+    const float diag = 6.0f + a.h * (a.q_ref * a.R[t]) * a.michaelis_k / (kpo * kpo);
+    const float id = 1.0f / diag;
+    inv_diag[t] = id;
+    rhs[t] = g[t] * id;
+}
+
+// One colour of red-black SOR on A mu = g (in place): mu <- (1-w) mu + w (nb * inv_diag + rhs).
+__global__ void __launch_bounds__(256) adjoint_color_kernel(AdjointParams a, int color,
+                                                            const float* __restrict__ inv_diag,
+                                                            const float* __restrict__ rhs, float* __restrict__ mu)
+{
+    const int i = blockIdx.z, j = blockIdx.y * blockDim.y + threadIdx.y;
+    const int k = 2 * (blockIdx.x * blockDim.x + threadIdx.x) + ((color + i + j) & 1);
+    if (j >= a.ny || k >= a.nz) return;
+    const int64_t plane = (int64_t)a.ny * a.nz;
+    const int64_t t = i * plane + (int64_t)j * a.nz + k;
+    if (!adj_updated(a, i, j, k, t)) return;
+    const float nb = mu[t - plane] + mu[t + plane] + mu[t - a.nz] + mu[t + a.nz] + mu[t - 1] + mu[t + 1];
+    mu[t] = (1.0f - a.omega) * mu[t] + a.omega * fmaf(nb, inv_diag[t], rhs[t]);
+}
+
+// Per-block maxima of |g - A mu| (scaled by inv_diag: |rhs + nb*inv_diag - mu|) for the check.
+__global__ void __launch_bounds__(256) adjoint_residual_kernel(AdjointParams a, const float* __restrict__ inv_diag,
+                                                               const float* __restrict__ rhs,
+                                                               const float* __restrict__ mu,
+                                                               float* __restrict__ partial)
+{
+    __shared__ float wm[4];
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t plane = (int64_t)a.ny * a.nz;
+    float m = 0.0f;
+    if (j < a.ny && k < a.nz) {
+        for (int i = blockIdx.z; i < a.nx; i += gridDim.z) {
+            const int64_t t = i * plane + (int64_t)j * a.nz + k;
+            if (!adj_updated(a, i, j, k, t)) continue;
+            const float nb = mu[t - plane] + mu[t + plane] + mu[t - a.nz] + mu[t + a.nz] + mu[t - 1] + mu[t + 1];
+            m = fmaxf(m, fabsf(fmaf(nb, inv_diag[t], rhs[t]) - mu[t]));
+        }
+    }
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    m = wave_max(m);
+    if ((tid & 63) == 0) wm[tid / 64] = m;
+    __syncthreads();
+    if (tid == 0) partial[(blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x] =
+        fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
+}
+
+// dL/dq_c = -(h / dx^3) sum_t w_ct mu_t O_t / (K + O_t) with the deposit's support rule.
+__global__ void __launch_bounds__(256) rate_gradient_kernel(AdjointParams a, const float* __restrict__ mu,
+                                                            const float* __restrict__ position, int64_t n, float ox,
+                                                            float oy, float oz, float inv_dx, float h_over_vol,
+                                                            float* __restrict__ grad)
+{
+    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    float ux = (position[3 * c + 0] - ox) * inv_dx;
+    float uy = (position[3 * c + 1] - oy) * inv_dx;
+    float uz = (position[3 * c + 2] - oz) * inv_dx;
+    int32_t i0 = (int32_t)floorf(ux), j0 = (int32_t)floorf(uy), k0 = (int32_t)floorf(uz);
+    if (i0 < 0 || j0 < 0 || k0 < 0 || i0 + 1 >= a.nx || j0 + 1 >= a.ny || k0 + 1 >= a.nz) {
+        grad[c] = 0.0f;
+        return;
+    }
+    float fx = ux - (float)i0, fy = uy - (float)j0, fz = uz - (float)k0;
+    const int64_t plane = (int64_t)a.ny * a.nz;
+    const int64_t base = i0 * plane + (int64_t)j0 * a.nz + k0;
+    float s = 0.0f;
+#pragma unroll
+    for (int di = 0; di < 2; ++di)
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj)
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) {
+                const int64_t t = base + di * plane + dj * a.nz + dk;
+                const float w = (di ? fx : 1.0f - fx) * (dj ? fy : 1.0f - fy) * (dk ? fz : 1.0f - fz);
+                const float o = a.O[t];
+                s += w * mu[t] * (o / (a.michaelis_k + o));
+            }
+    grad[c] = -h_over_vol * s;
+}
+
+// Block partial sums (double) of  mu_t (P_t - Q_t O_t/(K+O_t))  and of  mu_t * (Dirichlet neighbours)
+// + g_b on Dirichlet-face nodes, for dL/dD and dL/dO_b.
+__global__ void __launch_bounds__(256) global_gradient_kernel(AdjointParams a, const float* __restrict__ mu,
+                                                              const float* __restrict__ g,
+                                                              const float* __restrict__ production,
+                                                              double* __restrict__ partial)
+{
+    __shared__ double sd[256], sb[256];
+    const int j = blockIdx.y * blockDim.y + threadIdx.y, k = blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t plane = (int64_t)a.ny * a.nz;
+    double accD = 0.0, accB = 0.0;
+    if (j < a.ny && k < a.nz) {
+        for (int i = blockIdx.z; i < a.nx; i += gridDim.z) {
+            const int64_t t = i * plane + (int64_t)j * a.nz + k;
+            const bool face = i == 0 || j == 0 || k == 0 || i == a.nx - 1 || j == a.ny - 1 || k == a.nz - 1;
+            if (face) {
+                accB += (double)g[t];
+                continue;
+            }
+            if (a.fixed && a.fixed[t] != 0) continue;
+            const double m = mu[t];
+            const double o = a.O[t];
+            const double q = (double)a.q_ref * (double)a.R[t];
+            const double p = production ? (double)production[t] : 0.0;
+            accD += m * (p - q * o / ((double)a.michaelis_k + o));
+            const int nd = (i == 1) + (i == a.nx - 2) + (j == 1) + (j == a.ny - 2) + (k == 1) + (k == a.nz - 2);
+            accB += m * (double)nd;
+        }
+    }
+    const int tid = threadIdx.y * blockDim.x + threadIdx.x;
+    sd[tid] = accD;
+    sb[tid] = accB;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (tid < s2) {
+            sd[tid] += sd[tid + s2];
+            sb[tid] += sb[tid + s2];
+        }
+        __syncthreads();
+    }
+    if (tid == 0) {
+        const int b = (blockIdx.z * gridDim.y + blockIdx.y) * gridDim.x + blockIdx.x;
+        partial[2 * b] = sd[0];
+        partial[2 * b + 1] = sb[0];
+    }
+}
+
+// One-block reductions of the partials (fixed order: deterministic).
+__global__ void __launch_bounds__(256) max_reduce_kernel(const float* __restrict__ partial, int n, float* __restrict__ out)
+{
+    __shared__ float sm[256];
+    float m = 0.0f;
+    for (int b = threadIdx.x; b < n; b += 256) m = fmaxf(m, partial[b]);
+    sm[threadIdx.x] = m;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (threadIdx.x < s2) sm[threadIdx.x] = fmaxf(sm[threadIdx.x], sm[threadIdx.x + s2]);
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) out[0] = sm[0];
+}
+
+__global__ void __launch_bounds__(256) sum2_reduce_kernel(const double* __restrict__ partial, int n,
+                                                          double* __restrict__ out)
+{
+    __shared__ double s0[256], s1[256];
+    double a = 0.0, b = 0.0;
+    for (int i = threadIdx.x; i < n; i += 256) {
+        a += partial[2 * i];
+        b += partial[2 * i + 1];
+    }
+    s0[threadIdx.x] = a;
+    s1[threadIdx.x] = b;
+    __syncthreads();
+    for (int s2 = 128; s2 > 0; s2 >>= 1) {
+        if (threadIdx.x < s2) {
+            s0[threadIdx.x] += s0[threadIdx.x + s2];
+            s1[threadIdx.x] += s1[threadIdx.x + s2];
+        }
+        __syncthreads();
+    }
+    if (threadIdx.x == 0) {
+        out[0] = s0[0];
+        out[1] = s1[0];
+    }
+}
+
+static dim3 node_grid(int nx, int ny, int nz) { return dim3(grid_for(nz, 64), grid_for(ny, 4), nx); }
+static dim3 strided_grid(int nx, int ny, int nz)
+{
+    return dim3(grid_for(nz, 64), grid_for(ny, 4), (unsigned)(nx < 8 ? nx : 8));
+}
+
+}  // namespace wbc
+
+using namespace wbc;
+
+WBC_API int wbc_deposit_rates_i64(const float* position, const float* rate, int64_t n, float q_inv_ref, float ox,
+                                  float oy, float oz, float inv_dx, int32_t nx, int32_t ny, int32_t nz,
+                                  int64_t* accumulator, hipStream_t stream)
+{
+    if (n < 0 || n > INT32_MAX || nx < 3 || ny < 3 || nz < 3) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!position || !rate || !accumulator) return (int)hipErrorInvalidValue;
+    deposit_rates_kernel<<<grid_for(n, 256), 256, 0, stream>>>(position, rate, n, q_inv_ref, ox, oy, oz, inv_dx, nx,
+                                                                ny, nz, (unsigned long long*)accumulator);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+WBC_API int wbc_fixed_to_float(const int64_t* accumulator, int64_t nodes, float inv_voxel_volume, float* out,
+                               hipStream_t stream)
+{
+    if (nodes <= 0 || !accumulator || !out) return (int)hipErrorInvalidValue;
+    fixed_to_float_kernel<<<grid_for(nodes, 256), 256, 0, stream>>>((const long long*)accumulator, nodes,
+                                                                     inv_voxel_volume, out);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+WBC_API int wbc_sample_adjoint(const float* grad_out, const float* position, int64_t n, float ox, float oy, float oz,
+                               float inv_dx, int32_t nx, int32_t ny, int32_t nz, float* g, hipStream_t stream)
+{
+    if (n < 0 || n > INT32_MAX || nx < 3 || ny < 3 || nz < 3) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!grad_out || !position || !g) return (int)hipErrorInvalidValue;
+    sample_adjoint_kernel<<<grid_for(n, 256), 256, 0, stream>>>(grad_out, position, n, ox, oy, oz, inv_dx, nx, ny, nz,
+                                                                 g);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+// Workspace: inv_diag and rhs (nodes floats each), residual partials, double partials, and the
+// reduced scalars.
+WBC_API size_t wbc_adjoint_workspace_bytes(int32_t nx, int32_t ny, int32_t nz)
+{
+    if (nx < 3 || ny < 3 || nz < 3) return 0;
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    dim3 g = strided_grid(nx, ny, nz);
+    const int64_t parts = (int64_t)g.x * g.y * g.z;
+    auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
+    return (size_t)(up(nodes * 4) * 2 + up(parts * 4) + up(parts * 16) + 256);
+}
+
+// Solve A mu = g by red-black SOR from mu = 0, checking max|nb*inv_diag + rhs - mu| every
+// check_every sweeps against tolerance * max|rhs| (one host sync per check); then the rate
+// gradient per cell and the two global sums. Outputs: mu (grid), grad_rate[n], sums[2] =
+// {sum mu (P - Q O/(K+O)), sum g_face + sum mu * Dirichlet neighbours}, sweeps, residual.
+WBC_API int wbc_adjoint_solve(float omega, int32_t max_sweeps, float tolerance, int32_t check_every, float h, float q_ref,
+                              float michaelis_k, const int32_t* fixed, const float* O, const float* R,
+                              const float* production, const float* g, float* mu, const float* position, int64_t n,
+                              float ox, float oy, float oz, float dx, int32_t nx, int32_t ny, int32_t nz,
+                              float* grad_rate, double* sums, int32_t* sweeps_out, float* residual_out,
+                              void* workspace, size_t workspace_bytes, hipStream_t stream)
+{
+    if (nx < 3 || ny < 3 || nz < 3 || max_sweeps < 1 || check_every < 1) return (int)hipErrorInvalidValue;
+    if (!O || !R || !g || !mu || !workspace || !sums || !sweeps_out || !residual_out) return (int)hipErrorInvalidValue;
+    if (n > 0 && (!position || !grad_rate)) return (int)hipErrorInvalidValue;
+    if (workspace_bytes < wbc_adjoint_workspace_bytes(nx, ny, nz)) return (int)hipErrorInvalidValue;
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
+    char* ws = (char*)workspace;
+    float* inv_diag = (float*)ws;
+    float* rhs = (float*)(ws + up(nodes * 4));
+    float* partial = (float*)(ws + 2 * up(nodes * 4));
+    dim3 sg = strided_grid(nx, ny, nz);
+    const int parts = (int)(sg.x * sg.y * sg.z);
+    double* dpartial = (double*)(ws + 2 * up(nodes * 4) + up((int64_t)parts * 4));
+    char* scalars = ws + 2 * up(nodes * 4) + up((int64_t)parts * 4) + up((int64_t)parts * 16);
+    float* dmax = (float*)scalars;            // reduced maximum
+    double* dsums = (double*)(scalars + 64);  // reduced sums
+    AdjointParams a{omega, h, q_ref, michaelis_k, fixed, O, R, nx, ny, nz};
+    const dim3 blk(64, 4, 1);
+    adjoint_setup_kernel<<<node_grid(nx, ny, nz), blk, 0, stream>>>(a, g, inv_diag, rhs, mu);
+    WBC_CHECK_LAUNCH();
+    // Scale of the right-hand side for the relative stopping test: max|rhs| (the residual at mu = 0).
+    auto reduced_max = [&](float& out) -> hipError_t {
+        adjoint_residual_kernel<<<sg, blk, 0, stream>>>(a, inv_diag, rhs, mu, partial);
+        max_reduce_kernel<<<1, 256, 0, stream>>>(partial, parts, dmax);
+        WBC_TRY_ERR(hipGetLastError());
+        WBC_TRY_ERR(hipMemcpyAsync(&out, dmax, sizeof(float), hipMemcpyDeviceToHost, stream));
+        return hipStreamSynchronize(stream);
+    };
+    float scale = 0.0f;
+    WBC_TRY(reduced_max(scale));
+    const dim3 cg(grid_for((nz + 1) / 2, 64), grid_for(ny, 4), nx);
+    int32_t sweeps = 0;
+    float res = scale;
+    while (scale > 0.0f && sweeps < max_sweeps) {
+        const int32_t chunk = max_sweeps - sweeps < check_every ? max_sweeps - sweeps : check_every;
+        for (int32_t s = 0; s < chunk; ++s) {
+            adjoint_color_kernel<<<cg, blk, 0, stream>>>(a, 0, inv_diag, rhs, mu);
+            adjoint_color_kernel<<<cg, blk, 0, stream>>>(a, 1, inv_diag, rhs, mu);
+        }
+        WBC_CHECK_LAUNCH();
+        sweeps += chunk;
+        WBC_TRY(reduced_max(res));
+        res /= scale;
+        if (res < tolerance) break;
+    }
+    if (n > 0) {
+        rate_gradient_kernel<<<grid_for(n, 256), 256, 0, stream>>>(a, mu, position, n, ox, oy, oz, 1.0f / dx,
+                                                                    h / (dx * dx * dx), grad_rate);
+        WBC_CHECK_LAUNCH();
+    }
+    global_gradient_kernel<<<sg, blk, 0, stream>>>(a, mu, g, production, dpartial);
+    sum2_reduce_kernel<<<1, 256, 0, stream>>>(dpartial, parts, dsums);
+    WBC_CHECK_LAUNCH();
+    double two[2];
+    WBC_TRY(hipMemcpyAsync(two, dsums, sizeof(two), hipMemcpyDeviceToHost, stream));
+    WBC_TRY(hipStreamSynchronize(stream));
+    sums[0] = two[0];
+    sums[1] = two[1];
+    *sweeps_out = sweeps;
+    *residual_out = res;
+    return 0;
+}
