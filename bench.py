@@ -95,6 +95,38 @@ def sor_roofline(ox, geom, device, launches: int = 200) -> dict:
             "algorithmic_bytes_per_launch": bytes_per_launch}
 
 
+def adjoint_measurement(device) -> dict:
+    """Pattern 3 on its own workload (100k cells, 129^3): one forward solve from a uniform start and
+    one adjoint gradient (w.r.t. all per-cell rates, D and O_b) of sum (S - target)^2."""
+    import torch
+
+    from warpbiocell_amd import workloads as W
+    from warpbiocell_amd.adjoint import OxygenAdjointProblem
+    from warpbiocell_amd.fields.diffusion import SolverSettings
+    from warpbiocell_amd.fields.scalar_field import GridGeometry
+
+    w, rates, targets = W.adjoint_100k_129()
+    geom = GridGeometry.centered_cube(w.box_um, w.dx_um)
+    prob = OxygenAdjointProblem(geom, torch.from_numpy(w.positions), W.MICHAELIS_K, W.UPTAKE_MMHG_UM3_PER_H,
+                                SolverSettings(omega=w.omega, tolerance=w.tolerance, max_sweeps=w.max_sweeps,
+                                               check_every=w.check_every), device=device)
+    q = torch.from_numpy(rates).to(device)
+    T = torch.from_numpy(targets).to(device)
+    S = prob.forward(q, W.D_UM2_PER_H, w.boundary_mmHg)
+    prob.backward(2.0 * (S - T))
+    torch.cuda.synchronize()
+    t0 = time.perf_counter()
+    S = prob.forward(q, W.D_UM2_PER_H, w.boundary_mmHg)
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    prob.backward(2.0 * (S - T))
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    return {"workload": w.name, "forward_ms": round((t1 - t0) * 1e3, 3), "forward_sweeps": prob.last_forward.sweeps,
+            "adjoint_ms": round((t2 - t1) * 1e3, 3), "adjoint_sweeps": prob.last_backward[0],
+            "adjoint_over_forward": round((t2 - t1) / (t1 - t0), 3)}
+
+
 def snapshot(pop, ox):
     return {name: getattr(pop, name).cpu().numpy().copy() for name in
             ("position", "radius", "cell_state", "cell_type", "age", "oxygen_local", "glucose_local", "velocity",
@@ -135,6 +167,7 @@ def main() -> int:
     ap.add_argument("--workload", default="spheroid_100k_161")
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
+    ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (counter calibration)")
     args = ap.parse_args()
@@ -185,6 +218,9 @@ def main() -> int:
     roof = None if args.no_roofline else sor_roofline(ox, geom, device)
     if roof:
         log(f"  sor fused sweep: {roof['launch_us']} us, {roof['achieved']} GB/s algorithmic")
+    adj = None if args.no_adjoint else adjoint_measurement(device)
+    if adj:
+        log(f"  adjoint: {adj}")
     cpu = None
     if snap is not None:
         cpu = cpu_baseline(snap, w, geom, params, settings, lifecycle, contact, stepping)
@@ -211,6 +247,7 @@ def main() -> int:
                    "final_residual": reports[-1].field_residual},
         "roofline": roof,
         "sor_autotune": tune,
+        "adjoint": adj,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

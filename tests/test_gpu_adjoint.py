@@ -33,7 +33,7 @@ def test_gradients_match_the_fp64_adjoint(oracle, device):
     D, Ob, K = W.D_UM2_PER_H, 40.0, 1.0
     prob = OxygenAdjointProblem(geom, torch.from_numpy(pos), K, W.UPTAKE_MMHG_UM3_PER_H,
                                 SolverSettings(omega=1.8, tolerance=1e-6, max_sweeps=20000, check_every=10),
-                                AdjointSettings(tolerance=1e-7), device=device)
+                                AdjointSettings(), device=device)
     S = prob.forward(torch.from_numpy(rates), D, Ob)
     gq, gD, gOb = prob.backward(2.0 * (S - torch.from_numpy(targets).to(device)))
     f64, Q64, S64, rep = oracle.forward_f64(pos, rates.astype(np.float64), D, Ob, K, geom.origin, geom.dx, geom.shape)
@@ -91,8 +91,9 @@ def test_full_size_gradients_against_central_differences(device):
         eps = 1e-3 * base
         fd = (fn(eps) - fn(-eps)) / (2 * eps)
         errs[name] = abs(fd - grad) / abs(fd)
-    # Directional central difference over all 100k rates: direction v = +-1, step 1e-3 relative.
-    v = torch.from_numpy(np.random.default_rng(5).choice([-1.0, 1.0], w.n).astype(np.float32)).to(device)
+    # Directional central difference over all 100k rates (step 1e-3 relative, positive random
+    # weights so that the contributions add up instead of cancelling below the FP32 solve's noise).
+    v = torch.from_numpy(np.random.default_rng(5).uniform(0.5, 1.5, w.n).astype(np.float32)).to(device)
     dq = 1e-3 * q * v
     fd = (loss(q + dq, D, Ob)[0] - loss(q - dq, D, Ob)[0]) / 2.0
     adj = float((gq.double() * dq.double()).sum())

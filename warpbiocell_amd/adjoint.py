@@ -22,6 +22,7 @@ nodes (the symmetric operator and the face term of dL/dO_b assume both).
 from __future__ import annotations
 
 import ctypes
+import math
 from dataclasses import dataclass
 
 import torch
@@ -33,13 +34,17 @@ from warpbiocell_amd.fields.scalar_field import GridGeometry, ScalarField
 
 @dataclass(frozen=True)
 class AdjointSettings:
-    omega: float = 1.8
+    """Red-black SOR on A mu = g. ``omega=None`` takes the optimal value of the Dirichlet Laplacian,
+    2 / (1 + sin(pi / (n - 1))) for the largest axis (the diagonal shift only lowers the spectral
+    radius); the stopping test is max|A mu - g| / max|g|, both scaled by the diagonal."""
+
+    omega: float | None = None
     max_sweeps: int = 20000
-    tolerance: float = 1.0e-6  # max |A mu - g| / max |g|, both scaled by the diagonal
+    tolerance: float = 1.0e-5
     check_every: int = 50
 
     def __post_init__(self):
-        if not 0.0 < self.omega < 2.0:
+        if self.omega is not None and not 0.0 < self.omega < 2.0:
             raise ValueError("omega must lie in (0, 2)")
         if self.max_sweeps < 1 or self.check_every < 1 or self.tolerance <= 0.0:
             raise ValueError("max_sweeps, check_every and tolerance must be positive")
@@ -116,10 +121,11 @@ class OxygenAdjointProblem:
                   g.origin[2], 1.0 / g.dx, nx, ny, nz, self._g.data_ptr(), s)
         h = g.dx ** 2 / self._diffusion
         a = self.adjoint_settings
+        omega = a.omega if a.omega is not None else 2.0 / (1.0 + math.sin(math.pi / (max(g.shape) - 1)))
         grad_rate = torch.empty(self.n, dtype=torch.float32, device=self.device)
         sums = (ctypes.c_double * 2)()
         sweeps, residual = ctypes.c_int32(), ctypes.c_float()
-        _lib.call("wbc_adjoint_solve", a.omega, a.max_sweeps, a.tolerance, a.check_every, h, self.q_ref,
+        _lib.call("wbc_adjoint_solve", omega, a.max_sweeps, a.tolerance, a.check_every, h, self.q_ref,
                   self.michaelis_k, None, self.field.values.data_ptr(), self.rate_density.data_ptr(), None,
                   self._g.data_ptr(), self.mu.data_ptr(), self.positions.data_ptr(), self.n, g.origin[0], g.origin[1],
                   g.origin[2], g.dx, nx, ny, nz, grad_rate.data_ptr(), ctypes.byref(sums), ctypes.byref(sweeps),
