@@ -1,0 +1,157 @@
+// Per-cell and per-node operations shared by the single-population kernels and the batched
+// replica kernels, so that both evaluate the reference's expressions identically.
+#pragma once
+#include "wbc_common.hpp"
+
+namespace wbc {
+
+// deposit_trilinear for one cell (kernels/field_kernels.py:86-116): 8 int64 atomics of
+// int(w * 2^24 + 0.5) into the (4, nx, ny, nz) accumulator, layer = cell state.
+__device__ __forceinline__ void deposit_cell(const float* __restrict__ position, int64_t c, int32_t s, float ox,
+                                             float oy, float oz, float inv_dx, int32_t nx, int32_t ny, int32_t nz,
+                                             unsigned long long* __restrict__ acc)
+{
+    if (s < 0 || s >= 4) return;
+    float ux = (position[3 * c + 0] - ox) * inv_dx;
+    float uy = (position[3 * c + 1] - oy) * inv_dx;
+    float uz = (position[3 * c + 2] - oz) * inv_dx;
+    int32_t i0 = (int32_t)floorf(ux), j0 = (int32_t)floorf(uy), k0 = (int32_t)floorf(uz);
+    if (i0 < 0 || j0 < 0 || k0 < 0) return;
+    if (i0 + 1 >= nx || j0 + 1 >= ny || k0 + 1 >= nz) return;
+    float fx = ux - (float)i0, fy = uy - (float)j0, fz = uz - (float)k0;
+    const int64_t plane = (int64_t)ny * nz;
+    unsigned long long* base = acc + (int64_t)s * nx * plane + i0 * plane + (int64_t)j0 * nz + k0;
+#pragma unroll
+    for (int di = 0; di < 2; ++di) {
+        float wx = di == 0 ? 1.0f - fx : fx;
+#pragma unroll
+        for (int dj = 0; dj < 2; ++dj) {
+            float wy = dj == 0 ? 1.0f - fy : fy;
+#pragma unroll
+            for (int dk = 0; dk < 2; ++dk) {
+                float wz = dk == 0 ? 1.0f - fz : fz;
+                float w = wx * wy * wz;
+                long long q = (long long)(int32_t)(w * kFixedOne + 0.5f);
+                atomicAdd(base + di * plane + dj * nz + dk, (unsigned long long)q);
+            }
+        }
+    }
+}
+
+// fixed_point_to_density + combine_densities for node t of a grid of `nodes` nodes
+// (kernels/field_kernels.py:119-142).
+__device__ __forceinline__ float finalize_node(const long long* __restrict__ acc, int64_t nodes, int64_t t,
+                                               float inv_vol, float w0, float w1, float w2, float w3,
+                                               float* __restrict__ per_state)
+{
+    float d0 = (float)acc[t] / kFixedOne * inv_vol;
+    float d1 = (float)acc[nodes + t] / kFixedOne * inv_vol;
+    float d2 = (float)acc[2 * nodes + t] / kFixedOne * inv_vol;
+    float d3 = (float)acc[3 * nodes + t] / kFixedOne * inv_vol;
+    if (per_state) {
+        per_state[t] = d0;
+        per_state[nodes + t] = d1;
+        per_state[2 * nodes + t] = d2;
+        per_state[3 * nodes + t] = d3;
+    }
+    return w0 * d0 + w1 * d1 + w2 * d2 + w3 * d3;
+}
+
+// sample_trilinear for one cell (kernels/field_kernels.py:162-182).
+__device__ __forceinline__ float sample_cell(const float* __restrict__ field, int32_t nx, int32_t ny, int32_t nz,
+                                             const float* __restrict__ position, int64_t c, float ox, float oy,
+                                             float oz, float inv_dx)
+{
+    float ux = (position[3 * c + 0] - ox) * inv_dx;
+    float uy = (position[3 * c + 1] - oy) * inv_dx;
+    float uz = (position[3 * c + 2] - oz) * inv_dx;
+    ux = fminf(fmaxf(ux, 0.0f), (float)(nx - 1));
+    uy = fminf(fmaxf(uy, 0.0f), (float)(ny - 1));
+    uz = fminf(fmaxf(uz, 0.0f), (float)(nz - 1));
+    int32_t i0 = min((int32_t)floorf(ux), nx - 2);
+    int32_t j0 = min((int32_t)floorf(uy), ny - 2);
+    int32_t k0 = min((int32_t)floorf(uz), nz - 2);
+    float fx = ux - (float)i0, fy = uy - (float)j0, fz = uz - (float)k0;
+    const int64_t plane = (int64_t)ny * nz;
+    const float* f = field + i0 * plane + (int64_t)j0 * nz + k0;
+    float c00 = f[0] * (1.0f - fx) + f[plane] * fx;
+    float c10 = f[nz] * (1.0f - fx) + f[plane + nz] * fx;
+    float c01 = f[1] * (1.0f - fx) + f[plane + 1] * fx;
+    float c11 = f[nz + 1] * (1.0f - fx) + f[plane + nz + 1] * fx;
+    float c0 = c00 * (1.0f - fy) + c10 * fy;
+    float c1 = c01 * (1.0f - fy) + c11 * fy;
+    return c0 * (1.0f - fz) + c1 * fz;
+}
+
+// ---- hash grid --------------------------------------------------------------------------------
+__device__ __forceinline__ int32_t wrap_dim(int32_t c, int32_t dim)
+{
+    int32_t m = c % dim;
+    return m < 0 ? m + dim : m;
+}
+
+__device__ __forceinline__ int32_t cell_coord(float p, float inv_cell) { return (int32_t)floorf(p * inv_cell); }
+
+__device__ __forceinline__ int32_t bucket_of(const float* __restrict__ position, int64_t i, float inv_cell,
+                                             int32_t dim)
+{
+    int32_t cx = wrap_dim(cell_coord(position[3 * i + 0], inv_cell), dim);
+    int32_t cy = wrap_dim(cell_coord(position[3 * i + 1], inv_cell), dim);
+    int32_t cz = wrap_dim(cell_coord(position[3 * i + 2], inv_cell), dim);
+    return (cx * dim + cy) * dim + cz;
+}
+
+// contact_velocities for the point at bucket-ordered slot t (kernels/mechanics_kernels.py:36-57):
+// buckets x, then y, then z around the point, each in slot order. `start` holds the dim^3 + 1
+// bucket starts of this grid and `xyzr` its bucket-ordered (x, y, z, r); both are indexed
+// relative to `base` (the grid's first slot in a packed array).
+__device__ __forceinline__ void contact_gather(const int32_t* __restrict__ start, const float4* __restrict__ xyzr,
+                                               int32_t t, float inv_cell, int32_t dim, float query_radius,
+                                               float rate, float& vx, float& vy, float& vz, int32_t& count)
+{
+    const float4 P = xyzr[t];
+    vx = 0.0f;
+    vy = 0.0f;
+    vz = 0.0f;
+    count = 0;
+    const int32_t x0 = cell_coord(P.x - query_radius, inv_cell), x1 = cell_coord(P.x + query_radius, inv_cell);
+    const int32_t y0 = cell_coord(P.y - query_radius, inv_cell), y1 = cell_coord(P.y + query_radius, inv_cell);
+    const int32_t z0 = cell_coord(P.z - query_radius, inv_cell), z1 = cell_coord(P.z + query_radius, inv_cell);
+    const int32_t wz0 = wrap_dim(z0, dim);
+    const bool z_contiguous = wz0 + (z1 - z0) < dim;  // the z buckets are consecutive keys
+    for (int32_t cx = x0; cx <= x1; ++cx) {
+        for (int32_t cy = y0; cy <= y1; ++cy) {
+            const int32_t row = (wrap_dim(cx, dim) * dim + wrap_dim(cy, dim)) * dim;
+            for (int32_t cz = z0; cz <= z1; ++cz) {
+                int32_t s, e;
+                if (z_contiguous) {
+                    s = start[row + wz0];
+                    e = start[row + wz0 + (z1 - z0) + 1];
+                    cz = z1;  // one contiguous run covers every z bucket of this row
+                } else {
+                    int32_t b = row + wrap_dim(cz, dim);
+                    s = start[b];
+                    e = start[b + 1];
+                }
+                for (int32_t u = s; u < e; ++u) {
+                    if (u == t) continue;
+                    const float4 Q = xyzr[u];
+                    float dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
+                    float d = sqrtf(dx * dx + dy * dy + dz * dz);
+                    if (d < query_radius) {
+                        count += 1;
+                        float overlap = P.w + Q.w - d;
+                        if (overlap > 0.0f && d > kCoincidentEps) {
+                            float coef = rate * overlap / d;
+                            vx = vx + coef * dx;
+                            vy = vy + coef * dy;
+                            vz = vz + coef * dz;
+                        }
+                    }
+                }
+            }
+        }
+    }
+}
+
+}  // namespace wbc

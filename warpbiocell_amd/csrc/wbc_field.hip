@@ -7,6 +7,7 @@
 // Grid arrays are C order (i, j, k) with k contiguous, exactly the reference's Warp 3-D
 // arrays; the deposit accumulator is (state, i, j, k) int64 like CellDensities.
 #include "wbc_field.hpp"
+#include "wbc_cellops.hpp"
 
 namespace wbc {
 
@@ -22,32 +23,7 @@ __global__ void __launch_bounds__(256) deposit_trilinear_kernel(const float* __r
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    int32_t s = cell_state[c];
-    if (s < 0 || s >= 4) return;
-    float ux = (position[3 * c + 0] - ox) * inv_dx;
-    float uy = (position[3 * c + 1] - oy) * inv_dx;
-    float uz = (position[3 * c + 2] - oz) * inv_dx;
-    int32_t i0 = (int32_t)floorf(ux), j0 = (int32_t)floorf(uy), k0 = (int32_t)floorf(uz);
-    if (i0 < 0 || j0 < 0 || k0 < 0) return;
-    if (i0 + 1 >= nx || j0 + 1 >= ny || k0 + 1 >= nz) return;
-    float fx = ux - (float)i0, fy = uy - (float)j0, fz = uz - (float)k0;
-    const int64_t plane = (int64_t)ny * nz;
-    unsigned long long* base = acc + (int64_t)s * nx * plane + i0 * plane + (int64_t)j0 * nz + k0;
-#pragma unroll
-    for (int di = 0; di < 2; ++di) {
-        float wx = di == 0 ? 1.0f - fx : fx;
-#pragma unroll
-        for (int dj = 0; dj < 2; ++dj) {
-            float wy = dj == 0 ? 1.0f - fy : fy;
-#pragma unroll
-            for (int dk = 0; dk < 2; ++dk) {
-                float wz = dk == 0 ? 1.0f - fz : fz;
-                float w = wx * wy * wz;
-                long long q = (long long)(int32_t)(w * kFixedOne + 0.5f);
-                atomicAdd(base + di * plane + dj * nz + dk, (unsigned long long)q);
-            }
-        }
-    }
+    deposit_cell(position, c, cell_state[c], ox, oy, oz, inv_dx, nx, ny, nz, acc);
 }
 
 __global__ void __launch_bounds__(256) zero_u64_kernel(unsigned long long* __restrict__ p, int64_t count)
@@ -65,17 +41,8 @@ __global__ void __launch_bounds__(256) density_finalize_kernel(const long long* 
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nodes) return;
-    float d0 = (float)acc[t] / kFixedOne * inv_vol;
-    float d1 = (float)acc[nodes + t] / kFixedOne * inv_vol;
-    float d2 = (float)acc[2 * nodes + t] / kFixedOne * inv_vol;
-    float d3 = (float)acc[3 * nodes + t] / kFixedOne * inv_vol;
-    if (per_state) {
-        per_state[t] = d0;
-        per_state[nodes + t] = d1;
-        per_state[2 * nodes + t] = d2;
-        per_state[3 * nodes + t] = d3;
-    }
-    if (combined) combined[t] = w0 * d0 + w1 * d1 + w2 * d2 + w3 * d3;
+    const float v = finalize_node(acc, nodes, t, inv_vol, w0, w1, w2, w3, per_state);
+    if (combined) combined[t] = v;
 }
 
 __global__ void __launch_bounds__(256) combine_kernel(const float* __restrict__ per_state, int64_t nodes, float w0,
@@ -98,25 +65,7 @@ __global__ void __launch_bounds__(256) sample_trilinear_kernel(const float* __re
 {
     int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (c >= n) return;
-    float ux = (position[3 * c + 0] - ox) * inv_dx;
-    float uy = (position[3 * c + 1] - oy) * inv_dx;
-    float uz = (position[3 * c + 2] - oz) * inv_dx;
-    ux = fminf(fmaxf(ux, 0.0f), (float)(nx - 1));
-    uy = fminf(fmaxf(uy, 0.0f), (float)(ny - 1));
-    uz = fminf(fmaxf(uz, 0.0f), (float)(nz - 1));
-    int32_t i0 = min((int32_t)floorf(ux), nx - 2);
-    int32_t j0 = min((int32_t)floorf(uy), ny - 2);
-    int32_t k0 = min((int32_t)floorf(uz), nz - 2);
-    float fx = ux - (float)i0, fy = uy - (float)j0, fz = uz - (float)k0;
-    const int64_t plane = (int64_t)ny * nz;
-    const float* f = field + i0 * plane + (int64_t)j0 * nz + k0;
-    float c00 = f[0] * (1.0f - fx) + f[plane] * fx;
-    float c10 = f[nz] * (1.0f - fx) + f[plane + nz] * fx;
-    float c01 = f[1] * (1.0f - fx) + f[plane + 1] * fx;
-    float c11 = f[nz + 1] * (1.0f - fx) + f[plane + nz + 1] * fx;
-    float c0 = c00 * (1.0f - fy) + c10 * fy;
-    float c1 = c01 * (1.0f - fy) + c11 * fy;
-    out[c] = c0 * (1.0f - fz) + c1 * fz;
+    out[c] = sample_cell(field, nx, ny, nz, position, c, ox, oy, oz, inv_dx);
 }
 
 // --------------------------------------------------------------------------------------------

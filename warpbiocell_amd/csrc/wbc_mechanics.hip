@@ -16,6 +16,7 @@
 // fastest; a query walks x, then y, then z buckets, and each bucket in slot order. That fixes
 // the FP32 summation order of the gather, so the result does not depend on thread timing.
 #include "wbc_scan.hpp"
+#include "wbc_cellops.hpp"
 
 namespace wbc {
 
@@ -40,24 +41,13 @@ static HashGridLayout hashgrid_layout(int32_t dim, int64_t capacity)
     return L;
 }
 
-__device__ __forceinline__ int32_t wrap_dim(int32_t c, int32_t dim)
-{
-    int32_t m = c % dim;
-    return m < 0 ? m + dim : m;
-}
-
-__device__ __forceinline__ int32_t cell_coord(float p, float inv_cell) { return (int32_t)floorf(p * inv_cell); }
-
 __global__ void __launch_bounds__(256) hg_key_count(const float* __restrict__ position, int32_t n, float inv_cell,
                                                      int32_t dim, int32_t* __restrict__ key,
                                                      int32_t* __restrict__ cnt)
 {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    int32_t cx = wrap_dim(cell_coord(position[3 * i + 0], inv_cell), dim);
-    int32_t cy = wrap_dim(cell_coord(position[3 * i + 1], inv_cell), dim);
-    int32_t cz = wrap_dim(cell_coord(position[3 * i + 2], inv_cell), dim);
-    int32_t b = (cx * dim + cy) * dim + cz;
+    int32_t b = bucket_of(position, i, inv_cell, dim);
     key[i] = b;
     atomicAdd(cnt + b, 1);
 }
@@ -110,47 +100,10 @@ __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict_
     int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int32_t i = id[t];
+    float vx, vy, vz;
+    int32_t count;
+    contact_gather(start, xyzr, t, inv_cell, dim, query_radius, rate, vx, vy, vz, count);
     const float4 P = xyzr[t];
-    float vx = 0.0f, vy = 0.0f, vz = 0.0f;
-    int32_t count = 0;
-    const int32_t x0 = cell_coord(P.x - query_radius, inv_cell), x1 = cell_coord(P.x + query_radius, inv_cell);
-    const int32_t y0 = cell_coord(P.y - query_radius, inv_cell), y1 = cell_coord(P.y + query_radius, inv_cell);
-    const int32_t z0 = cell_coord(P.z - query_radius, inv_cell), z1 = cell_coord(P.z + query_radius, inv_cell);
-    const int32_t wz0 = wrap_dim(z0, dim);
-    const bool z_contiguous = wz0 + (z1 - z0) < dim;  // the z buckets are consecutive keys
-    for (int32_t cx = x0; cx <= x1; ++cx) {
-        for (int32_t cy = y0; cy <= y1; ++cy) {
-            const int32_t row = (wrap_dim(cx, dim) * dim + wrap_dim(cy, dim)) * dim;
-            for (int32_t cz = z0; cz <= z1; ++cz) {
-                int32_t s, e;
-                if (z_contiguous) {
-                    s = start[row + wz0];
-                    e = start[row + wz0 + (z1 - z0) + 1];
-                    cz = z1;  // one contiguous run covers every z bucket of this row
-                } else {
-                    int32_t b = row + wrap_dim(cz, dim);
-                    s = start[b];
-                    e = start[b + 1];
-                }
-                for (int32_t u = s; u < e; ++u) {
-                    if (u == t) continue;
-                    const float4 Q = xyzr[u];
-                    float dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
-                    float d = sqrtf(dx * dx + dy * dy + dz * dz);
-                    if (d < query_radius) {
-                        count += 1;
-                        float overlap = P.w + Q.w - d;
-                        if (overlap > 0.0f && d > kCoincidentEps) {
-                            float coef = rate * overlap / d;
-                            vx = vx + coef * dx;
-                            vy = vy + coef * dy;
-                            vz = vz + coef * dz;
-                        }
-                    }
-                }
-            }
-        }
-    }
     velocity[3 * i + 0] = vx;
     velocity[3 * i + 1] = vy;
     velocity[3 * i + 2] = vz;
