@@ -18,6 +18,7 @@ workload from the exact state the GPU's first timed step started from.
 from __future__ import annotations
 
 import argparse
+import ctypes
 import json
 import os
 import sys
@@ -28,6 +29,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
+METRIC = "cell-steps/s @100k cells,161\u00b3 SOR; batched-sweep & adjoint speedup vs plain"
 
 
 def log(msg: str) -> None:
@@ -57,7 +59,23 @@ def setup(workload_name: str, device):
     return w, geom, params, settings, lifecycle, contact, stepping, pop, ox, grid
 
 
-def sor_roofline(ox, geom, device, launches: int = 200) -> dict:
+KERNEL_OF_KIND = {"tile": "sor_fused_kernel<{tj}, {ti}>", "march": "sor_march_kernel<{tj}, {ti}>"}
+COUNTER_TABLE = ROOT / "profiles" / "sor_counters_spheroid_100k_161.md"
+
+
+def measured_traffic(shape: str, workload: str):
+    """HBM bytes per launch of tile shape `shape` from the committed FETCH_SIZE / WRITE_SIZE table
+    (scaled by the copy calibration), or None when the table does not cover this workload/shape."""
+    if workload != "spheroid_100k_161" or not COUNTER_TABLE.exists():
+        return None
+    for line in COUNTER_TABLE.read_text().splitlines():
+        cells = [c.strip() for c in line.strip().strip("|").split("|")]
+        if len(cells) >= 6 and cells[0] == shape:
+            return int(cells[5])
+    return None
+
+
+def sor_roofline(ox, geom, device, workload: str, launches: int = 200) -> dict:
     """Average duration of one fused SOR sweep launch, measured with HIP events on its stream.
 
     Algorithmic bytes of one red+black sweep of the fused kernel: the field read once, the
@@ -89,10 +107,16 @@ def sor_roofline(ox, geom, device, launches: int = 200) -> dict:
     per_launch_s = start.elapsed_time(end) / 1e3 / launches
     bytes_per_launch = 12 * geom.node_count
     achieved = bytes_per_launch / per_launch_s / 1e9
+    cfg = _lib.query("wbc_sor_fused_get_config")
+    kind, tj, ti = (ctypes.c_int32() for _ in range(3))
+    _lib.call("wbc_sor_fused_config_shape", cfg, ctypes.byref(kind), ctypes.byref(tj), ctypes.byref(ti))
+    family = "march" if kind.value == 1 else "tile"
+    shape = f"march60x{tj.value}/{ti.value}" if family == "march" else f"tile60x{tj.value}x{ti.value}"
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": None,
-            "kernel": "sor_fused_kernel", "launch_us": round(per_launch_s * 1e6, 3),
-            "algorithmic_bytes_per_launch": bytes_per_launch}
+            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(shape, workload),
+            "kernel": KERNEL_OF_KIND[family].format(tj=tj.value, ti=ti.value), "shape": shape,
+            "launch_us": round(per_launch_s * 1e6, 3), "algorithmic_bytes_per_launch": bytes_per_launch,
+            "traffic_source": str(COUNTER_TABLE.relative_to(ROOT))}
 
 
 def adjoint_measurement(device) -> dict:
@@ -215,7 +239,7 @@ def main() -> int:
     log(f"  timed: {elapsed:.3f} s, sweeps/step mean {sum(sweeps) / len(sweeps):.1f} "
         f"(min {min(sweeps)}, max {max(sweeps)}), unconverged steps {sum(not r.field_converged for r in reports)}")
 
-    roof = None if args.no_roofline else sor_roofline(ox, geom, device)
+    roof = None if args.no_roofline else sor_roofline(ox, geom, device, w.name)
     if roof:
         log(f"  sor fused sweep: {roof['launch_us']} us, {roof['achieved']} GB/s algorithmic")
     adj = None if args.no_adjoint else adjoint_measurement(device)
@@ -227,7 +251,7 @@ def main() -> int:
         log(f"  cpu baseline: {cpu['value']} cell-steps/s ({cpu['seconds']} s, {cpu['sweeps']} sweeps)")
 
     result = {
-        "metric": "cell-steps/s @100k cells,161^3 SOR",
+        "metric": METRIC,
         "value": round(cell_steps / elapsed, 1),
         "unit": "cell-steps/s",
         "n_gpus": 1,
