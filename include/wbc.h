@@ -189,6 +189,87 @@ int wbc_adjoint_solve(float omega, int32_t max_sweeps, float tolerance, int32_t 
                       int32_t* sweeps_out, float* residual_out, void* workspace, size_t workspace_bytes,
                       wbc_stream_t stream);
 
+/* ---- batched replica sweep -----------------------------------------------------------------
+ * No reference counterpart: the reference runs a parameter sweep (simulation/sweep.py:115-162) as
+ * sequential Experiments, each calling cell_step (simulation/simulator.py:58-81) on its own
+ * population and oxygen field. Here up to WBC_BATCH_MAX_REPLICAS replicas with their own cell
+ * counts, grids and parameters advance together: one cell_step of every replica is
+ *   deposit_combined -> steady-state solve -> sample  (fields/species.py update, oxygen weights)
+ *   -> decide_lifecycle (no divisions: division_rate must be 0)
+ *   -> `substeps` x (hash-grid build, contact gather, x += v dt)   (relax_contacts, no region).
+ * The solves of all replicas run in ONE persistent launch: every phase sweeps every replica that
+ * has not yet converged, separated by a grid barrier (bounded wait; a timeout sets the error flag
+ * read by wbc_batch_report, it never hangs). Each replica's residual is checked on the reference
+ * schedule (every check_every sweeps and at max_sweeps), computed inside the next sweep from the
+ * values it stages in LDS; its field, sweep count and residual are bit-identical to a serial
+ * wbc_solve_steady_state of that replica. wbc_batch_step captures the whole step in a hipGraph
+ * on first use (use_graph = 1) and replays it; use_graph = 0 enqueues the same kernels directly.
+ *
+ * Packing: replica r owns cell slots [cell_offset, cell_offset + capacity) of every per-cell array
+ * (cell_offset = sum of the capacities before it) and field nodes [node_offset, node_offset +
+ * nx*ny*nz) of `field` (node_offset = sum of the node counts before it). All faces are Dirichlet. */
+#define WBC_BATCH_MAX_REPLICAS 64
+
+typedef struct wbc_replica {
+    int32_t cell_offset, capacity, count;   /* slots; `count` active cells (the prefix) */
+    int32_t nx, ny, nz;                     /* oxygen grid */
+    int64_t node_offset;
+    float origin[3], inv_dx, inv_voxel_volume;
+    float weight[4];                        /* uptake weight per cell state (species.py uptake_weights) */
+    float omega, dx2_over_D, uptake_max, michaelis_k, reference, tolerance; /* diffusion.py:79-123 */
+    int32_t max_sweeps, check_every;
+    float dt_cells, death_rate, anoxic_death_rate, division_rate;            /* lifecycle.py:96-132 */
+    int32_t inhibition_threshold;
+    float hypoxia_threshold, death_threshold, glucose_threshold, glucose_death_threshold;
+    int32_t necrosis_requires_glucose, phenotype_model;
+    float apoptosis_rate, necrosis_rate;
+    float cell_size, query_radius, contact_rate, dt_mechanics;              /* mechanics.py:179-189 */
+    int32_t substeps;
+} wbc_replica;
+
+/* Packed per-cell arrays (cells/state.py field names), each of sum(capacity) entries. */
+typedef struct wbc_batch_cells {
+    float* position;          /* (slots, 3) */
+    float* radius;
+    int32_t* cell_state;
+    float* age;
+    float* oxygen_local;
+    float* glucose_local;
+    float* velocity;          /* (slots, 3) */
+    int32_t* neighbor_count;
+    int32_t* divide_flag;
+    int32_t* division_offset;
+    int32_t* fate_flags;
+    uint32_t* rng_state;
+} wbc_batch_cells;
+
+typedef struct wbc_batch* wbc_batch_t;
+
+/* sizeof(wbc_replica) (which = 0) or sizeof(wbc_batch_cells) (which = 1), for bindings to check
+ * their struct layout against. */
+size_t wbc_batch_struct_size(int32_t which);
+
+size_t wbc_batch_workspace_bytes(const wbc_replica* replicas, int32_t count, int32_t hash_dim);
+/* Validates the replicas, copies their descriptors into the workspace and zeroes the hash-grid
+ * counts; `cells`, `field` and `workspace` must outlive the handle. Allocates the handle (host). */
+int wbc_batch_create(const wbc_replica* replicas, int32_t count, int32_t hash_dim, const wbc_batch_cells* cells,
+                     float* field, void* workspace, size_t workspace_bytes, wbc_stream_t stream, wbc_batch_t* out);
+/* Schedule of the persistent solve: tile shape `tile` in [0, wbc_batch_schedule_count()) (60 x TJ x TI
+ * nodes, see wbc_batch_schedule_shape) and workgroups per CU (0: as many as fit). Resets the graph. */
+int wbc_batch_schedule_count(void);
+int wbc_batch_schedule_shape(int32_t tile, int32_t* tj, int32_t* ti, int32_t* max_blocks_per_cu);
+int wbc_batch_set_schedule(wbc_batch_t batch, int32_t tile, int32_t blocks_per_cu);
+/* `steps` cell steps of every replica, enqueued on `stream`. */
+int wbc_batch_step(wbc_batch_t batch, int32_t steps, int32_t use_graph, wbc_stream_t stream);
+/* Only the steady-state solves of every replica (one persistent launch), from the current fields. */
+int wbc_batch_solve(wbc_batch_t batch, wbc_stream_t stream);
+/* Synchronizes `stream`, then copies the last solve's report per replica (any pointer may be NULL):
+ * sweeps, residual, converged, sweeps summed over every solve since creation; *error is nonzero
+ * when a grid barrier timed out (results of that launch are invalid). */
+int wbc_batch_report(wbc_batch_t batch, wbc_stream_t stream, int32_t* sweeps, float* residual, int32_t* converged,
+                     int64_t* total_sweeps, int32_t* error);
+int wbc_batch_destroy(wbc_batch_t batch);
+
 #ifdef __cplusplus
 }
 #endif

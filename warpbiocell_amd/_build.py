@@ -24,6 +24,7 @@ from pathlib import Path
 
 PACKAGE_DIR = Path(__file__).resolve().parent
 CSRC = PACKAGE_DIR / "csrc"
+INCLUDE = PACKAGE_DIR.parent / "include"
 LIBRARY = PACKAGE_DIR / "libwbc_hip.so"
 SOURCES = ("wbc_scan.hip", "wbc_cells.hip", "wbc_field.hip", "wbc_sor_fused.hip", "wbc_sor_march.hip", "wbc_mechanics.hip",
            "wbc_batched.hip", "wbc_adjoint.hip")
@@ -68,7 +69,7 @@ def build_library(verbose: bool = False) -> Path:
 
         def compile_one(src: Path) -> Path:
             obj = Path(tmp) / (src.stem + ".o")
-            cmd = [hipcc, *HIP_FLAGS, "-I", str(CSRC), "-c", str(src), "-o", str(obj)]
+            cmd = [hipcc, *HIP_FLAGS, "-I", str(CSRC), "-I", str(INCLUDE), "-c", str(src), "-o", str(obj)]
             res = subprocess.run(cmd, capture_output=True, text=True)
             out = "\n".join(line for line in (res.stdout + res.stderr).splitlines() if "packed-fp32-ops" not in line)
             if res.returncode != 0:

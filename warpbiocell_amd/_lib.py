@@ -59,6 +59,16 @@ SIGNATURES: dict[str, tuple] = {
     "wbc_adjoint_workspace_bytes": (c_size_t, [I32, I32, I32]),
     "wbc_adjoint_solve": (I32, [F32, I32, F32, I32, F32, F32, F32, P, P, P, P, P, P, P, I64, F32, F32, F32, F32,
                                 I32, I32, I32, P, P, P, P, P, c_size_t, P]),
+    "wbc_batch_struct_size": (c_size_t, [I32]),
+    "wbc_batch_workspace_bytes": (c_size_t, [P, I32, I32]),
+    "wbc_batch_create": (I32, [P, I32, I32, P, P, P, c_size_t, P, P]),
+    "wbc_batch_schedule_count": (I32, []),
+    "wbc_batch_schedule_shape": (I32, [I32, P, P, P]),
+    "wbc_batch_set_schedule": (I32, [P, I32, I32]),
+    "wbc_batch_step": (I32, [P, I32, I32, P]),
+    "wbc_batch_solve": (I32, [P, P]),
+    "wbc_batch_report": (I32, [P, P, P, P, P, P, P]),
+    "wbc_batch_destroy": (I32, [P]),
 }
 
 _lib = None

@@ -154,4 +154,91 @@ __device__ __forceinline__ void contact_gather(const int32_t* __restrict__ start
     }
 }
 
+__device__ __forceinline__ float oxygen_factor(float o, float hypoxia_threshold, float death_threshold)
+{
+    if (o >= hypoxia_threshold) return 1.0f;
+    if (o <= death_threshold) return 0.0f;
+    return (o - death_threshold) / (hypoxia_threshold - death_threshold);
+}
+
+struct LifecycleArgs {
+    float dt, death_rate, anoxic_death_rate, division_rate;
+    int32_t inhibition_threshold;
+    float hypoxia_threshold, death_threshold, glucose_threshold, glucose_death_threshold;
+    int32_t necrosis_requires_glucose, phenotype_model;
+    float apoptosis_rate, necrosis_rate;
+};
+
+// hash_grid build step 4 for the point at scatter position t: slot order inside its bucket
+// (rank = number of smaller slot ids in the bucket) and the bucket-ordered (x, y, z, r) copy.
+__device__ __forceinline__ void hg_order_point(const int32_t* __restrict__ key, const int32_t* __restrict__ start,
+                                               const int32_t* __restrict__ tmp, const float* __restrict__ position,
+                                               const float* __restrict__ radius, int32_t t, int32_t* __restrict__ id,
+                                               float4* __restrict__ xyzr)
+{
+    const int32_t me = tmp[t];
+    const int32_t b = key[me];
+    const int32_t s = start[b], e = start[b + 1];
+    int32_t rank = 0;
+    for (int32_t u = s; u < e; ++u) rank += tmp[u] < me ? 1 : 0;
+    id[s + rank] = me;
+    xyzr[s + rank] = make_float4(position[3 * me + 0], position[3 * me + 1], position[3 * me + 2], radius[me]);
+}
+
+// One cell's lifecycle decision (kernels/cell_kernels.py:77-154): death draw first (always
+// consumed), then state, then the division draw for cells that may divide.
+__device__ __forceinline__ void lifecycle_decide_cell(const LifecycleArgs& a, int64_t i, int32_t* __restrict__ cell_state,
+                                                      float* __restrict__ age,
+                                                      const int32_t* __restrict__ neighbor_count,
+                                                      const float* __restrict__ oxygen_local,
+                                                      const float* __restrict__ glucose_local,
+                                                      const int32_t* __restrict__ fate_flags,
+                                                      uint32_t* __restrict__ rng_state,
+                                                      int32_t* __restrict__ divide_flag)
+{
+    divide_flag[i] = 0;
+    if (cell_state[i] == kStateDead) return;
+    age[i] = age[i] + a.dt;
+    uint32_t rng = rng_state[i];
+    float o = oxygen_local[i];
+    float g = glucose_local[i];
+    int32_t flags = a.phenotype_model == 1 ? fate_flags[i] : 0;
+
+    // Death first; its draw is always consumed (cell_kernels.py:115-132).
+    bool lethal = o < a.death_threshold;
+    if (a.necrosis_requires_glucose != 0) lethal = lethal && (g < a.glucose_death_threshold);
+    float rate = a.death_rate;
+    if (lethal) rate = fmaxf(a.death_rate, a.anoxic_death_rate);
+    if (flags & kFateNecrosis) rate = fmaxf(rate, a.necrosis_rate);
+    if (flags & kFateApoptosis) rate = fmaxf(rate, a.apoptosis_rate);
+    float p_death = 1.0f - exp_f32(-rate * a.dt);
+    if (randf(rng) < p_death) {
+        cell_state[i] = kStateDead;
+        rng_state[i] = rng;
+        return;
+    }
+
+    bool crowded = neighbor_count[i] >= a.inhibition_threshold;
+    bool may_divide = !crowded;
+    float factor = 1.0f;
+    if (a.phenotype_model == 1)
+        may_divide = may_divide && ((flags & kFateProliferation) != 0) && ((flags & kFateGrowthArrest) == 0);
+    else
+        factor = oxygen_factor(o, a.hypoxia_threshold, a.death_threshold) *
+                 oxygen_factor(g, a.glucose_threshold, a.glucose_death_threshold);
+
+    if (o < a.hypoxia_threshold)
+        cell_state[i] = kStateHypoxic;
+    else if (may_divide)
+        cell_state[i] = kStateProliferative;
+    else
+        cell_state[i] = kStateQuiescent;
+
+    if (may_divide) {
+        float p_divide = 1.0f - exp_f32(-a.division_rate * factor * a.dt);
+        if (randf(rng) < p_divide) divide_flag[i] = 1;
+    }
+    rng_state[i] = rng;
+}
+
 }  // namespace wbc

@@ -4,6 +4,7 @@
 // (lifecycle_decide), cells/lifecycle.py:146 + kernels/cell_kernels.py:157-200
 // (array_scan + place_daughters).
 #include "wbc_scan.hpp"
+#include "wbc_cellops.hpp"
 
 namespace wbc {
 
@@ -14,21 +15,6 @@ __global__ void __launch_bounds__(256) init_rng_kernel(uint32_t* __restrict__ rn
     if (i < capacity) rng_state[i] = pcg((uint32_t)seed + pcg((uint32_t)(int32_t)i));
 }
 
-__device__ __forceinline__ float oxygen_factor(float o, float hypoxia_threshold, float death_threshold)
-{
-    if (o >= hypoxia_threshold) return 1.0f;
-    if (o <= death_threshold) return 0.0f;
-    return (o - death_threshold) / (hypoxia_threshold - death_threshold);
-}
-
-struct LifecycleArgs {
-    float dt, death_rate, anoxic_death_rate, division_rate;
-    int32_t inhibition_threshold;
-    float hypoxia_threshold, death_threshold, glucose_threshold, glucose_death_threshold;
-    int32_t necrosis_requires_glucose, phenotype_model;
-    float apoptosis_rate, necrosis_rate;
-};
-
 __global__ void __launch_bounds__(256) lifecycle_decide_kernel(
     LifecycleArgs a, int32_t* __restrict__ cell_state, float* __restrict__ age,
     const int32_t* __restrict__ neighbor_count, const float* __restrict__ oxygen_local,
@@ -37,49 +23,8 @@ __global__ void __launch_bounds__(256) lifecycle_decide_kernel(
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n) return;
-    divide_flag[i] = 0;
-    if (cell_state[i] == kStateDead) return;
-    age[i] = age[i] + a.dt;
-    uint32_t rng = rng_state[i];
-    float o = oxygen_local[i];
-    float g = glucose_local[i];
-    int32_t flags = a.phenotype_model == 1 ? fate_flags[i] : 0;
-
-    // Death first; its draw is always consumed (cell_kernels.py:115-132).
-    bool lethal = o < a.death_threshold;
-    if (a.necrosis_requires_glucose != 0) lethal = lethal && (g < a.glucose_death_threshold);
-    float rate = a.death_rate;
-    if (lethal) rate = fmaxf(a.death_rate, a.anoxic_death_rate);
-    if (flags & kFateNecrosis) rate = fmaxf(rate, a.necrosis_rate);
-    if (flags & kFateApoptosis) rate = fmaxf(rate, a.apoptosis_rate);
-    float p_death = 1.0f - exp_f32(-rate * a.dt);
-    if (randf(rng) < p_death) {
-        cell_state[i] = kStateDead;
-        rng_state[i] = rng;
-        return;
-    }
-
-    bool crowded = neighbor_count[i] >= a.inhibition_threshold;
-    bool may_divide = !crowded;
-    float factor = 1.0f;
-    if (a.phenotype_model == 1)
-        may_divide = may_divide && ((flags & kFateProliferation) != 0) && ((flags & kFateGrowthArrest) == 0);
-    else
-        factor = oxygen_factor(o, a.hypoxia_threshold, a.death_threshold) *
-                 oxygen_factor(g, a.glucose_threshold, a.glucose_death_threshold);
-
-    if (o < a.hypoxia_threshold)
-        cell_state[i] = kStateHypoxic;
-    else if (may_divide)
-        cell_state[i] = kStateProliferative;
-    else
-        cell_state[i] = kStateQuiescent;
-
-    if (may_divide) {
-        float p_divide = 1.0f - exp_f32(-a.division_rate * factor * a.dt);
-        if (randf(rng) < p_divide) divide_flag[i] = 1;
-    }
-    rng_state[i] = rng;
+    lifecycle_decide_cell(a, i, cell_state, age, neighbor_count, oxygen_local, glucose_local, fate_flags, rng_state,
+                          divide_flag);
 }
 
 // The k-th dividing cell in slot order fills slot count + k (cell_kernels.py:179-180).

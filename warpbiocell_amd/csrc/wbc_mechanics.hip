@@ -71,13 +71,7 @@ __global__ void __launch_bounds__(256) hg_order(const int32_t* __restrict__ key,
 {
     int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
-    int32_t me = tmp[t];
-    int32_t b = key[me];
-    int32_t s = start[b], e = start[b + 1];
-    int32_t rank = 0;
-    for (int32_t u = s; u < e; ++u) rank += tmp[u] < me ? 1 : 0;
-    id[s + rank] = me;
-    xyzr[s + rank] = make_float4(position[3 * me + 0], position[3 * me + 1], position[3 * me + 2], radius[me]);
+    hg_order_point(key, start, tmp, position, radius, t, id, xyzr);
 }
 
 __global__ void __launch_bounds__(256) hg_set_total(int32_t* __restrict__ start, int64_t nb, int32_t n)

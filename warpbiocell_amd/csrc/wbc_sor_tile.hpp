@@ -50,14 +50,45 @@ __device__ __forceinline__ float sor_update(const SorParams& p, const float* __r
     return fmaxf(v, 0.0f);
 }
 
+// |r| / max(6 o, reference) of steady_state_residual (kernels/field_kernels.py:236-243) at the
+// node held in LDS at e (row strides kLK and kLK*LJ), t its linear index: the same expressions in
+// the same order as residual_node below, with the neighbours read from LDS.
+template <int LJ>
+__device__ __forceinline__ float residual_lds(const SorParams& p, const float* __restrict__ s, int e, int t,
+                                              float dens, bool q_finite)
+{
+    constexpr int SJ = kLK, SI = kLK * LJ;
+    const float o = s[e];
+    const float kpo = p.michaelis_k + o;
+    float c = 0.0f;
+    if (!(dens == 0.0f && kpo > 0.0f && q_finite)) c = dens * p.uptake_max / kpo;
+    float prod = 0.0f;
+    if (p.production) {
+        float sf = 1.0f;
+        if (p.has_source) {
+            float v = p.source[t];
+            sf = v / (p.source_k + v);
+        }
+        prod = p.production[t] * sf;
+    }
+    const float nb = s[e - SI] + s[e + SI] + s[e - SJ] + s[e + SJ] + s[e - 1] + s[e + 1];
+    const float r = nb + p.dx2_over_D * prod - (6.0f + p.dx2_over_D * c) * o;
+    return fabsf(r) / fmaxf(6.0f * o, p.reference);
+}
+
 // One red+black sweep of the tile at (k0, j0, i0), src -> dst, by a 256-thread workgroup; `s`
 // is the workgroup's LDS of SorTile<TJ, TI>::kRows * kLK floats. Called uniformly by the whole
 // workgroup. Grids are below 2^31 nodes, so node indices are int32 throughout.
-template <int TJ, int TI>
-__device__ __forceinline__ void sor_tile_body(const SorParams& p, const float* __restrict__ src,
-                                              float* __restrict__ dst, int k0, int j0, int i0, bool qf,
-                                              float* __restrict__ s)
+// kResidual (or, with kResidualFlag, the runtime `residual` argument): also return this thread's
+// maximum of the residual of `src` (the field before the sweep) over the updated nodes of the
+// tile, from the values already staged in LDS; 0 otherwise.
+template <int TJ, int TI, bool kResidual = false, bool kResidualFlag = false>
+__device__ __forceinline__ float sor_tile_body(const SorParams& p, const float* __restrict__ src,
+                                               float* __restrict__ dst, int k0, int j0, int i0, bool qf,
+                                               float* __restrict__ s, bool residual = false)
 {
+    const bool with_res = kResidual || (kResidualFlag && residual);
+    float rmax = 0.0f;
     using T = SorTile<TJ, TI>;
     constexpr int kLoadIters = (T::kRows + 3) / 4;      // LDS rows per wave
     constexpr int kRedIters = (T::kRingRows + 7) / 8;   // half-wave rows, red pass
@@ -121,6 +152,22 @@ __device__ __forceinline__ void sor_tile_body(const SorParams& p, const float* _
     }
     __syncthreads();
 
+    if ((kResidual || kResidualFlag) && with_res) {
+        // Black nodes of the tile, before the red pass changes their red neighbours.
+#pragma unroll
+        for (int m = 0; m < kBlackIters; ++m) {
+            const int rr = wave * 2 + half + 8 * m;
+            const int jj = rr % TJ, ii = rr / TJ;
+            const int gi = i0 + ii, gj = j0 + jj, gk0 = k0 + 2 * q;
+            if (!(rr < T::kRows2 && q < kTK / 2 && gi < nx && gj < ny && gk0 < nz)) continue;
+            const int par = ((gi + gj + k0) & 1) ^ 1;
+            const int t0 = gi * plane + gj * nz + gk0;
+            if (gi > 0 && gj > 0 && gi < nx - 1 && gj < ny - 1 && ok_blk[par] && !(p.fixed && p.fixed[t0 + par] != 0))
+                rmax = fmaxf(rmax, residual_lds<T::LJ>(p, s, ((ii + 2) * T::LJ + (jj + 2)) * kLK + (2 * q + 2) + par,
+                                                       t0 + par, dblack[m], qf));
+        }
+    }
+
     // Red pass ((i+j+k) even) over the tile and its 1-node ring, in place in LDS. Each half-wave
     // takes one row of 31 pairs (k0-1 .. k0+60).
 #pragma unroll
@@ -134,6 +181,11 @@ __device__ __forceinline__ void sor_tile_body(const SorParams& p, const float* _
         const int t = gi * plane + gj * nz + gk;
         if (p.fixed && p.fixed[t] != 0) continue;
         const int e = ((ii + 1) * T::LJ + (jj + 1)) * kLK + (gk - k0 + 2);
+        if ((kResidual || kResidualFlag) && with_res) {
+            // Red nodes of the tile (not of the ring), before their own update.
+            if (ii >= 1 && ii <= TI && jj >= 1 && jj <= TJ && gk >= k0 && gk < k0 + kTK)
+                rmax = fmaxf(rmax, residual_lds<T::LJ>(p, s, e, t, dred[m], qf));
+        }
         s[e] = sor_update<T::LJ>(p, s, e, t, dred[m], qf);
     }
     __syncthreads();
@@ -161,6 +213,7 @@ __device__ __forceinline__ void sor_tile_body(const SorParams& p, const float* _
             if (two) dst[t0 + 1] = out1;
         }
     }
+    return rmax;
 }
 
 
