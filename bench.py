@@ -38,28 +38,15 @@ def log(msg: str) -> None:
 
 def setup(workload_name: str, device):
     from warpbiocell_amd import workloads as W
-    from warpbiocell_amd.cells.lifecycle import LifecycleParams
-    from warpbiocell_amd.cells.mechanics import ContactParams, make_neighbor_grid
-    from warpbiocell_amd.cells.state import CellPopulation
-    from warpbiocell_amd.fields.diffusion import SolverSettings
-    from warpbiocell_amd.fields.oxygen import OxygenField, OxygenParams
-    from warpbiocell_amd.fields.scalar_field import GridGeometry
-    from warpbiocell_amd.simulation.simulator import TimeStepping
+    from warpbiocell_amd.simulation import scenario
 
-    w = getattr(W, workload_name)()
-    geom = GridGeometry.centered_cube(w.box_um, w.dx_um)
-    params = OxygenParams(W.D_UM2_PER_H, W.UPTAKE_MMHG_UM3_PER_H, W.MICHAELIS_K, w.boundary_mmHg)
-    settings = SolverSettings(omega=w.omega, tolerance=w.tolerance, max_sweeps=w.max_sweeps, check_every=w.check_every)
-    lifecycle = LifecycleParams(division_rate=w.division_rate, death_rate=0.0, inhibition_threshold=10 ** 6)
-    contact = ContactParams(stiffness=W.CONTACT_RATE, damping=1.0, max_radius=float(w.radii.max()))
-    stepping = TimeStepping(dt_cells=W.DT_CELLS, dt_mechanics=W.DT_MECHANICS)
-    pop = CellPopulation.from_numpy(w.positions, w.radii, capacity=w.capacity, device=device, seed=w.seed)
-    ox = OxygenField(geom, params, settings, device=device)
-    grid = make_neighbor_grid(contact, device)
-    return w, geom, params, settings, lifecycle, contact, stepping, pop, ox, grid
+    sc = scenario.build(getattr(W, workload_name)(), device)
+    return (sc.workload, sc.geometry, sc.params, sc.settings, sc.lifecycle, sc.contact, sc.stepping,
+            sc.population, sc.oxygen, sc.grid)
 
 
-KERNEL_OF_KIND = {"tile": "sor_fused_kernel<{tj}, {ti}>", "march": "sor_march_kernel<{tj}, {ti}>"}
+KERNEL_OF_KIND = {"tile": "sor_fused_kernel<{tj}, {ti}>", "march": "sor_march_kernel<{tj}, {ti}>",
+                  "pipe": "sor_pipe_kernel<{tj}, {ti}>", "copy": "sor_copy_kernel<{tj}, {ti}>"}
 COUNTER_TABLE = ROOT / "profiles" / "sor_counters_spheroid_100k_161.md"
 
 
@@ -110,8 +97,10 @@ def sor_roofline(ox, geom, device, workload: str, launches: int = 200) -> dict:
     cfg = _lib.query("wbc_sor_fused_get_config")
     kind, tj, ti = (ctypes.c_int32() for _ in range(3))
     _lib.call("wbc_sor_fused_config_shape", cfg, ctypes.byref(kind), ctypes.byref(tj), ctypes.byref(ti))
-    family = "march" if kind.value == 1 else "tile"
-    shape = f"march60x{tj.value}/{ti.value}" if family == "march" else f"tile60x{tj.value}x{ti.value}"
+    from warpbiocell_amd.autotune import shape_name
+
+    family = {0: "tile", 1: "march", 2: "pipe", 3: "copy"}[kind.value]
+    shape = shape_name(kind.value, tj.value, ti.value)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(shape, workload),
             "kernel": KERNEL_OF_KIND[family].format(tj=tj.value, ti=ti.value), "shape": shape,
@@ -149,6 +138,73 @@ def adjoint_measurement(device) -> dict:
     return {"workload": w.name, "forward_ms": round((t1 - t0) * 1e3, 3), "forward_sweeps": prob.last_forward.sweeps,
             "adjoint_ms": round((t2 - t1) * 1e3, 3), "adjoint_sweeps": prob.last_backward[0],
             "adjoint_over_forward": round((t2 - t1) / (t1 - t0), 3)}
+
+
+def batched_measurement(device, steps: int) -> dict:
+    """Batched sweep on the 15-replica workload: `steps` cell steps of every replica, once
+    serially with the plain step (replica after replica, as the reference's sweep runs them, each
+    with its autotuned SOR tile shape) and once batched (one hipGraph replay per step, the solves
+    of all replicas in one persistent launch, autotuned schedule); the final states are compared
+    bit for bit."""
+    import numpy as np
+    import torch
+
+    from warpbiocell_amd import _lib
+    from warpbiocell_amd import workloads as W
+    from warpbiocell_amd.autotune import tune_batch, tune_sor
+    from warpbiocell_amd.simulation import scenario
+    from warpbiocell_amd.simulation.batched import CELL_FIELDS, BatchedSweep
+    from warpbiocell_amd.simulation.simulator import cell_step
+
+    ws = W.sweep_15()
+    serial = [scenario.build(w, device) for w in ws]
+    batched = [scenario.build(w, device) for w in ws]
+    configs = []
+    for sc in serial:  # tune every distinct grid before the timed region (cached per shape)
+        configs.append(tune_sor(sc.oxygen.field, sc.oxygen.density, sc.oxygen.species.kinetics,
+                                sc.settings.omega)["choice"])
+    tuner = BatchedSweep([sc.replica() for sc in batched])
+    tuner.step(1, use_graph=False)
+    tune = tune_batch(tuner)
+    tuner.close()
+    del tuner
+    batch = BatchedSweep([sc.replica() for sc in batched])
+    tune_batch(batch)
+    batch.capture()
+    torch.cuda.synchronize()
+    log(f"  batched sweep: {len(ws)} replicas, {sum(w.n for w in ws)} cells, "
+        f"{sum(sc.geometry.node_count for sc in batched)} nodes; schedule {tune['schedule']}")
+
+    t0 = time.perf_counter()
+    serial_sweeps = 0
+    for sc, cfg in zip(serial, configs):
+        _lib.call("wbc_sor_fused_set_config", cfg)
+        for _ in range(steps):
+            serial_sweeps += cell_step(sc.population, sc.grid, sc.lifecycle, sc.contact, sc.stepping,
+                                       oxygen=sc.oxygen).field_sweeps
+    torch.cuda.synchronize()
+    t1 = time.perf_counter()
+    batch.step(steps, use_graph=True)
+    torch.cuda.synchronize()
+    t2 = time.perf_counter()
+    batched_sweeps = sum(batch.total_sweeps())
+    same = serial_sweeps == batched_sweeps
+    for i, sc in enumerate(serial):
+        n = sc.population.count
+        for name in CELL_FIELDS:
+            a = getattr(sc.population, name)[:n].cpu().numpy().view(np.uint8)
+            same = same and np.array_equal(a, batch.population_view(i, name)[:n].cpu().numpy().view(np.uint8))
+        same = same and np.array_equal(sc.oxygen.values.cpu().numpy().view(np.uint32),
+                                       batch.field_view(i).cpu().numpy().view(np.uint32))
+    cells = sum(w.n for w in ws)
+    batch.close()
+    return {"workload": "sweep_15", "replicas": len(ws), "cells": cells,
+            "nodes": sum(sc.geometry.node_count for sc in batched), "steps": steps,
+            "serial_s": round(t1 - t0, 4), "batched_s": round(t2 - t1, 4),
+            "speedup": round((t1 - t0) / (t2 - t1), 3), "bit_exact": bool(same),
+            "sweeps": batched_sweeps, "serial_cell_steps_per_s": round(cells * steps / (t1 - t0), 1),
+            "batched_cell_steps_per_s": round(cells * steps / (t2 - t1), 1),
+            "schedule": tune["schedule"], "schedule_ms_per_solve": tune["ms_per_solve"]}
 
 
 def snapshot(pop, ox):
@@ -192,6 +248,8 @@ def main() -> int:
     ap.add_argument("--no-cpu-baseline", action="store_true")
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
+    ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (counter calibration)")
     args = ap.parse_args()
@@ -245,6 +303,9 @@ def main() -> int:
     adj = None if args.no_adjoint else adjoint_measurement(device)
     if adj:
         log(f"  adjoint: {adj}")
+    bat = None if args.no_batched else batched_measurement(device, args.batched_steps)
+    if bat:
+        log(f"  batched: {bat}")
     cpu = None
     if snap is not None:
         cpu = cpu_baseline(snap, w, geom, params, settings, lifecycle, contact, stepping)
@@ -272,6 +333,7 @@ def main() -> int:
         "roofline": roof,
         "sor_autotune": tune,
         "adjoint": adj,
+        "batched": bat,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

@@ -255,11 +255,14 @@ size_t wbc_batch_workspace_bytes(const wbc_replica* replicas, int32_t count, int
 int wbc_batch_create(const wbc_replica* replicas, int32_t count, int32_t hash_dim, const wbc_batch_cells* cells,
                      float* field, void* workspace, size_t workspace_bytes, wbc_stream_t stream, wbc_batch_t* out);
 /* Schedule of the persistent solve: tile shape `tile` in [0, wbc_batch_schedule_count()) (60 x TJ x TI
- * nodes, see wbc_batch_schedule_shape) and workgroups per CU (0: as many as fit). Resets the graph. */
+ * nodes, see wbc_batch_schedule_shape; the second half of the table runs the same shapes with each
+ * workgroup copying its next tile while computing the current one) and workgroups per CU (0: as
+ * many as fit). Resets the graph. */
 int wbc_batch_schedule_count(void);
 int wbc_batch_schedule_shape(int32_t tile, int32_t* tj, int32_t* ti, int32_t* max_blocks_per_cu);
 int wbc_batch_set_schedule(wbc_batch_t batch, int32_t tile, int32_t blocks_per_cu);
-/* `steps` cell steps of every replica, enqueued on `stream`. */
+/* `steps` cell steps of every replica, enqueued on `stream`. With use_graph = 1 the step is
+ * captured on first use; steps = 0 with use_graph = 1 only captures it. */
 int wbc_batch_step(wbc_batch_t batch, int32_t steps, int32_t use_graph, wbc_stream_t stream);
 /* Only the steady-state solves of every replica (one persistent launch), from the current fields. */
 int wbc_batch_solve(wbc_batch_t batch, wbc_stream_t stream);

@@ -1,7 +1,12 @@
-"""Autotuner for the fused SOR sweep: times every tile shape of the library's table on the
-caller's grid (HIP events on the launch stream) and selects the fastest for the solve loop.
+"""Autotuners.
 
-Results are cached per (device, grid shape); ``tune_sor`` is cheap to call again.
+``tune_sor``: the fused SOR sweep. Times every tile shape of the library's table on the caller's
+grid (HIP events on the launch stream) and selects the fastest for the solve loop. Cached per
+(device, grid shape).
+
+``tune_batch``: the persistent solve of a batched replica sweep. Times every (tile shape,
+workgroups per CU) schedule on the batch's own replicas and selects the fastest. Cached per
+(device, replica grids and cell counts).
 """
 
 from __future__ import annotations
@@ -15,13 +20,19 @@ from warpbiocell_amd import _lib
 _cache: dict = {}
 
 
+def shape_name(kind: int, tj: int, ti: int) -> str:
+    return {0: f"tile60x{tj}x{ti}", 1: f"march60x{tj}/{ti}", 2: f"pipe60x{tj}x{ti}", 3: f"copy60x{tj}x{ti}"}[kind]
+
+
 def config_shapes() -> list[str]:
-    """Names of the shape table: 'tile60xTJxTI' (3-D tile) or 'march60xTJ/CI' (marching i)."""
+    """Names of the shape table: 'tile60xTJxTI' (3-D tile), 'march60xTJ/CI' (marching i) or
+    'pipe60xTJxTI' (3-D tile, pipelined persistent workgroups) or 'copy60xTJxTI' (3-D tile,
+    global->LDS copies)."""
     out = []
     for c in range(_lib.query("wbc_sor_fused_config_count")):
         kind, tj, ti = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         _lib.call("wbc_sor_fused_config_shape", c, ctypes.byref(kind), ctypes.byref(tj), ctypes.byref(ti))
-        out.append(f"tile60x{tj.value}x{ti.value}" if kind.value == 0 else f"march60x{tj.value}/{ti.value}")
+        out.append(shape_name(kind.value, tj.value, ti.value))
     return out
 
 
@@ -61,3 +72,47 @@ def tune_sor(field, density: torch.Tensor, kinetics, omega: float, sweeps: int =
                        "us_per_sweep": {s: round(t, 3) for s, t in zip(shapes, times)}}
     _lib.call("wbc_sor_fused_set_config", _cache[key]["choice"])
     return _cache[key]
+
+
+def batch_schedules() -> list[tuple[int, int, str]]:
+    """(tile, workgroups per CU, name) of every schedule of the persistent batched solve: each
+    tile shape at 1, 2, ... up to the occupancy limit workgroups per CU (0 = the limit)."""
+    out = []
+    for tile in range(_lib.query("wbc_batch_schedule_count")):
+        tj, ti, most = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
+        _lib.call("wbc_batch_schedule_shape", tile, ctypes.byref(tj), ctypes.byref(ti), ctypes.byref(most))
+        kind = "pipe" if tile >= _lib.query("wbc_batch_schedule_count") // 2 else "copy"
+        for bpc in range(1, most.value + 1):
+            out.append((tile, bpc, f"{kind}60x{tj.value}x{ti.value}@{bpc}/CU"))
+    return out
+
+
+def tune_batch(batch, repeats: int = 1) -> dict:
+    """Time the persistent solve of every replica of ``batch`` under every schedule, from the same
+    start fields (restored after each run; the solve's results do not depend on the schedule),
+    and select the fastest. Cached per (device, replica grids and cell counts)."""
+    key = ("batch", str(batch.device), tuple((r.oxygen.geometry.shape, r.population.count) for r in batch.replicas))
+    if key not in _cache:
+        stream = torch.cuda.current_stream(batch.device)
+        start = batch.field.clone()
+        times = {}
+        for tile, bpc, name in batch_schedules():
+            batch.set_schedule(tile, bpc)
+            best = float("inf")
+            for _ in range(repeats + 1):  # the first run warms up
+                batch.field.copy_(start)
+                a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+                a.record(stream)
+                batch.solve()
+                b.record(stream)
+                b.synchronize()
+                best = min(best, a.elapsed_time(b))
+            batch.report()  # raises if a barrier timed out
+            times[(tile, bpc, name)] = best
+        batch.field.copy_(start)
+        choice = min(times, key=times.get)
+        _cache[key] = {"tile": choice[0], "blocks_per_cu": choice[1], "schedule": choice[2],
+                       "ms_per_solve": {n: round(t, 3) for (_, _, n), t in times.items()}}
+    c = _cache[key]
+    batch.set_schedule(c["tile"], c["blocks_per_cu"])
+    return c

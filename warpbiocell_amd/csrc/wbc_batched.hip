@@ -18,10 +18,10 @@
 // write buf[(c+1) % 3] and buf[(c+2) % 3] and never field_c; after the loop field_c is copied
 // into buf[0] when c % 3 != 0.
 //
-// Grid barrier: two-level (per-XCD-group counter, then a top counter), release/acquire at agent
-// scope, generation word; every wait is bounded and a timeout raises an error flag that makes
-// every workgroup leave the kernel. The grid never exceeds the number of workgroups that fit
-// on the device at once.
+// Grid barrier: arrival stamps per workgroup, gathered by workgroup 0, which publishes the new
+// generation (release/acquire fences at agent scope, relaxed polling). Every wait is bounded and a
+// timeout raises an error flag that makes every workgroup leave the kernel. The grid never
+// exceeds the number of workgroups that fit on the device at once.
 #include <hip/hip_runtime.h>
 
 #include <climits>
@@ -58,11 +58,24 @@ struct SolveState {
     int32_t converged;
 };
 
+constexpr int kMaxGrid = 4096;  // workgroups of the persistent launch
+
+// Grid barrier state. Workgroups register under their XCD (hardware XCC id) at launch. The first
+// barrier of a launch is the general one: every workgroup releases its own writes at agent scope
+// and stamps its arrival (the barrier's target generation) in its own word; workgroup 0 waits
+// for every stamp, then publishes the generation in one word per XCD (own 256-byte line each),
+// which that XCD's workgroups poll. Every later barrier writes back each XCD's L2 once: the
+// XCD's leader (first registered workgroup) waits for its members' stamps (their stores have
+// reached the L2 they share), releases at agent scope and stamps the XCD; workgroup 0 waits for
+// the XCD stamps and publishes. Every workgroup acquires after the publication. No
+// read-modify-write except the registration, and no word with more than one writer per barrier.
 struct alignas(256) BarrierState {
-    unsigned group[8][64];  // one arrival counter per XCD group, each on its own 256-byte line
-    unsigned top[64];
-    unsigned gen[64];
+    unsigned gen[8][64];
+    unsigned xcd_stamp[8][64];
+    unsigned members[8][64];
     int32_t error[64];
+    unsigned arrive[kMaxGrid];
+    int32_t roster[8][kMaxGrid];
 };
 
 __device__ __forceinline__ bool is_check(const ReplicaDev& d, int c)
@@ -76,42 +89,132 @@ __device__ __forceinline__ int32_t load_i32(const int32_t* p)
 }
 
 // Returns false when this or another workgroup's wait timed out.
-__device__ __forceinline__ bool grid_sync(BarrierState* bar, unsigned& gen_local, int32_t* lds_flag)
+__device__ __forceinline__ unsigned load_u32(const unsigned* p)
 {
-    __threadfence();
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
+__device__ __forceinline__ bool reached(unsigned stamp, unsigned target) { return (int)(stamp - target) >= 0; }
+
+// Grid barrier number `target` (generations increase by one per barrier). The workgroup barrier
+// completes every wave's stores to the L2 its workgroup shares; thread 0's agent-scope fence then
+// releases them to the other XCDs before the arrival stamp, and the fence after the wait acquires
+// the other workgroups' writes before the workgroup barrier lets its waves load. Polls are relaxed
+// loads (an acquire load would invalidate caches on every poll). Every wait is bounded; on a
+// timeout the error flag is raised and the caller leaves the kernel. Returns false on error.
+__device__ __forceinline__ bool grid_sync(BarrierState* bar, unsigned target, int32_t* lds_flag)
+{
+    const int tid = threadIdx.x;
     __syncthreads();
-    if (threadIdx.x == 0) {
-        const unsigned g = gen_local;
-        const unsigned nb = gridDim.x;
-        const unsigned x = blockIdx.x & 7u;
-        const unsigned groups = nb < 8u ? nb : 8u;
-        const unsigned in_group = (nb - x + 7u) / 8u;
-        const unsigned old =
-            __hip_atomic_fetch_add(&bar->group[x][0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-        if (old == in_group - 1u) {
-            __hip_atomic_store(&bar->group[x][0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-            const unsigned o2 = __hip_atomic_fetch_add(&bar->top[0], 1u, __ATOMIC_ACQ_REL, __HIP_MEMORY_SCOPE_AGENT);
-            if (o2 == groups - 1u) {
-                __hip_atomic_store(&bar->top[0], 0u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
-                __hip_atomic_store(&bar->gen[0], g + 1u, __ATOMIC_RELEASE, __HIP_MEMORY_SCOPE_AGENT);
-            }
-        }
+    if (tid == 0) {
+        __threadfence();
+        __hip_atomic_store(&bar->arrive[blockIdx.x], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
+    bool ok = true;
+    if (blockIdx.x == 0) {
         unsigned spins = 0;
-        bool ok = true;
-        while (__hip_atomic_load(&bar->gen[0], __ATOMIC_ACQUIRE, __HIP_MEMORY_SCOPE_AGENT) == g) {
+        while (true) {
+            bool mine = true;
+            for (unsigned b = tid; b < gridDim.x; b += blockDim.x) mine = mine && reached(load_u32(&bar->arrive[b]), target);
+            if (__syncthreads_and(mine)) break;
+            if (++spins > kSpinLimit) {
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (tid == 0) {
+            if (!ok) __hip_atomic_store(&bar->error[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            for (int g = 0; g < 8; ++g)
+                __hip_atomic_store(&bar->gen[g][0], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else if (tid == 0) {
+        const unsigned* gen = &bar->gen[blockIdx.x & 7][0];
+        unsigned spins = 0;
+        while (!reached(load_u32(gen), target)) {
             if (++spins > kSpinLimit) {
                 __hip_atomic_store(&bar->error[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
                 ok = false;
                 break;
             }
-            __builtin_amdgcn_s_sleep(2);
+            __builtin_amdgcn_s_sleep(1);
         }
-        gen_local = g + 1u;
+    }
+    if (tid == 0) {
         if (load_i32(&bar->error[0]) != 0) ok = false;
         *lds_flag = ok ? 1 : 0;
+        __threadfence();
     }
     __syncthreads();
-    __threadfence();
+    return *lds_flag != 0;
+}
+
+__device__ __forceinline__ int xcc_id() { return __builtin_amdgcn_s_getreg(20 | (0 << 6) | (3 << 11)) & 7; }
+
+// The XCD-aggregated barrier (see BarrierState). x: this workgroup's XCD, n_x: workgroups
+// registered on it, leader: this workgroup is the XCD's first registered one.
+__device__ __forceinline__ bool grid_sync_xcd(BarrierState* bar, unsigned target, int x, int n_x, bool leader,
+                                              int32_t* lds_flag)
+{
+    const int tid = threadIdx.x;
+    __syncthreads();  // every wave's stores have completed (to this XCD's L2)
+    if (tid == 0) __hip_atomic_store(&bar->arrive[blockIdx.x], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    bool ok = true;
+    if (leader) {
+        unsigned spins = 0;
+        while (true) {
+            bool mine = true;
+            for (int i = tid; i < n_x; i += blockDim.x)
+                mine = mine && reached(load_u32(&bar->arrive[bar->roster[x][i]]), target);
+            if (__syncthreads_and(mine)) break;
+            if (++spins > kSpinLimit) {
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (tid == 0) {
+            __builtin_amdgcn_fence(__ATOMIC_RELEASE, "agent");  // write back this XCD's L2
+            __hip_atomic_store(&bar->xcd_stamp[x][0], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    }
+    if (blockIdx.x == 0) {
+        unsigned spins = 0;
+        while (true) {
+            const bool mine = tid >= 8 || load_u32(&bar->members[tid][0]) == 0u ||
+                              reached(load_u32(&bar->xcd_stamp[tid][0]), target);
+            if (__syncthreads_and(mine)) break;
+            if (++spins > kSpinLimit) {
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+        if (tid == 0) {
+            if (!ok) __hip_atomic_store(&bar->error[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+            __threadfence();
+            for (int g = 0; g < 8; ++g)
+                __hip_atomic_store(&bar->gen[g][0], target, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        }
+    } else if (tid == 0) {
+        unsigned spins = 0;
+        while (!reached(load_u32(&bar->gen[x][0]), target)) {
+            if (++spins > kSpinLimit) {
+                __hip_atomic_store(&bar->error[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+                ok = false;
+                break;
+            }
+            __builtin_amdgcn_s_sleep(1);
+        }
+    }
+    if (tid == 0) {
+        if (!ok) __hip_atomic_store(&bar->error[0], 1, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        if (load_i32(&bar->error[0]) != 0) ok = false;
+        *lds_flag = ok ? 1 : 0;
+        __builtin_amdgcn_fence(__ATOMIC_ACQUIRE, "agent");
+    }
+    __syncthreads();
     return *lds_flag != 0;
 }
 
@@ -121,7 +224,10 @@ __host__ __device__ constexpr int tiles_of(int nx, int ny, int nz)
     return ((nz + kTK - 1) / kTK) * ((ny + TJ - 1) / TJ) * ((nx + TI - 1) / TI);
 }
 
-template <int TJ, int TI>
+// Field rows reach LDS by global->LDS copies (no registers hold them). kPipe: each workgroup copies
+// the next tile's rows into a second LDS buffer and loads its densities while it computes the
+// current tile.
+template <int TJ, int TI, bool kPipe>
 __global__ void __launch_bounds__(256) batch_sor_kernel(const ReplicaDev* __restrict__ rep, int32_t R,
                                                         int32_t max_phase, float* __restrict__ partial,
                                                         int32_t partial_stride, SolveState* __restrict__ state,
@@ -129,7 +235,9 @@ __global__ void __launch_bounds__(256) batch_sor_kernel(const ReplicaDev* __rest
                                                         BarrierState* __restrict__ bar)
 {
     using T = SorTile<TJ, TI>;
+    using I = TileIters<TJ, TI>;
     __shared__ float s[T::kRows * kLK];
+    __shared__ float s2[kPipe ? T::kRows * kLK : 1];
     __shared__ int32_t tile_off[kMaxReplicas + 1];
     __shared__ int32_t active[kMaxReplicas];
     __shared__ float wmax[4];
@@ -143,9 +251,18 @@ __global__ void __launch_bounds__(256) batch_sor_kernel(const ReplicaDev* __rest
         }
         tile_off[R] = acc;
     }
-    unsigned gen_local = 0;
-    if (tid == 0) gen_local = __hip_atomic_load(&bar->gen[0], __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    __shared__ unsigned base_gen;
+    __shared__ int32_t my_slot;
+    const int xcd = xcc_id();
+    if (tid == 0) {
+        base_gen = load_u32(&bar->gen[0][0]);
+        my_slot = (int32_t)__hip_atomic_fetch_add(&bar->members[xcd][0], 1u, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+        __hip_atomic_store(&bar->roster[xcd][my_slot], (int32_t)blockIdx.x, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+    }
     __syncthreads();
+    unsigned target = base_gen;
+    int n_xcd = 0;  // known after the first barrier, which makes every registration visible
+    const bool leader = my_slot == 0;
     const int32_t total = tile_off[R];
     // XCD-aware split: the workgroups of XCD group x (blockIdx % 8 == x) share one contiguous
     // eighth of the work items, so neighbouring tiles meet in the same L2.
@@ -186,38 +303,105 @@ __global__ void __launch_bounds__(256) batch_sor_kernel(const ReplicaDev* __rest
         }
 
         // Sweep q of every active replica.
-        for (int32_t it = lo + local; it < hi; it += per_group) {
-            int rl = 0, rh = R;  // largest r with tile_off[r] <= it
+        auto replica_of = [&](int32_t it) {  // largest r with tile_off[r] <= it (the same in every lane)
+            int rl = 0, rh = R;
             while (rh - rl > 1) {
                 const int mid = (rl + rh) >> 1;
                 if (tile_off[mid] <= it) rl = mid; else rh = mid;
             }
-            // r and its tile offset are the same in every lane: keep them (and with them every
-            // load of the replica's descriptor) scalar.
-            const int r = __builtin_amdgcn_readfirstlane(rl);
-            if (!__builtin_amdgcn_readfirstlane(active[r])) continue;
+            return __builtin_amdgcn_readfirstlane(rl);
+        };
+        auto origin_of = [&](int32_t it, int r, int& k0, int& j0, int& i0) {
             const ReplicaDev& d = rep[r];
             const int L = it - __builtin_amdgcn_readfirstlane(tile_off[r]);
             const int tkn = (d.p.nz + kTK - 1) / kTK, tjn = (d.p.ny + TJ - 1) / TJ;
-            const int tk = L % tkn, tj = (L / tkn) % tjn, ti = L / (tkn * tjn);
-            const float* src = d.buf[q % 3];
-            float* dst = d.buf[(q + 1) % 3];
-            const bool qf = __builtin_isfinite(d.p.uptake_max);
-            const bool chk = is_check(d, q);
-            float m = sor_tile_body<TJ, TI, false, true>(d.p, src, dst, tk * kTK, tj * TJ, ti * TI, qf, s, chk);
-            if (chk) {
-                m = wave_max(m);
-                if ((tid & (kWave - 1)) == 0) wmax[tid / kWave] = m;
-                __syncthreads();
-                if (tid == 0)
-                    partial[(q & 1) * partial_stride + it] =
-                        fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+            k0 = (L % tkn) * kTK;
+            j0 = ((L / tkn) % tjn) * TJ;
+            i0 = (L / (tkn * tjn)) * TI;
+        };
+        auto store_partial = [&](int32_t it, float m) {
+            m = wave_max(m);
+            if ((tid & (kWave - 1)) == 0) wmax[tid / kWave] = m;
+            lds_barrier();
+            if (tid == 0) partial[(q & 1) * partial_stride + it] = fmaxf(fmaxf(wmax[0], wmax[1]), fmaxf(wmax[2], wmax[3]));
+        };
+        if constexpr (!kPipe) {
+            for (int32_t it = lo + local; it < hi; it += per_group) {
+                // r and its tile offset are the same in every lane: keep them (and with them every
+                // load of the replica's descriptor) scalar.
+                const int r = replica_of(it);
+                if (!__builtin_amdgcn_readfirstlane(active[r])) continue;
+                const ReplicaDev& d = rep[r];
+                int k0, j0, i0;
+                origin_of(it, r, k0, j0, i0);
+                int t2 = threadIdx.x;
+                asm volatile("" : "+v"(t2));
+                const TileLane L(t2, k0, d.p.nz);
+                float dr[I::kRed], db[I::kBlack];
+                lds_barrier();  // the previous tile's passes no longer read s
+                tile_copy_rows<TJ, TI>(d.p, d.buf[q % 3], s, L, k0, j0, i0);
+                tile_densities<TJ, TI>(d.p, L, k0, j0, i0, dr, db);
+                dma_barrier();
+                const bool chk = is_check(d, q);
+                const float m = tile_passes<TJ, TI, true, true>(d.p, s, d.buf[(q + 1) % 3], L, k0, j0, i0,
+                                                                __builtin_isfinite(d.p.uptake_max), dr, db, chk);
+                if (chk) store_partial(it, m);
+            }
+        } else {
+            auto next_active = [&](int32_t it) {
+                for (; it < hi; it += per_group)
+                    if (__builtin_amdgcn_readfirstlane(active[replica_of(it)])) return it;
+                return hi;
+            };
+            auto prefetch = [&](int32_t it, float* buf, float (&dr)[I::kRed], float (&db)[I::kBlack]) {
+                const int r = replica_of(it);
+                int k0, j0, i0;
+                origin_of(it, r, k0, j0, i0);
+                const ReplicaDev& d = rep[r];
+                int t2 = threadIdx.x;
+                asm volatile("" : "+v"(t2));
+                const TileLane L(t2, k0, d.p.nz);
+                tile_copy_rows<TJ, TI>(d.p, d.buf[q % 3], buf, L, k0, j0, i0);
+                tile_densities<TJ, TI>(d.p, L, k0, j0, i0, dr, db);
+            };
+            auto compute = [&](int32_t it, float* buf, const float (&dr)[I::kRed], const float (&db)[I::kBlack]) {
+                const int r = replica_of(it);
+                int k0, j0, i0;
+                origin_of(it, r, k0, j0, i0);
+                const ReplicaDev& d = rep[r];
+                int t2 = threadIdx.x;
+                asm volatile("" : "+v"(t2));
+                const TileLane L(t2, k0, d.p.nz);
+                const bool chk = is_check(d, q);
+                const float m = tile_passes<TJ, TI, true, true>(d.p, buf, d.buf[(q + 1) % 3], L, k0, j0, i0,
+                                                                __builtin_isfinite(d.p.uptake_max), dr, db, chk);
+                if (chk) store_partial(it, m);
+            };
+            // Two-stage pipeline, unrolled by two so the density registers alternate without copies.
+            float ar[I::kRed], ab[I::kBlack], br[I::kRed], bb[I::kBlack];
+            int32_t cur = next_active(lo + local);
+            if (cur < hi) prefetch(cur, s, ar, ab);
+            while (cur < hi) {
+                dma_barrier();  // s holds tile cur; s2 is free
+                int32_t nxt = next_active(cur + per_group);
+                if (nxt < hi) prefetch(nxt, s2, br, bb);
+                compute(cur, s, ar, ab);
+                cur = nxt;
+                if (cur >= hi) break;
+                dma_barrier();  // s2 holds tile cur; s is free
+                nxt = next_active(cur + per_group);
+                if (nxt < hi) prefetch(nxt, s, ar, ab);
+                compute(cur, s2, br, bb);
+                cur = nxt;
             }
         }
-        if (!grid_sync(bar, gen_local, &flag)) {
+        const bool synced = q == 0 ? grid_sync(bar, ++target, &flag)
+                                   : grid_sync_xcd(bar, ++target, xcd, n_xcd, leader, &flag);
+        if (!synced) {
             ok = false;
             break;
         }
+        if (q == 0) n_xcd = (int)load_u32(&bar->members[xcd][0]);
     }
     if (!ok) return;
 
@@ -246,11 +430,16 @@ __global__ void __launch_bounds__(256) batch_reset_kernel(SolveState* __restrict
         st[t].residual = __builtin_huge_valf();
         st[t].converged = 0;
     }
-    if (t < 8) bar->group[t][0] = 0u;
-    if (t == 0) {
-        bar->top[0] = 0u;
-        bar->error[0] = 0;
+    // Every arrival stamp back to the published generation (a launch that timed out may have
+    // left stamps one ahead), every copy of the generation equal.
+    const unsigned g = bar->gen[0][0];
+    for (int b = t; b < kMaxGrid; b += 256) bar->arrive[b] = g;
+    if (t < 8) {
+        bar->gen[t][0] = g;
+        bar->xcd_stamp[t][0] = g;
+        bar->members[t][0] = 0u;
     }
+    if (t == 0) bar->error[0] = 0;
 }
 
 __global__ void __launch_bounds__(256) batch_zero_i64_kernel(long long* __restrict__ p, int64_t n)
@@ -392,20 +581,26 @@ struct BatchTile {
 constexpr BatchTile kBatchTiles[] = {{4, 4}, {4, 8}, {8, 4}, {8, 8}};
 constexpr int kBatchTileCount = sizeof(kBatchTiles) / sizeof(kBatchTiles[0]);
 
+// Schedule table: tile index t < kBatchTileCount runs one tile at a time per workgroup,
+// t + kBatchTileCount the pipelined loop.
 static const void* batch_kernel_fn(int tile)
 {
     switch (tile) {
-    case 0: return (const void*)batch_sor_kernel<4, 4>;
-    case 1: return (const void*)batch_sor_kernel<4, 8>;
-    case 2: return (const void*)batch_sor_kernel<8, 4>;
-    case 3: return (const void*)batch_sor_kernel<8, 8>;
+    case 0: return (const void*)batch_sor_kernel<4, 4, false>;
+    case 1: return (const void*)batch_sor_kernel<4, 8, false>;
+    case 2: return (const void*)batch_sor_kernel<8, 4, false>;
+    case 3: return (const void*)batch_sor_kernel<8, 8, false>;
+    case 4: return (const void*)batch_sor_kernel<4, 4, true>;
+    case 5: return (const void*)batch_sor_kernel<4, 8, true>;
+    case 6: return (const void*)batch_sor_kernel<8, 4, true>;
+    case 7: return (const void*)batch_sor_kernel<8, 8, true>;
     default: return nullptr;
     }
 }
 
 static int tiles_for(int tile, int nx, int ny, int nz)
 {
-    switch (tile) {
+    switch (tile % kBatchTileCount) {
     case 0: return tiles_of<4, 4>(nx, ny, nz);
     case 1: return tiles_of<4, 8>(nx, ny, nz);
     case 2: return tiles_of<8, 4>(nx, ny, nz);
@@ -511,6 +706,7 @@ static hipError_t batch_grid(wbc_batch* b)
     if (per_cu < 1) return hipErrorInvalidConfiguration;
     const int use = (b->blocks_per_cu > 0 && b->blocks_per_cu < per_cu) ? b->blocks_per_cu : per_cu;
     int grid = use * cus / 8 * 8;
+    if (grid > kMaxGrid) grid = kMaxGrid;
     if (grid < 8) return hipErrorInvalidConfiguration;
     b->grid = grid;
     return hipSuccess;
@@ -527,20 +723,20 @@ static hipError_t enqueue_solve(wbc_batch* b, hipStream_t s)
     long long* totals = (long long*)(ws + L.total_sweeps);
     batch_reset_kernel<<<1, 256, 0, s>>>(st, b->R, bar);
     WBC_TRY_ERR(hipGetLastError());
+#define WBC_BATCH_LAUNCH(TJ, TI, PIPE)                                                                             \
+    batch_sor_kernel<TJ, TI, PIPE><<<b->grid, 256, 0, s>>>(rep, b->R, b->max_phase, partial, L.partial_stride, st, \
+                                                           totals, bar)
     switch (b->tile) {
-    case 0:
-        batch_sor_kernel<4, 4><<<b->grid, 256, 0, s>>>(rep, b->R, b->max_phase, partial, L.partial_stride, st, totals, bar);
-        break;
-    case 1:
-        batch_sor_kernel<4, 8><<<b->grid, 256, 0, s>>>(rep, b->R, b->max_phase, partial, L.partial_stride, st, totals, bar);
-        break;
-    case 2:
-        batch_sor_kernel<8, 4><<<b->grid, 256, 0, s>>>(rep, b->R, b->max_phase, partial, L.partial_stride, st, totals, bar);
-        break;
-    default:
-        batch_sor_kernel<8, 8><<<b->grid, 256, 0, s>>>(rep, b->R, b->max_phase, partial, L.partial_stride, st, totals, bar);
-        break;
+    case 0: WBC_BATCH_LAUNCH(4, 4, false); break;
+    case 1: WBC_BATCH_LAUNCH(4, 8, false); break;
+    case 2: WBC_BATCH_LAUNCH(8, 4, false); break;
+    case 3: WBC_BATCH_LAUNCH(8, 8, false); break;
+    case 4: WBC_BATCH_LAUNCH(4, 4, true); break;
+    case 5: WBC_BATCH_LAUNCH(4, 8, true); break;
+    case 6: WBC_BATCH_LAUNCH(8, 4, true); break;
+    default: WBC_BATCH_LAUNCH(8, 8, true); break;
     }
+#undef WBC_BATCH_LAUNCH
     return hipGetLastError();
 }
 
@@ -679,6 +875,7 @@ WBC_API int wbc_batch_create(const wbc_replica* replicas, int32_t count, int32_t
     WBC_TRY(hipMemsetAsync(ws + L.total_sweeps, 0, sizeof(long long) * count, stream));
     WBC_TRY(hipMemsetAsync(ws + L.bar, 0, sizeof(BarrierState), stream));
     WBC_TRY(hipMemsetAsync(ws + L.cnt, 0, sizeof(int32_t) * L.buckets, stream));
+    WBC_TRY(hipMemsetAsync(ws + L.density, 0, sizeof(float) * L.nodes, stream));
     WBC_TRY(hipStreamSynchronize(stream));  // the host vectors above go out of scope
 
     wbc_batch* b = new (std::nothrow) wbc_batch();
@@ -706,13 +903,13 @@ WBC_API size_t wbc_batch_struct_size(int32_t which)
     return which == 0 ? sizeof(wbc_replica) : which == 1 ? sizeof(wbc_batch_cells) : 0;
 }
 
-WBC_API int wbc_batch_schedule_count(void) { return kBatchTileCount; }
+WBC_API int wbc_batch_schedule_count(void) { return 2 * kBatchTileCount; }
 
 WBC_API int wbc_batch_schedule_shape(int32_t tile, int32_t* tj, int32_t* ti, int32_t* max_blocks_per_cu)
 {
-    if (tile < 0 || tile >= kBatchTileCount || !tj || !ti || !max_blocks_per_cu) return (int)hipErrorInvalidValue;
-    *tj = kBatchTiles[tile].tj;
-    *ti = kBatchTiles[tile].ti;
+    if (tile < 0 || tile >= 2 * kBatchTileCount || !tj || !ti || !max_blocks_per_cu) return (int)hipErrorInvalidValue;
+    *tj = kBatchTiles[tile % kBatchTileCount].tj;
+    *ti = kBatchTiles[tile % kBatchTileCount].ti;
     int per_cu = 0;
     WBC_TRY(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_fn(tile), 256, 0));
     *max_blocks_per_cu = per_cu;
@@ -721,7 +918,7 @@ WBC_API int wbc_batch_schedule_shape(int32_t tile, int32_t* tj, int32_t* ti, int
 
 WBC_API int wbc_batch_set_schedule(wbc_batch_t b, int32_t tile, int32_t blocks_per_cu)
 {
-    if (!b || tile < 0 || tile >= kBatchTileCount || blocks_per_cu < 0) return (int)hipErrorInvalidValue;
+    if (!b || tile < 0 || tile >= 2 * kBatchTileCount || blocks_per_cu < 0) return (int)hipErrorInvalidValue;
     const int old_tile = b->tile, old_bpc = b->blocks_per_cu;
     b->tile = tile;
     b->blocks_per_cu = blocks_per_cu;
@@ -739,7 +936,7 @@ WBC_API int wbc_batch_set_schedule(wbc_batch_t b, int32_t tile, int32_t blocks_p
 WBC_API int wbc_batch_step(wbc_batch_t b, int32_t steps, int32_t use_graph, hipStream_t stream)
 {
     if (!b || steps < 0) return (int)hipErrorInvalidValue;
-    if (steps == 0) return 0;
+    if (steps == 0 && !use_graph) return 0;
     if (!use_graph) {
         for (int i = 0; i < steps; ++i) WBC_TRY(enqueue_step(b, stream));
         return 0;

@@ -34,6 +34,88 @@ __global__ void __launch_bounds__(256) sor_fused_kernel(SorParams p, const float
     sor_tile_body<TJ, TI>(p, src, dst, tk * kTK, tj * TJ, ti * TI, q_finite != 0, s);
 }
 
+// Copy variant: one tile per workgroup as sor_fused_kernel, but the field rows go straight from
+// memory to LDS (global->LDS copies), so no registers hold them and more workgroups fit per CU.
+template <int TJ, int TI>
+__global__ void __launch_bounds__(256) sor_copy_kernel(SorParams p, const float* __restrict__ src,
+                                                       float* __restrict__ dst, const int32_t* __restrict__ done,
+                                                       int tiles_k, int tiles_j, int q_finite)
+{
+    using I = TileIters<TJ, TI>;
+    __shared__ float s[SorTile<TJ, TI>::kRows * kLK];
+    if (done && *done) return;
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int k0 = (tile % tiles_k) * kTK;
+    const int j0 = ((tile / tiles_k) % tiles_j) * TJ;
+    const int i0 = (tile / (tiles_k * tiles_j)) * TI;
+    const TileLane L(threadIdx.x, k0, p.nz);
+    float dr[I::kRed], db[I::kBlack];
+    tile_copy_rows<TJ, TI>(p, src, s, L, k0, j0, i0);
+    tile_densities<TJ, TI>(p, L, k0, j0, i0, dr, db);
+    dma_barrier();
+    tile_passes<TJ, TI, false, true>(p, s, dst, L, k0, j0, i0, q_finite != 0, dr, db, false);
+}
+
+// Pipelined persistent variant: the grid is the number of workgroups resident at once; the
+// workgroups of XCD group x (blockIdx % 8) share one contiguous eighth of the tiles. Each
+// workgroup copies the next tile's field rows into its second LDS buffer (global->LDS, no
+// registers) and loads its densities while it computes the current tile; the two passes of a
+// tile synchronise over LDS only, so those copies stay in flight.
+template <int TJ, int TI>
+__global__ void __launch_bounds__(256) sor_pipe_kernel(SorParams p, const float* __restrict__ src,
+                                                       float* __restrict__ dst, const int32_t* __restrict__ done,
+                                                       int tiles_k, int tiles_j, int q_finite)
+{
+    using T = SorTile<TJ, TI>;
+    using I = TileIters<TJ, TI>;
+    __shared__ float s[T::kRows * kLK];
+    __shared__ float s2[T::kRows * kLK];
+    if (done && *done) return;
+    const int total = tiles_k * tiles_j * ((p.nx + TI - 1) / TI);
+    const int per_group = gridDim.x / 8, x = blockIdx.x & 7, local = blockIdx.x >> 3;
+    const int lo = (int)((int64_t)total * x / 8), hi = (int)((int64_t)total * (x + 1) / 8);
+    const bool qf = q_finite != 0;
+    auto origin_of = [&](int it, int& k0, int& j0, int& i0) {
+        k0 = (it % tiles_k) * kTK;
+        j0 = ((it / tiles_k) % tiles_j) * TJ;
+        i0 = (it / (tiles_k * tiles_j)) * TI;
+    };
+    auto prefetch = [&](int it, float* buf, float (&dr)[I::kRed], float (&db)[I::kBlack]) {
+        int k0, j0, i0;
+        origin_of(it, k0, j0, i0);
+        int t2 = threadIdx.x;
+        asm volatile("" : "+v"(t2));
+        const TileLane L(t2, k0, p.nz);
+        tile_copy_rows<TJ, TI>(p, src, buf, L, k0, j0, i0);
+        tile_densities<TJ, TI>(p, L, k0, j0, i0, dr, db);
+    };
+    auto compute = [&](int it, float* buf, const float (&dr)[I::kRed], const float (&db)[I::kBlack]) {
+        int k0, j0, i0;
+        origin_of(it, k0, j0, i0);
+        int t2 = threadIdx.x;
+        asm volatile("" : "+v"(t2));
+        const TileLane L(t2, k0, p.nz);
+        tile_passes<TJ, TI, false, true>(p, buf, dst, L, k0, j0, i0, qf, dr, db, false);
+    };
+    // Two-stage pipeline, unrolled by two so the density registers alternate without copies.
+    float ar[I::kRed], ab[I::kBlack], br[I::kRed], bb[I::kBlack];
+    int cur = lo + local;
+    if (cur < hi) prefetch(cur, s, ar, ab);
+    while (cur < hi) {
+        dma_barrier();  // s holds tile cur; s2 is free
+        int nxt = cur + per_group;
+        if (nxt < hi) prefetch(nxt, s2, br, bb);
+        compute(cur, s, ar, ab);
+        cur = nxt;
+        if (cur >= hi) break;
+        dma_barrier();  // s2 holds tile cur; s is free
+        nxt = cur + per_group;
+        if (nxt < hi) prefetch(nxt, s, ar, ab);
+        compute(cur, s2, br, bb);
+        cur = nxt;
+    }
+}
+
 // Residual with per-block maxima: block (64 x 4) covers a k-row segment of 4 rows and strides
 // over i. kAtomic = false stores one partial per block (reduced by residual_finish_kernel, no
 // same-address atomics); kAtomic = true folds the block maximum into out[0] with an unsigned
@@ -103,13 +185,17 @@ __global__ void __launch_bounds__(256) copy_f32_kernel(const float* __restrict__
 
 // ---- launch helpers ---------------------------------------------------------------------------
 // Tile shapes (TJ x TI; 60 along k) the autotuner chooses from.
-// kind 0: 3-D tile 60 x TJ x TI (this file); kind 1: 60 x TJ tile marching CI planes along i.
+// kind 0: 3-D tile 60 x TJ x TI (this file); kind 1: 60 x TJ tile marching CI planes along i;
+// kind 2: 3-D tile 60 x TJ x TI, pipelined persistent workgroups (this file); kind 3: 3-D tile
+// 60 x TJ x TI with global->LDS copies (this file).
 struct FusedConfig {
     int kind, tj, ti;
 };
 constexpr FusedConfig kFusedConfigs[] = {{0, 4, 4}, {0, 4, 8}, {0, 8, 4}, {0, 8, 8}, {0, 16, 4}, {0, 4, 16},
                                          {0, 2, 16}, {0, 16, 2}, {1, 12, 8}, {1, 12, 16}, {1, 12, 24},
-                                         {1, 28, 8}, {1, 28, 16}, {1, 12, 4}};
+                                         {1, 28, 8}, {1, 28, 16}, {1, 12, 4}, {2, 4, 4}, {2, 4, 8},
+                                         {2, 8, 4}, {2, 8, 8}, {3, 4, 4}, {3, 4, 8}, {3, 8, 4}, {3, 8, 8},
+                                         {3, 2, 8}, {3, 8, 2}};
 constexpr int kFusedConfigCount = sizeof(kFusedConfigs) / sizeof(kFusedConfigs[0]);
 int g_fused_config = 3;
 constexpr int kResidualIStride = 8;
@@ -120,6 +206,38 @@ static hipError_t launch_fused_tile(const SorParams& p, const float* src, float*
 {
     const int tk = (p.nz + kTK - 1) / kTK, tj = (p.ny + TJ - 1) / TJ, ti = (p.nx + TI - 1) / TI;
     sor_fused_kernel<TJ, TI><<<tk * tj * ti, 256, 0, s>>>(p, src, dst, done, tk, tj, qf);
+    return hipGetLastError();
+}
+
+template <int TJ, int TI>
+static hipError_t launch_copy_tile(const SorParams& p, const float* src, float* dst, const int32_t* done, int qf,
+                                   hipStream_t s)
+{
+    const int tk = (p.nz + kTK - 1) / kTK, tj = (p.ny + TJ - 1) / TJ, ti = (p.nx + TI - 1) / TI;
+    sor_copy_kernel<TJ, TI><<<tk * tj * ti, 256, 0, s>>>(p, src, dst, done, tk, tj, qf);
+    return hipGetLastError();
+}
+
+template <int TJ, int TI>
+static hipError_t launch_pipe_tile(const SorParams& p, const float* src, float* dst, const int32_t* done, int qf,
+                                   hipStream_t s)
+{
+    static int resident[16] = {0};  // workgroups resident at once, per device
+    int dev = 0;
+    WBC_TRY_ERR(hipGetDevice(&dev));
+    if (dev < 0 || dev >= 16) return hipErrorInvalidDevice;
+    if (resident[dev] == 0) {
+        int cus = 0, per_cu = 0;
+        WBC_TRY_ERR(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, dev));
+        WBC_TRY_ERR(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, sor_pipe_kernel<TJ, TI>, 256, 0));
+        resident[dev] = (per_cu > 0 ? per_cu : 1) * cus / 8 * 8;
+        if (resident[dev] < 8) resident[dev] = 8;
+    }
+    const int tk = (p.nz + kTK - 1) / kTK, tj = (p.ny + TJ - 1) / TJ, ti = (p.nx + TI - 1) / TI;
+    const int tiles = tk * tj * ti;
+    int grid = resident[dev];
+    if (grid > tiles) grid = (tiles + 7) / 8 * 8;  // a multiple of 8: every XCD group gets its share
+    sor_pipe_kernel<TJ, TI><<<grid, 256, 0, s>>>(p, src, dst, done, tk, tj, qf);
     return hipGetLastError();
 }
 
@@ -149,6 +267,16 @@ hipError_t launch_sor_fused_cfg(int cfg, const SorParams& p, const float* src, f
     case 5: return launch_fused_tile<4, 16>(p, src, dst, done, qf, s);
     case 6: return launch_fused_tile<2, 16>(p, src, dst, done, qf, s);
     case 7: return launch_fused_tile<16, 2>(p, src, dst, done, qf, s);
+    case 14: return launch_pipe_tile<4, 4>(p, src, dst, done, qf, s);
+    case 15: return launch_pipe_tile<4, 8>(p, src, dst, done, qf, s);
+    case 16: return launch_pipe_tile<8, 4>(p, src, dst, done, qf, s);
+    case 17: return launch_pipe_tile<8, 8>(p, src, dst, done, qf, s);
+    case 18: return launch_copy_tile<4, 4>(p, src, dst, done, qf, s);
+    case 19: return launch_copy_tile<4, 8>(p, src, dst, done, qf, s);
+    case 20: return launch_copy_tile<8, 4>(p, src, dst, done, qf, s);
+    case 21: return launch_copy_tile<8, 8>(p, src, dst, done, qf, s);
+    case 22: return launch_copy_tile<2, 8>(p, src, dst, done, qf, s);
+    case 23: return launch_copy_tile<8, 2>(p, src, dst, done, qf, s);
     default:
         if (cfg >= 8 && cfg < 8 + kMarchConfigCount) return launch_sor_march_cfg(cfg - 8, p, src, dst, done, qf, s);
         return hipErrorInvalidValue;

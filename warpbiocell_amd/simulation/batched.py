@@ -214,6 +214,10 @@ class BatchedSweep:
         """Enqueue ``steps`` cell steps of every replica (no host synchronisation)."""
         _lib.call("wbc_batch_step", self._h(), int(steps), 1 if use_graph else 0, self.stream)
 
+    def capture(self) -> None:
+        """Capture the step's hipGraph now (the first ``step(use_graph=True)`` would otherwise)."""
+        _lib.call("wbc_batch_step", self._h(), 0, 1, self.stream)
+
     def solve(self) -> None:
         """Only the steady-state solves of every replica, from their current fields."""
         _lib.call("wbc_batch_solve", self._h(), self.stream)

@@ -15,6 +15,10 @@ substeps per step). Cells are spheres with uniformly random centres in a ball.
                       1280 um; fixed 20 sweeps per step; 5 % of cells divide per step.
   adjoint_100k_129    100,000 cells as spheroid_100k_161 on a 129^3 grid over 1280 um, per-cell
                       uptake log-normal (median 5.538e7, sigma 0.3), targets uniform in [0, 40].
+  sweep_15            15 replicas = boundary {20, 40, 60, 100, 150} mmHg x seeds {0, 1, 2}; cell
+                      counts log-uniform in [2,000, 200,000] and grids drawn from {41, 81, 121,
+                      161}^3 over a 1600 um box (one draw per replica from a fixed generator);
+                      cells as spheroid_100k_161; solver as spheroid_1k_41; 100 steps.
 """
 
 from __future__ import annotations
@@ -48,6 +52,7 @@ class Workload:
     division_rate: float = 0.0
     capacity: int | None = None
     seed: int = 0
+    grid_nodes: int | None = None  # nodes per axis when set (dx = box / (nodes - 1)), else from dx
 
     @property
     def n(self) -> int:
@@ -55,6 +60,8 @@ class Workload:
 
     @property
     def grid_n(self) -> int:
+        if self.grid_nodes is not None:
+            return int(self.grid_nodes)
         return int(math.ceil(self.box_um / self.dx_um)) + 1
 
 
@@ -111,3 +118,25 @@ def adjoint_100k_129(seed: int = 0, n: int = 100_000):
     rates = (UPTAKE_MMHG_UM3_PER_H * np.exp(0.3 * rng.standard_normal(n))).astype(np.float32)
     targets = rng.uniform(0.0, 40.0, n).astype(np.float32)
     return w, rates, targets
+
+
+SWEEP_BOUNDARIES = (20.0, 40.0, 60.0, 100.0, 150.0)
+SWEEP_SEEDS = (0, 1, 2)
+SWEEP_GRIDS = (41, 81, 121, 161)
+
+
+def sweep_15(draw_seed: int = 0, box_um: float = 1600.0) -> list[Workload]:
+    """The replicas of the 15-run parameter sweep, in the order of the reference's sweep runs
+    (grid value outer, seed inner) and labelled as it labels them (``boundary_mmHg=20_seed=0``)."""
+    rng = np.random.default_rng(10_000 + draw_seed)
+    out = []
+    for ob in SWEEP_BOUNDARIES:
+        for seed in SWEEP_SEEDS:
+            n = int(round(math.exp(rng.uniform(math.log(2_000), math.log(200_000)))))
+            g = int(rng.choice(SWEEP_GRIDS))
+            w = spheroid_100k_161(seed=seed, n=n)
+            w.name = f"boundary_mmHg={ob:g}_seed={seed}"
+            w.boundary_mmHg = ob
+            w.box_um, w.dx_um, w.grid_nodes = box_um, box_um / (g - 1), g
+            out.append(w)
+    return out
