@@ -68,12 +68,23 @@ def lib() -> ctypes.CDLL:
                                             p, p, p, p, p, p, p, p, i64]),
             "orc_place_daughters": (None, [i32, f32, p, p, p, p, p, p, p, p, p, p, p, p, p, i64]),
             "orc_adjoint_sor_f64": (None, [i32] + [ctypes.c_double] * 4 + [p, p, p, p, i32, i32, i32]),
+            "orc_set_threads": (None, [i32]),
+            "orc_get_threads": (i32, []),
         }
         for name, (res, args) in sig.items():
             fn = getattr(L, name)
             fn.restype = res
             fn.argtypes = args
     return _lib
+
+
+def set_threads(n: int) -> None:
+    """Threads of the oracle's parallel loops (results do not depend on it)."""
+    lib().orc_set_threads(int(n))
+
+
+def get_threads() -> int:
+    return int(lib().orc_get_threads())
 
 
 def _p(a: np.ndarray | None):

@@ -26,8 +26,14 @@
  *   place_daughters          kernels/cell_kernels.py:157-200
  *   adjoint SOR (FP64)       no reference counterpart: transposed red-black solve used as the
  *                            FP64 gradient oracle for the steady-state adjoint
+ *
+ * Built with OpenMP: the loops without cross-iteration dependences (one colour pass of SOR,
+ * the residual maximum, the per-cell contact gather, integration, lifecycle decisions) run over
+ * orc_set_threads() threads. Results do not depend on the thread count: a colour pass reads only
+ * the other colour, a maximum is order-free, and every other loop writes per-iteration slots.
  */
 #include <math.h>
+#include <omp.h>
 #include <stdint.h>
 #include <stdlib.h>
 #include <string.h>
@@ -158,6 +164,10 @@ static float orc_acos_f32(float x)
     if (x < -0.5f) return pi - 2.0f * orc_asin_small(sqrtf((1.0f + x) * 0.5f));
     return pio2 - orc_asin_small(x);
 }
+
+/* Threads used by the parallel loops (the benchmark's CPU baseline runs 1 and all host cores). */
+void orc_set_threads(int32_t n) { omp_set_num_threads(n > 0 ? n : 1); }
+int32_t orc_get_threads(void) { return (int32_t)omp_get_max_threads(); }
 
 float orc_expf(float x) { return orc_exp_f32(x); }
 float orc_sinf(float x) { return orc_sin_f32(x); }
@@ -312,6 +322,7 @@ void orc_sample_trilinear(const float* field, int32_t nx, int32_t ny, int32_t nz
                              int32_t nx, int32_t ny, int32_t nz)                                            \
     {                                                                                                       \
         const int64_t plane = (int64_t)ny * nz;                                                             \
+        _Pragma("omp parallel for schedule(static)")                                                        \
         for (int32_t i = 0; i < nx; ++i)                                                                    \
             for (int32_t j = 0; j < ny; ++j)                                                                \
                 for (int32_t k = 0; k < nz; ++k) {                                                          \
@@ -336,6 +347,7 @@ void orc_sample_trilinear(const float* field, int32_t nx, int32_t ny, int32_t nz
     {                                                                                                       \
         const int64_t plane = (int64_t)ny * nz;                                                             \
         T res = (T)0;                                                                                       \
+        _Pragma("omp parallel for schedule(static) reduction(max : res)")                                   \
         for (int32_t i = 0; i < nx; ++i)                                                                    \
             for (int32_t j = 0; j < ny; ++j)                                                                \
                 for (int32_t k = 0; k < nz; ++k) {                                                          \
@@ -439,6 +451,7 @@ void orc_contact_velocities(const float* position, const float* radius, int64_t 
                             float* velocity, int32_t* neighbor_count)
 {
     const float inv_cell = 1.0f / cell_size;
+#pragma omp parallel for schedule(dynamic, 256)
     for (int64_t t = 0; t < n; ++t) {
         int32_t i = point_id[t];
         float xi = position[3 * i], yi = position[3 * i + 1], zi = position[3 * i + 2];
@@ -478,6 +491,7 @@ void orc_contact_velocities(const float* position, const float* radius, int64_t 
 
 void orc_integrate_positions(float* position, const float* velocity, int64_t n, float dt)
 {
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < 3 * n; ++i) position[i] = position[i] + velocity[i] * dt;
 }
 
@@ -520,6 +534,7 @@ void orc_lifecycle_decide(float dt, float death_rate, float anoxic_death_rate, f
                           const float* glucose_local, const int32_t* fate_flags, uint32_t* rng_state,
                           int32_t* divide_flag, int64_t n)
 {
+#pragma omp parallel for schedule(static)
     for (int64_t i = 0; i < n; ++i) {
         divide_flag[i] = 0;
         if (cell_state[i] == STATE_DEAD) continue;

@@ -214,3 +214,36 @@ def test_hypoxia_and_crowding_set_the_state(oracle):
     pop.neighbor_count[:n] = [0, 0, 0, 9, 9, 9]
     oracle.lifecycle_decide(pop, _lc(hypoxia_threshold=8.0, death_threshold=2.0, inhibition_threshold=8), 0.1)
     np.testing.assert_array_equal(pop.cell_state[:n], [2, 2, 0, 1, 2, 1])
+
+
+# ---- thread count ---------------------------------------------------------------------------------
+
+def test_oracle_step_does_not_depend_on_the_thread_count(oracle):
+    """The oracle's OpenMP loops (colour passes, residual maximum, contact gather, lifecycle) give
+    the same bits on one thread and on eight: a full step with deaths, contacts and a solve."""
+    from warpbiocell_amd import workloads as W
+
+    w = W.spheroid_1k_41()
+    a = oracle.SolveArgs(dx=w.dx_um, diffusion=W.D_UM2_PER_H, uptake_max=W.UPTAKE_MMHG_UM3_PER_H,
+                         michaelis_k=W.MICHAELIS_K, boundary_value=w.boundary_mmHg)
+    lifecycle = _lc(death_rate=2.0, hypoxia_threshold=20.0, death_threshold=5.0, anoxic_death_rate=1.0)
+    contact = SimpleNamespace(query_radius=10.0, rate=10.0)
+    stepping = SimpleNamespace(dt_cells=0.1, dt_mechanics=0.025, mechanics_substeps=4)
+    results = []
+    default_threads = oracle.get_threads()
+    for threads in (1, 8):
+        oracle.set_threads(threads)
+        assert oracle.get_threads() == threads
+        pop = oracle.OraclePopulation.from_numpy(w.positions, w.radii, seed=3)
+        n = w.grid_n
+        ox = oracle.OracleOxygen(shape=(n, n, n), origin=(-0.5 * (n - 1) * w.dx_um,) * 3, args=a, omega=1.8,
+                                 max_sweeps=200, tolerance=1e-6, check_every=10)
+        for _ in range(2):
+            oracle.oracle_cell_step(pop, lifecycle, contact, stepping, oxygen=ox)
+        results.append((pop, ox.values.copy()))
+    oracle.set_threads(default_threads)
+    (p1, f1), (p8, f8) = results
+    assert np.array_equal(f1.view(np.uint32), f8.view(np.uint32))
+    for name in ("position", "velocity", "neighbor_count", "cell_state", "age", "rng_state", "oxygen_local"):
+        assert np.array_equal(getattr(p1, name).view(np.uint8), getattr(p8, name).view(np.uint8)), name
+    assert (p1.cell_state[: p1.count] == 3).any()  # deaths happened
