@@ -48,7 +48,8 @@ def setup(workload_name: str, device):
 
 
 KERNEL_OF_KIND = {"tile": "sor_fused_kernel<{tj}, {ti}>", "march": "sor_march_kernel<{tj}, {ti}>",
-                  "pipe": "sor_pipe_kernel<{tj}, {ti}>", "copy": "sor_copy_kernel<{tj}, {ti}>"}
+                  "pipe": "sor_pipe_kernel<{tj}, {ti}>", "copy": "sor_copy_kernel<{tj}, {ti}>",
+                  "packed": "sor_packed_kernel<C, {tj}>"}
 COUNTER_TABLE = ROOT / "profiles" / "sor_counters_spheroid_100k_161.md"
 
 
@@ -65,49 +66,52 @@ def measured_traffic(shape: str, workload: str):
 
 
 def sor_roofline(ox, geom, device, workload: str, launches: int = 200) -> dict:
-    """Average duration of one fused SOR sweep launch, measured with HIP events on its stream.
+    """Average duration of one red+black SOR sweep with the selected shape, measured with HIP
+    events on its stream over `launches` sweeps in one multi-sweep call.
 
     Algorithmic bytes of one red+black sweep: 16 B per node (two colour passes each reading the
-    field, one field write, one density read). The fused kernel reads the field once, so its own
-    minimum is 12 B per node (`achieved_at_12B`)."""
+    field, one field write, one density read). The fused kernels read the field once, so their own
+    minimum is 12 B per node (`achieved_at_12B`). A fused shape is one launch per sweep; the
+    colour-packed shape is two launches per sweep (one per colour), and its pack/unpack at the ends
+    of the call are inside the timed region, amortised over the sweeps."""
     import torch
 
     from warpbiocell_amd import _lib
+    from warpbiocell_amd.autotune import shape_name
     from warpbiocell_amd.fields.diffusion import solver_scalars
 
     kin = ox.species.kinetics
     dx2_over_D, _ = solver_scalars(ox.field, kin)
     stream = torch.cuda.current_stream(device)
-    bufs = [ox.field.values.clone(), torch.empty_like(ox.field.values)]
-
-    def sweep(c):
-        _lib.call("wbc_sor_sweep_fused", ox.settings.omega, dx2_over_D, kin.uptake_max, kin.michaelis_k, None,
-                  ox.density.data_ptr(), None, 0, None, 1.0, bufs[c & 1].data_ptr(), bufs[(c + 1) & 1].data_ptr(),
-                  *geom.shape, stream.cuda_stream)
-
-    for c in range(4):
-        sweep(c)
-    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
-    start.record(stream)
-    for c in range(launches):
-        sweep(c)
-    end.record(stream)
-    end.synchronize()
-    per_launch_s = start.elapsed_time(end) / 1e3 / launches
-    bytes_per_launch = 16 * geom.node_count
-    achieved = bytes_per_launch / per_launch_s / 1e9
     cfg = _lib.query("wbc_sor_fused_get_config")
     kind, tj, ti = (ctypes.c_int32() for _ in range(3))
     _lib.call("wbc_sor_fused_config_shape", cfg, ctypes.byref(kind), ctypes.byref(tj), ctypes.byref(ti))
-    from warpbiocell_amd.autotune import shape_name
+    src = ox.field.values.clone()
+    dst = torch.empty_like(src)
+    ws = torch.empty(_lib.query("wbc_sor_sweeps_workspace_bytes", *geom.shape), dtype=torch.uint8, device=device)
 
-    family = {0: "tile", 1: "march", 2: "pipe", 3: "copy"}[kind.value]
+    def sweeps(n):
+        _lib.call("wbc_sor_sweeps_cfg", cfg, n, ox.settings.omega, dx2_over_D, kin.uptake_max, kin.michaelis_k, None,
+                  ox.density.data_ptr(), None, 0, None, 1.0, src.data_ptr(), dst.data_ptr(), *geom.shape,
+                  ws.data_ptr(), ws.numel(), stream.cuda_stream)
+
+    sweeps(4)
+    start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+    start.record(stream)
+    sweeps(launches)
+    end.record(stream)
+    end.synchronize()
+    per_sweep_s = start.elapsed_time(end) / 1e3 / launches
+    bytes_per_sweep = 16 * geom.node_count
+    achieved = bytes_per_sweep / per_sweep_s / 1e9
+    family = {0: "tile", 1: "march", 2: "pipe", 3: "copy", 4: "packed"}[kind.value]
     shape = shape_name(kind.value, tj.value, ti.value)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(shape, workload),
             "kernel": KERNEL_OF_KIND[family].format(tj=tj.value, ti=ti.value), "shape": shape,
-            "launch_us": round(per_launch_s * 1e6, 3), "algorithmic_bytes_per_launch": bytes_per_launch,
-            "achieved_at_12B": round(12 * geom.node_count / per_launch_s / 1e9, 1),
+            "launches_per_sweep": 2 if family == "packed" else 1, "sweep_us": round(per_sweep_s * 1e6, 3),
+            "algorithmic_bytes_per_sweep": bytes_per_sweep,
+            "achieved_at_12B": round(12 * geom.node_count / per_sweep_s / 1e9, 1),
             "traffic_source": str(COUNTER_TABLE.relative_to(ROOT))}
 
 
@@ -319,7 +323,7 @@ def main() -> int:
 
     roof = None if args.no_roofline else sor_roofline(ox, geom, device, w.name)
     if roof:
-        log(f"  sor fused sweep: {roof['launch_us']} us, {roof['achieved']} GB/s algorithmic")
+        log(f"  sor sweep ({roof['shape']}): {roof['sweep_us']} us, {roof['achieved']} GB/s algorithmic")
     adj = None if args.no_adjoint else adjoint_measurement(device)
     if adj:
         log(f"  adjoint: {adj}")

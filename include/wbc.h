@@ -101,10 +101,16 @@ int wbc_sor_sweep_fused(float omega, float dx2_over_D, float uptake_max, float m
                         wbc_stream_t stream);
 
 /* Shape table of the fused sweep, entries (kind, TJ, TI): kind 0 is a 3-D tile of 60 x TJ x TI
- * nodes (k, j, i); kind 1 a 60 x TJ tile marching TI planes along i. The autotuner times every
- * entry on the caller's grid and selects one; the solve loop uses the selected entry.
- * wbc_sor_sweep_fused_cfg runs entry `cfg` (-1: the selected one). No reference counterpart:
- * Warp picks launch shapes itself. */
+ * nodes (k, j, i); kind 1 a 60 x TJ tile marching TI planes along i; kind 2 the 3-D tile on
+ * pipelined persistent workgroups; kind 3 the 3-D tile with global->LDS copies; kind 4 the two
+ * colour passes on colour-packed arrays (field and density of each colour contiguous per row;
+ * Dirichlet faces, no pinned nodes, no production). The autotuner times every entry on the
+ * caller's grid and selects one; the solve loop uses the selected entry (kind 4: packing the
+ * field once per solve). wbc_sor_sweep_fused_cfg runs entry `cfg` (-1: the selected one) for one
+ * sweep; kind 4 is not available there (a lone sweep under that selection takes the 60 x 4 x 8
+ * copy tile). wbc_sor_sweeps_cfg runs `sweeps` sweeps of any entry from src into dst through a
+ * workspace of wbc_sor_sweeps_workspace_bytes. No reference counterpart: Warp picks launch shapes
+ * itself. */
 int wbc_sor_fused_config_count(void);
 int wbc_sor_fused_config_shape(int32_t cfg, int32_t* kind, int32_t* tj, int32_t* ti);
 int wbc_sor_fused_set_config(int32_t cfg);
@@ -113,6 +119,12 @@ int wbc_sor_sweep_fused_cfg(int32_t cfg, float omega, float dx2_over_D, float up
                             const int32_t* fixed, const float* density, const float* production, int32_t has_source,
                             const float* source, float source_k, const float* src, float* dst, int32_t nx,
                             int32_t ny, int32_t nz, wbc_stream_t stream);
+size_t wbc_sor_sweeps_workspace_bytes(int32_t nx, int32_t ny, int32_t nz);
+int wbc_sor_sweeps_cfg(int32_t cfg, int32_t sweeps, float omega, float dx2_over_D, float uptake_max,
+                       float michaelis_k, const int32_t* fixed, const float* density, const float* production,
+                       int32_t has_source, const float* source, float source_k, const float* src, float* dst,
+                       int32_t nx, int32_t ny, int32_t nz, void* workspace, size_t workspace_bytes,
+                       wbc_stream_t stream);
 
 /* Replaces the steady_state_residual launch, fields/diffusion.py:114-119 (kernel
  * kernels/field_kernels.py:215-243): *residual = max(*residual, max over updated nodes). The

@@ -99,8 +99,22 @@ def test_colour_passes_and_residual_are_bit_exact(oracle, device, boundary, pinn
     assert np.float32(gpu_res) == np.float32(oracle.residual(ref, density, a, fixed=fx, production=production))
 
 
-def test_solve_loop_matches_the_oracle_report_and_field(oracle, device):
-    """The whole chunked loop: same sweeps, same residual, same field bits (41^3 spheroid)."""
+@pytest.mark.parametrize("shape_name", ["tile60x8x8", "copy60x4x8", "packed/2", "packed/8"])
+def test_solve_loop_matches_the_oracle_report_and_field(oracle, device, shape_name):
+    """The whole chunked loop: same sweeps, same residual, same field bits (41^3 spheroid), with
+    the solve loop's sweeps on a one-launch tile and on the colour-packed arrays."""
+    from warpbiocell_amd import _lib
+    from warpbiocell_amd.autotune import config_shapes
+
+    previous = _lib.query("wbc_sor_fused_get_config")
+    _lib.call("wbc_sor_fused_set_config", config_shapes().index(shape_name))
+    try:
+        _solve_loop_matches(oracle, device)
+    finally:
+        _lib.call("wbc_sor_fused_set_config", previous)
+
+
+def _solve_loop_matches(oracle, device):
     w = W.spheroid_1k_41()
     geom = GridGeometry.centered_cube(w.box_um, w.dx_um)
     settings = SolverSettings(omega=1.8, tolerance=1e-6, max_sweeps=2000, check_every=10)
@@ -179,14 +193,30 @@ def test_fused_sweep_equals_two_colour_passes(oracle, device, shape, pinned):
         for color in (0, 1):
             oracle.sor_sweep(r, density, a, 1.8, color, fixed=fixed, production=production)
         refs.append(r)
-    for cfg in range(_lib.query("wbc_sor_fused_config_count")):
-        bufs[0].copy_(torch.from_numpy(values))
-        for sweep in range(4):
-            _lib.call("wbc_sor_sweep_fused_cfg", cfg, 1.8, a.dx2_over_D, a.uptake_max, a.michaelis_k, _lib.ptr(f_t),
-                      d_t.data_ptr(), _lib.ptr(p_t), 0, None, 1.0, bufs[sweep % 2].data_ptr(),
-                      bufs[(sweep + 1) % 2].data_ptr(), *shape, _lib.stream_of(device))
-            np.testing.assert_array_equal(bufs[(sweep + 1) % 2].cpu().numpy(), refs[sweep + 1],
-                                          err_msg=f"config {cfg}, sweep {sweep}")
+    from warpbiocell_amd.autotune import config_shapes
+
+    nbytes = _lib.query("wbc_sor_sweeps_workspace_bytes", *shape)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
+    for cfg, name in enumerate(config_shapes()):
+        if name.startswith("packed") and pinned:
+            continue  # the colour-packed entry runs only without pinned nodes and production
+        if not name.startswith("packed"):
+            # one launch per sweep, src -> dst
+            bufs[0].copy_(torch.from_numpy(values))
+            for sweep in range(4):
+                _lib.call("wbc_sor_sweep_fused_cfg", cfg, 1.8, a.dx2_over_D, a.uptake_max, a.michaelis_k,
+                          _lib.ptr(f_t), d_t.data_ptr(), _lib.ptr(p_t), 0, None, 1.0, bufs[sweep % 2].data_ptr(),
+                          bufs[(sweep + 1) % 2].data_ptr(), *shape, _lib.stream_of(device))
+                np.testing.assert_array_equal(bufs[(sweep + 1) % 2].cpu().numpy(), refs[sweep + 1],
+                                              err_msg=f"config {name}, sweep {sweep}")
+        # several sweeps in one call (packing once for the colour-packed entry)
+        for sweeps in (1, 2, 3):
+            bufs[0].copy_(torch.from_numpy(values))
+            _lib.call("wbc_sor_sweeps_cfg", cfg, sweeps, 1.8, a.dx2_over_D, a.uptake_max, a.michaelis_k,
+                      _lib.ptr(f_t), d_t.data_ptr(), _lib.ptr(p_t), 0, None, 1.0, bufs[0].data_ptr(),
+                      bufs[1].data_ptr(), *shape, ws.data_ptr(), nbytes, _lib.stream_of(device))
+            np.testing.assert_array_equal(bufs[1].cpu().numpy(), refs[sweeps],
+                                          err_msg=f"config {name}, {sweeps} sweeps in one call")
 
 
 def test_depleted_core_solve_is_bit_exact(oracle, device):

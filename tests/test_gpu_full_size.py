@@ -54,13 +54,14 @@ def test_fused_sweeps_bit_exact_at_full_size(oracle, c1, device):
     for sweep in range(2):
         for color in (0, 1):
             oracle.sor_sweep(ref, density, a, settings.omega, color)
+    nbytes = _lib.query("wbc_sor_sweeps_workspace_bytes", *geom.shape)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=device)
     for cfg in range(_lib.query("wbc_sor_fused_config_count")):
         bufs = [torch.from_numpy(start).to(device), torch.empty_like(ox.field.values)]
-        for sweep in range(2):
-            _lib.call("wbc_sor_sweep_fused_cfg", cfg, settings.omega, a.dx2_over_D, a.uptake_max, a.michaelis_k, None,
-                      ox.density.data_ptr(), None, 0, None, 1.0, bufs[sweep % 2].data_ptr(),
-                      bufs[(sweep + 1) % 2].data_ptr(), *geom.shape, _lib.stream_of(device))
-        np.testing.assert_array_equal(bufs[0].cpu().numpy(), ref, err_msg=f"config {cfg}")
+        _lib.call("wbc_sor_sweeps_cfg", cfg, 2, settings.omega, a.dx2_over_D, a.uptake_max, a.michaelis_k, None,
+                  ox.density.data_ptr(), None, 0, None, 1.0, bufs[0].data_ptr(), bufs[1].data_ptr(), *geom.shape,
+                  ws.data_ptr(), nbytes, _lib.stream_of(device))
+        np.testing.assert_array_equal(bufs[1].cpu().numpy(), ref, err_msg=f"config {cfg}")
 
 
 def test_reported_residual_is_the_oracle_residual_of_the_gpu_field(oracle, c1):

@@ -43,6 +43,9 @@ SIGNATURES: dict[str, tuple] = {
     "wbc_sor_fused_set_config": (I32, [I32]),
     "wbc_sor_fused_get_config": (I32, []),
     "wbc_sor_sweep_fused_cfg": (I32, [I32, F32, F32, F32, F32, P, P, P, I32, P, F32, P, P, I32, I32, I32, P]),
+    "wbc_sor_sweeps_workspace_bytes": (c_size_t, [I32, I32, I32]),
+    "wbc_sor_sweeps_cfg": (I32, [I32, I32, F32, F32, F32, F32, P, P, P, I32, P, F32, P, P, I32, I32, I32, P, c_size_t,
+                                 P]),
     "wbc_solve_workspace_bytes": (c_size_t, [I32, I32, I32, I32, I32]),
     "wbc_solve_steady_state": (I32, [F32, I32, F32, I32, F32, F32, F32, F32, I32, I32, I32, P, P, P, I32, P, F32,
                                      P, I32, I32, I32, P, c_size_t, P, P, P, P, P]),

@@ -21,13 +21,15 @@ _cache: dict = {}
 
 
 def shape_name(kind: int, tj: int, ti: int) -> str:
-    return {0: f"tile60x{tj}x{ti}", 1: f"march60x{tj}/{ti}", 2: f"pipe60x{tj}x{ti}", 3: f"copy60x{tj}x{ti}"}[kind]
+    return {0: f"tile60x{tj}x{ti}", 1: f"march60x{tj}/{ti}", 2: f"pipe60x{tj}x{ti}", 3: f"copy60x{tj}x{ti}",
+            4: f"packed/{tj}"}[kind]
 
 
 def config_shapes() -> list[str]:
     """Names of the shape table: 'tile60xTJxTI' (3-D tile), 'march60xTJ/CI' (marching i) or
-    'pipe60xTJxTI' (3-D tile, pipelined persistent workgroups) or 'copy60xTJxTI' (3-D tile,
-    global->LDS copies)."""
+    'pipe60xTJxTI' (3-D tile, pipelined persistent workgroups), 'copy60xTJxTI' (3-D tile,
+    global->LDS copies) or 'packed/RJ' (colour passes on colour-packed arrays, RJ rows per
+    thread)."""
     out = []
     for c in range(_lib.query("wbc_sor_fused_config_count")):
         kind, tj, ti = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
@@ -37,39 +39,55 @@ def config_shapes() -> list[str]:
 
 
 def time_config(cfg: int, field, density: torch.Tensor, kinetics, omega: float, sweeps: int = 50) -> float:
-    """Microseconds per fused sweep with tile shape ``cfg`` on ``field``'s grid (values untouched)."""
+    """Microseconds per sweep with shape ``cfg`` on ``field``'s grid (values untouched): ``sweeps``
+    sweeps in one ``wbc_sor_sweeps_cfg`` call (for the colour-packed entry that includes packing
+    once and unpacking once), after one warm-up call."""
     from warpbiocell_amd.fields.diffusion import solver_scalars
 
     dev = field.device
     stream = torch.cuda.current_stream(dev)
     dx2_over_D, _ = solver_scalars(field, kinetics)
-    bufs = [field.values.clone(), torch.empty_like(field.values)]
+    nx, ny, nz = field.geometry.shape
+    src, dst = field.values.clone(), torch.empty_like(field.values)
+    nbytes = _lib.query("wbc_sor_sweeps_workspace_bytes", nx, ny, nz)
+    ws = torch.empty(nbytes, dtype=torch.uint8, device=dev)
 
-    def sweep(c):
-        _lib.call("wbc_sor_sweep_fused_cfg", cfg, omega, dx2_over_D, kinetics.uptake_max, kinetics.michaelis_k,
-                  _lib.ptr(field.fixed_or_none), density.data_ptr(), None, 0, None, 1.0, bufs[c & 1].data_ptr(),
-                  bufs[(c + 1) & 1].data_ptr(), *field.geometry.shape, stream.cuda_stream)
+    def run(n):
+        _lib.call("wbc_sor_sweeps_cfg", cfg, n, omega, dx2_over_D, kinetics.uptake_max, kinetics.michaelis_k,
+                  _lib.ptr(field.fixed_or_none), density.data_ptr(), None, 0, None, 1.0, src.data_ptr(),
+                  dst.data_ptr(), nx, ny, nz, ws.data_ptr(), nbytes, stream.cuda_stream)
 
-    for c in range(4):
-        sweep(c)
+    run(4)
     a, b = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     a.record(stream)
-    for c in range(sweeps):
-        sweep(c)
+    run(sweeps)
     b.record(stream)
     b.synchronize()
     return a.elapsed_time(b) * 1e3 / sweeps
 
 
+def supported_configs(field) -> list[int]:
+    """Shape-table entries that can run on ``field`` (the colour-packed one needs Dirichlet faces
+    and no pinned nodes; production terms are the caller's concern)."""
+    out = []
+    for c, name in enumerate(config_shapes()):
+        if name.startswith("packed") and (field.fixed_or_none is not None or any(field.boundary_flags)):
+            continue
+        out.append(c)
+    return out
+
+
 def tune_sor(field, density: torch.Tensor, kinetics, omega: float, sweeps: int = 50) -> dict:
     """Time every tile shape, select the fastest; returns {shape: us} and the choice."""
-    key = (str(field.device), tuple(field.geometry.shape))
+    key = (str(field.device), tuple(field.geometry.shape), field.fixed_or_none is not None,
+           tuple(field.boundary_flags))
     if key not in _cache:
         shapes = config_shapes()
-        times = [time_config(c, field, density, kinetics, omega, sweeps) for c in range(len(shapes))]
-        best = min(range(len(shapes)), key=lambda c: times[c])
+        cfgs = supported_configs(field)
+        times = {c: time_config(c, field, density, kinetics, omega, sweeps) for c in cfgs}
+        best = min(times, key=times.get)
         _cache[key] = {"choice": best, "shape": shapes[best],
-                       "us_per_sweep": {s: round(t, 3) for s, t in zip(shapes, times)}}
+                       "us_per_sweep": {shapes[c]: round(t, 3) for c, t in times.items()}}
     _lib.call("wbc_sor_fused_set_config", _cache[key]["choice"])
     return _cache[key]
 

@@ -294,18 +294,25 @@ hipError_t get_events(hipEvent_t** out)
 
 namespace {
 struct SolveLayout {
-    size_t state, slots, partial, scratch, total;
+    size_t state, slots, partial, scratch, packed, packed_each, total;
 };
 
-SolveLayout solve_layout(int32_t chunks, int32_t partials, int64_t nodes)
+// Residual partials of either residual kernel, a second field buffer for the ping-pong sweeps
+// and the four colour-packed arrays (field and density per colour) for the packed sweeps.
+SolveLayout solve_layout(int32_t chunks, const SorParams& p)
 {
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    const int32_t partials = residual_partials(p) > residual_packed_partials(p) ? residual_partials(p)
+                                                                                : residual_packed_partials(p);
+    const int64_t nodes = (int64_t)p.nx * p.ny * p.nz;
     SolveLayout L;
     size_t off = 0;
     L.state = off;   off += up(4 * sizeof(int32_t));
     L.slots = off;   off += up((size_t)chunks * sizeof(float));
     L.partial = off; off += up((size_t)partials * sizeof(float));
     L.scratch = off; off += up((size_t)nodes * sizeof(float));
+    L.packed_each = up((size_t)packed_floats(p) * sizeof(float));
+    L.packed = off;  off += 4 * L.packed_each;
     L.total = off;
     return L;
 }
@@ -321,7 +328,7 @@ WBC_API size_t wbc_solve_workspace_bytes(int32_t max_sweeps, int32_t check_every
     p.ny = ny;
     p.nz = nz;
     int32_t chunks = (max_sweeps + check_every - 1) / check_every;
-    return solve_layout(chunks, residual_partials(p), (int64_t)nx * ny * nz).total;
+    return solve_layout(chunks, p).total;
 }
 
 WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolerance, int32_t check_every,
@@ -342,19 +349,26 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
     size_t need = wbc_solve_workspace_bytes(max_sweeps, check_every, nx, ny, nz);
     if (need == 0 || workspace_bytes < need) return (int)hipErrorInvalidValue;
     const int32_t chunks = (max_sweeps + check_every - 1) / check_every;
-    const int32_t partials = residual_partials(p);
     const int64_t nodes = (int64_t)nx * ny * nz;
-    const SolveLayout L = solve_layout(chunks, partials, nodes);
+    const SolveLayout L = solve_layout(chunks, p);
     char* ws = (char*)workspace;
     int32_t* state = (int32_t*)(ws + L.state);  // [0] converged latch, [1] converged chunk
     float* slots = (float*)(ws + L.slots);      // one residual per chunk
     float* partial = (float*)(ws + L.partial);
     float* buf[2] = {field, (float*)(ws + L.scratch)};
-    const bool fused = fused_supported(p);
+    // The colour-packed sweeps when selected and applicable (pack once, unpack at the end), else
+    // out-of-place fused sweeps on all-Dirichlet grids, else the two colour passes in place.
+    const int packed_rows = packed_supported(p) ? packed_rows_of(g_fused_config) : 0;
+    const bool packed = packed_rows != 0;
+    const bool fused = !packed && fused_supported(p);
+    const int32_t partials = packed ? residual_packed_partials(p) : residual_partials(p);
+    PackedField pf{{(float*)(ws + L.packed), (float*)(ws + L.packed + L.packed_each)},
+                   {(float*)(ws + L.packed + 2 * L.packed_each), (float*)(ws + L.packed + 3 * L.packed_each)}};
     hipEvent_t* ev;
     WBC_TRY(get_events(&ev));
     zero_i32_kernel<<<1, 256, 0, stream>>>(state, 4);
     WBC_CHECK_LAUNCH();
+    if (packed) WBC_TRY(launch_pack(p, field, density, pf, stream));
 
     auto chunk_len = [&](int32_t c) {
         return max_sweeps - c * check_every < check_every ? max_sweeps - c * check_every : check_every;
@@ -367,7 +381,9 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
         while (queued < chunks && queued - checked <= kLag) {
             const int32_t len = chunk_len(queued);
             for (int32_t s = 0; s < len; ++s) {
-                if (fused) {
+                if (packed) {
+                    WBC_TRY(launch_sor_packed(p, pf, packed_rows, state, stream));
+                } else if (fused) {
                     WBC_TRY(launch_sor_fused(p, buf[cur], buf[cur ^ 1], state, stream));
                     cur ^= 1;
                 } else {
@@ -375,7 +391,10 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
                     WBC_TRY(launch_sor_color(p, 1, field, state, stream));
                 }
             }
-            WBC_TRY(launch_residual_partial(p, buf[cur], partial, state, stream));
+            if (packed)
+                WBC_TRY(launch_residual_packed(p, pf, partial, state, stream));
+            else
+                WBC_TRY(launch_residual_partial(p, buf[cur], partial, state, stream));
             WBC_TRY(launch_residual_finish(partial, partials, slots, queued, tolerance, state, stream));
             WBC_TRY(hipMemcpyAsync(host_residual + queued, slots + queued, sizeof(float), hipMemcpyDeviceToHost,
                                    stream));
@@ -392,8 +411,9 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
             break;
         }
     }
-    // The field after the last executed chunk is in buf[sweeps % 2] (fused path); chunks queued
-    // after convergence did nothing.
+    // The field after the last executed chunk is in buf[sweeps % 2] (fused path) or in the packed
+    // arrays; chunks queued after convergence did nothing.
+    if (packed) WBC_TRY(launch_unpack(p, pf, field, stream));
     if (fused) result_buf = sweeps & 1;
     if (result_buf != 0) WBC_TRY(launch_copy_f32(buf[1], field, nodes, stream));
     *sweeps_out = sweeps;

@@ -57,6 +57,28 @@ hipError_t launch_residual_finish(const float* partial, int count, float* residu
 hipError_t launch_residual_atomic(const SorParams& p, const float* field, float* out, hipStream_t s);
 hipError_t launch_copy_f32(const float* a, float* b, int64_t n, hipStream_t s);
 
+// Colour-packed layout (wbc_sor_packed.hip): per colour, one field and one density array of
+// packed_floats(p) floats each. Dirichlet faces, no pinned nodes, no production only.
+struct PackedField {
+    float* f[2];
+    float* d[2];
+};
+bool packed_supported(const SorParams& p);
+int64_t packed_floats(const SorParams& p);
+int residual_packed_partials(const SorParams& p);
+hipError_t launch_pack(const SorParams& p, const float* field, const float* density, PackedField& pf, hipStream_t s);
+hipError_t launch_unpack(const SorParams& p, const PackedField& pf, float* field, hipStream_t s);
+// One sweep (both colour passes); each thread takes `rows` (2, 4 or 8) consecutive rows j.
+hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, const int32_t* done, hipStream_t s);
+hipError_t launch_residual_packed(const SorParams& p, const PackedField& pf, float* partial, const int32_t* done,
+                                  hipStream_t s);
+
+// The sweep shape table (wbc_sor_fused.hip): the entry the solve loop uses, and the colour-packed
+// entries (which the solve loop runs on packed arrays for its whole duration): rows per thread,
+// or 0 for an entry that is not colour-packed.
+extern int g_fused_config;
+int packed_rows_of(int cfg);
+
 // 2-D-tiled variant that marches along i (wbc_sor_march.hip); cfg indexes its own shape table.
 constexpr int kMarchConfigCount = 6;
 hipError_t launch_sor_march_cfg(int cfg, const SorParams& p, const float* src, float* dst, const int32_t* done,

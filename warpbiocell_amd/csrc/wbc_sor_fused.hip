@@ -187,7 +187,8 @@ __global__ void __launch_bounds__(256) copy_f32_kernel(const float* __restrict__
 // Tile shapes (TJ x TI; 60 along k) the autotuner chooses from.
 // kind 0: 3-D tile 60 x TJ x TI (this file); kind 1: 60 x TJ tile marching CI planes along i;
 // kind 2: 3-D tile 60 x TJ x TI, pipelined persistent workgroups (this file); kind 3: 3-D tile
-// 60 x TJ x TI with global->LDS copies (this file).
+// 60 x TJ x TI with global->LDS copies (this file); kind 4: two colour passes on colour-packed
+// arrays (wbc_sor_packed.hip), which only the multi-sweep entry point and the solve loop run.
 struct FusedConfig {
     int kind, tj, ti;
 };
@@ -195,8 +196,13 @@ constexpr FusedConfig kFusedConfigs[] = {{0, 4, 4}, {0, 4, 8}, {0, 8, 4}, {0, 8,
                                          {0, 2, 16}, {0, 16, 2}, {1, 12, 8}, {1, 12, 16}, {1, 12, 24},
                                          {1, 28, 8}, {1, 28, 16}, {1, 12, 4}, {2, 4, 4}, {2, 4, 8},
                                          {2, 8, 4}, {2, 8, 8}, {3, 4, 4}, {3, 4, 8}, {3, 8, 4}, {3, 8, 8},
-                                         {3, 2, 8}, {3, 8, 2}};
+                                         {3, 2, 8}, {3, 8, 2}, {4, 2, 0}, {4, 4, 0}, {4, 8, 0}};
 constexpr int kFusedConfigCount = sizeof(kFusedConfigs) / sizeof(kFusedConfigs[0]);
+
+int packed_rows_of(int cfg)
+{
+    return (cfg >= 0 && cfg < kFusedConfigCount && kFusedConfigs[cfg].kind == 4) ? kFusedConfigs[cfg].tj : 0;
+}
 int g_fused_config = 3;
 constexpr int kResidualIStride = 8;
 
@@ -283,9 +289,14 @@ hipError_t launch_sor_fused_cfg(int cfg, const SorParams& p, const float* src, f
     }
 }
 
+// A single sweep with the selected shape. The colour-packed entry needs its arrays for a whole
+// solve, so a lone sweep under that selection takes the 60 x 4 x 8 copy tile.
+constexpr int kSingleSweepFallback = 19;
+
 hipError_t launch_sor_fused(const SorParams& p, const float* src, float* dst, const int32_t* done, hipStream_t s)
 {
-    return launch_sor_fused_cfg(g_fused_config, p, src, dst, done, s);
+    return launch_sor_fused_cfg(packed_rows_of(g_fused_config) ? kSingleSweepFallback : g_fused_config, p, src, dst,
+                                done, s);
 }
 
 hipError_t launch_residual_partial(const SorParams& p, const float* field, float* partial, const int32_t* done,
@@ -352,10 +363,11 @@ WBC_API int wbc_sor_sweep_fused_cfg(int32_t cfg, float omega, float dx2_over_D, 
     if (nx < 3 || ny < 3 || nz < 3 || (int64_t)nx * ny * nz >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;
     if (!density || !src || !dst || src == dst) return (int)hipErrorInvalidValue;
     if (has_source && (!source || !production)) return (int)hipErrorInvalidValue;
-    if (cfg < -1 || cfg >= kFusedConfigCount) return (int)hipErrorInvalidValue;
+    if (cfg < -1 || cfg >= kFusedConfigCount || packed_rows_of(cfg)) return (int)hipErrorInvalidValue;
     SorParams p{omega, dx2_over_D, uptake_max, michaelis_k, 1.0f, 0, 0, 0, fixed, density, production,
                 has_source, source, source_k, nx, ny, nz};
-    return (int)launch_sor_fused_cfg(cfg < 0 ? g_fused_config : cfg, p, src, dst, nullptr, stream);
+    if (cfg < 0) return (int)launch_sor_fused(p, src, dst, nullptr, stream);
+    return (int)launch_sor_fused_cfg(cfg, p, src, dst, nullptr, stream);
 }
 
 // One fused red+black sweep, src -> dst (distinct buffers, Dirichlet faces only).
@@ -370,4 +382,65 @@ WBC_API int wbc_sor_sweep_fused(float omega, float dx2_over_D, float uptake_max,
     SorParams p{omega, dx2_over_D, uptake_max, michaelis_k, 1.0f, 0, 0, 0, fixed, density, production,
                 has_source, source, source_k, nx, ny, nz};
     return (int)launch_sor_fused(p, src, dst, nullptr, stream);
+}
+
+// ---- several sweeps with one shape ----------------------------------------------------------
+namespace {
+struct SweepsLayout {
+    size_t tmp, packed, total;
+};
+SweepsLayout sweeps_layout(int32_t nx, int32_t ny, int32_t nz)
+{
+    auto up = [](size_t b) { return (b + 255) / 256 * 256; };
+    SweepsLayout L;
+    const size_t nodes = (size_t)nx * ny * nz, half = (size_t)nx * ny * ((nz + 1) / 2);
+    L.tmp = 0;
+    L.packed = up(nodes * sizeof(float));
+    L.total = L.packed + 4 * up(half * sizeof(float));
+    return L;
+}
+}  // namespace
+
+WBC_API size_t wbc_sor_sweeps_workspace_bytes(int32_t nx, int32_t ny, int32_t nz)
+{
+    if (nx < 3 || ny < 3 || nz < 3 || (int64_t)nx * ny * nz >= ((int64_t)1 << 31)) return 0;
+    return sweeps_layout(nx, ny, nz).total;
+}
+
+// `sweeps` sweeps with shape `cfg` (-1: the selected one) from src into dst (distinct buffers;
+// src unchanged). Tile shapes ping-pong through the workspace; the colour-packed entry packs src
+// once, runs its passes and unpacks into dst.
+WBC_API int wbc_sor_sweeps_cfg(int32_t cfg, int32_t sweeps, float omega, float dx2_over_D, float uptake_max,
+                               float michaelis_k, const int32_t* fixed, const float* density, const float* production,
+                               int32_t has_source, const float* source, float source_k, const float* src, float* dst,
+                               int32_t nx, int32_t ny, int32_t nz, void* workspace, size_t workspace_bytes,
+                               hipStream_t stream)
+{
+    if (nx < 3 || ny < 3 || nz < 3 || (int64_t)nx * ny * nz >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;
+    if (!density || !src || !dst || src == dst || !workspace || sweeps < 1) return (int)hipErrorInvalidValue;
+    if (has_source && (!source || !production)) return (int)hipErrorInvalidValue;
+    if (cfg < -1 || cfg >= kFusedConfigCount) return (int)hipErrorInvalidValue;
+    const SweepsLayout L = sweeps_layout(nx, ny, nz);
+    if (workspace_bytes < L.total) return (int)hipErrorInvalidValue;
+    if (cfg < 0) cfg = g_fused_config;
+    SorParams p{omega, dx2_over_D, uptake_max, michaelis_k, 1.0f, 0, 0, 0, fixed, density, production,
+                has_source, source, source_k, nx, ny, nz};
+    char* ws = (char*)workspace;
+    if (const int rows = packed_rows_of(cfg)) {
+        if (!packed_supported(p)) return (int)hipErrorInvalidValue;
+        const size_t bytes = (L.total - L.packed) / 4;
+        PackedField pf{{(float*)(ws + L.packed), (float*)(ws + L.packed + bytes)},
+                       {(float*)(ws + L.packed + 2 * bytes), (float*)(ws + L.packed + 3 * bytes)}};
+        WBC_TRY(launch_pack(p, src, density, pf, stream));
+        for (int32_t s = 0; s < sweeps; ++s) WBC_TRY(launch_sor_packed(p, pf, rows, nullptr, stream));
+        return (int)launch_unpack(p, pf, dst, stream);
+    }
+    float* tmp = (float*)(ws + L.tmp);
+    const float* cur = src;
+    for (int32_t s = 0; s < sweeps; ++s) {
+        float* out = ((sweeps - 1 - s) % 2 == 0) ? dst : tmp;  // the last sweep lands in dst
+        WBC_TRY(launch_sor_fused_cfg(cfg, p, cur, out, nullptr, stream));
+        cur = out;
+    }
+    return 0;
 }
