@@ -136,8 +136,10 @@ int wbc_steady_state_residual(float dx2_over_D, float uptake_max, float michaeli
                               wbc_stream_t stream);
 
 /* Replaces solve_steady_state, fields/diffusion.py:79-123, whole loop: chunks of check_every
- * sweeps, a residual after each, stop below `tolerance` or at max_sweeps. host_residual is
- * PINNED host memory of at least ceil(max_sweeps / check_every) floats. Outputs mirror
+ * sweeps, a residual after each, stop below `tolerance` or at max_sweeps. host_residual is host
+ * memory of at least ceil(max_sweeps / check_every) floats that receives each checked chunk's
+ * residual (the kernels write them into library-owned mapped host memory, read after each
+ * chunk's event). Outputs mirror
  * SolveReport (diffusion.py:64-68). All-Dirichlet grids run fused sweeps ping-ponging through
  * a second field buffer in the workspace; the result always ends in `field`. The host reads
  * chunk c's residual while chunk c+1 runs; a device-side latch turns sweeps queued after

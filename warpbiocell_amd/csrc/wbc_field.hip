@@ -277,6 +277,36 @@ struct EventPool {
 };
 EventPool g_pools[16];
 
+// Mapped, coherent host memory the residual kernels write each chunk's residual into, so the host
+// reads it after the chunk's event without a device-to-host copy (per device, grown on demand).
+struct HostSlots {
+    float* host = nullptr;
+    float* dev = nullptr;
+    int32_t cap = 0;
+};
+HostSlots g_host_slots[16];
+
+hipError_t get_host_slots(int32_t n, float** host, float** dev)
+{
+    int d = 0;
+    WBC_TRY_ERR(hipGetDevice(&d));
+    if (d < 0 || d >= 16) return hipErrorInvalidDevice;
+    HostSlots& h = g_host_slots[d];
+    if (h.cap < n) {
+        if (h.host) WBC_TRY_ERR(hipHostFree(h.host));
+        h.host = h.dev = nullptr;
+        h.cap = 0;
+        const int32_t cap = n < 1024 ? 1024 : n;
+        WBC_TRY_ERR(hipHostMalloc((void**)&h.host, (size_t)cap * sizeof(float),
+                                  hipHostMallocMapped | hipHostMallocCoherent));
+        WBC_TRY_ERR(hipHostGetDevicePointer((void**)&h.dev, h.host, 0));
+        h.cap = cap;
+    }
+    *host = h.host;
+    *dev = h.dev;
+    return hipSuccess;
+}
+
 hipError_t get_events(hipEvent_t** out)
 {
     int dev = 0;
@@ -366,6 +396,8 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
                    {(float*)(ws + L.packed + 2 * L.packed_each), (float*)(ws + L.packed + 3 * L.packed_each)}};
     hipEvent_t* ev;
     WBC_TRY(get_events(&ev));
+    float *slots_host, *slots_mapped;
+    WBC_TRY(get_host_slots(chunks, &slots_host, &slots_mapped));
     zero_i32_kernel<<<1, 256, 0, stream>>>(state, 4);
     WBC_CHECK_LAUNCH();
     if (packed) WBC_TRY(launch_pack(p, field, density, pf, stream));
@@ -395,15 +427,14 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
                 WBC_TRY(launch_residual_packed(p, pf, partial, state, stream));
             else
                 WBC_TRY(launch_residual_partial(p, buf[cur], partial, state, stream));
-            WBC_TRY(launch_residual_finish(partial, partials, slots, queued, tolerance, state, stream));
-            WBC_TRY(hipMemcpyAsync(host_residual + queued, slots + queued, sizeof(float), hipMemcpyDeviceToHost,
-                                   stream));
+            WBC_TRY(launch_residual_finish(partial, partials, slots, slots_mapped, queued, tolerance, state, stream));
             WBC_TRY(hipEventRecord(ev[queued % kEventPool], stream));
             ++queued;
         }
         if (checked == queued) break;
         WBC_TRY(hipEventSynchronize(ev[checked % kEventPool]));
-        residual = host_residual[checked];
+        residual = ((volatile float*)slots_host)[checked];
+        host_residual[checked] = residual;
         sweeps += chunk_len(checked);
         ++checked;
         if (residual < tolerance) {

@@ -52,24 +52,34 @@ hipError_t launch_sor_fused(const SorParams& p, const float* src, float* dst, co
 int residual_partials(const SorParams& p);
 hipError_t launch_residual_partial(const SorParams& p, const float* field, float* partial, const int32_t* done,
                                    hipStream_t s);
-hipError_t launch_residual_finish(const float* partial, int count, float* residual, int32_t chunk, float tol,
-                                  int32_t* state, hipStream_t s);
+// Reduce the residual partials into residual[chunk] (and host_out[chunk], mapped host memory, when
+// not null) and latch convergence in state[0..1].
+hipError_t launch_residual_finish(const float* partial, int count, float* residual, float* host_out, int32_t chunk,
+                                  float tol, int32_t* state, hipStream_t s);
 hipError_t launch_residual_atomic(const SorParams& p, const float* field, float* out, hipStream_t s);
 hipError_t launch_copy_f32(const float* a, float* b, int64_t n, hipStream_t s);
 
 // Colour-packed layout (wbc_sor_packed.hip): per colour, one field and one density array of
-// packed_floats(p) floats each. Dirichlet faces, no pinned nodes, no production only.
+// packed_floats(p) floats each: nx ny ceil(nz / 2) nodes and kPackedPadRows rows of padding, which
+// the colour passes read (and never use) for rows past the last interior row. Dirichlet faces, no
+// pinned nodes, no production only.
 struct PackedField {
     float* f[2];
     float* d[2];
 };
+constexpr int kPackedPadRows = 8;
+inline int64_t packed_floats_of(int32_t nx, int32_t ny, int32_t nz)
+{
+    return ((int64_t)nx * ny + kPackedPadRows) * ((nz + 1) / 2);
+}
 bool packed_supported(const SorParams& p);
-int64_t packed_floats(const SorParams& p);
+inline int64_t packed_floats(const SorParams& p) { return packed_floats_of(p.nx, p.ny, p.nz); }
 int residual_packed_partials(const SorParams& p);
 hipError_t launch_pack(const SorParams& p, const float* field, const float* density, PackedField& pf, hipStream_t s);
 hipError_t launch_unpack(const SorParams& p, const PackedField& pf, float* field, hipStream_t s);
 // One sweep (both colour passes); each thread takes `rows` (2, 4 or 8) consecutive rows j.
 hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, const int32_t* done, hipStream_t s);
+// Residual partials of both colours (residual_packed_partials(p) of them, for launch_residual_finish).
 hipError_t launch_residual_packed(const SorParams& p, const PackedField& pf, float* partial, const int32_t* done,
                                   hipStream_t s);
 

@@ -157,7 +157,8 @@ __global__ void __launch_bounds__(256) residual_partial_kernel(SorParams p, cons
 
 // Reduce the partials into residual[chunk] and latch convergence (one block).
 __global__ void __launch_bounds__(256) residual_finish_kernel(const float* __restrict__ partial, int count,
-                                                              float* __restrict__ residual, int32_t chunk,
+                                                              float* __restrict__ residual,
+                                                              float* __restrict__ host_out, int32_t chunk,
                                                               float tolerance, int32_t* __restrict__ state)
 {
     __shared__ float wave_max_lds[4];
@@ -170,6 +171,7 @@ __global__ void __launch_bounds__(256) residual_finish_kernel(const float* __res
     if (threadIdx.x == 0) {
         m = fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
         residual[chunk] = m;
+        if (host_out) host_out[chunk] = m;
         if (m < tolerance) {
             state[0] = 1;
             state[1] = chunk;
@@ -314,10 +316,10 @@ hipError_t launch_residual_atomic(const SorParams& p, const float* field, float*
     return hipGetLastError();
 }
 
-hipError_t launch_residual_finish(const float* partial, int count, float* residual, int32_t chunk, float tol,
-                                  int32_t* state, hipStream_t s)
+hipError_t launch_residual_finish(const float* partial, int count, float* residual, float* host_out, int32_t chunk,
+                                  float tol, int32_t* state, hipStream_t s)
 {
-    residual_finish_kernel<<<1, 256, 0, s>>>(partial, count, residual, chunk, tol, state);
+    residual_finish_kernel<<<1, 256, 0, s>>>(partial, count, residual, host_out, chunk, tol, state);
     return hipGetLastError();
 }
 
@@ -393,7 +395,7 @@ SweepsLayout sweeps_layout(int32_t nx, int32_t ny, int32_t nz)
 {
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
     SweepsLayout L;
-    const size_t nodes = (size_t)nx * ny * nz, half = (size_t)nx * ny * ((nz + 1) / 2);
+    const size_t nodes = (size_t)nx * ny * nz, half = (size_t)packed_floats_of(nx, ny, nz);
     L.tmp = 0;
     L.packed = up(nodes * sizeof(float));
     L.total = L.packed + 4 * up(half * sizeof(float));

@@ -20,8 +20,6 @@
 
 namespace wbc {
 
-constexpr int kPackedRows = 4;  // grid rows (j) per 256-thread workgroup, 64 k-slots each
-
 __device__ __forceinline__ int packed_nz2(int nz) { return (nz + 1) >> 1; }
 
 __global__ void __launch_bounds__(256) pack_kernel(const float* __restrict__ field, const float* __restrict__ density,
@@ -60,15 +58,137 @@ __global__ void __launch_bounds__(256) unpack_kernel(const float* __restrict__ f
     }
 }
 
-// One colour pass over the interior nodes of colour C, in place in `own`. Each thread takes RJ
-// consecutive rows j of one plane at one packed index: the other colour's rows j0-1 .. j0+RJ at
-// that index serve as j-neighbours of all RJ rows and as the shared k-neighbour of each, so a
-// thread issues 6 RJ + 2 loads for RJ nodes, all before it consumes any.
-//
-// Lanes run over a plane flattened as (row group, packed index): lane u of plane i takes row
-// group g = u / nz2 (rows 1 + g RJ ..) at kk = u - g nz2, so every lane of a wave has a node
-// whatever nz2 is; a wave's addresses are at most two unit-stride runs. The division is a
-// multiply-high by nz2_magic = floor(2^32 / nz2) + 1, exact for u nz2 < 2^32 (checked on the host).
+// The values one thread needs for RJ rows j0 .. j0+RJ-1 of plane i at packed index kk of one
+// colour. Row j0 has parity P ((colour + i + j0) & 1, the same for every lane of a workgroup since
+// RJ is even), so row j0 + r has parity P ^ (r & 1) and its k-neighbours of the other colour sit at
+// kk - 1 + par and kk + par. The other colour's rows j0-1 .. j0+RJ at kk serve as j-neighbours of
+// all RJ rows and as the shared k-neighbour of each: 6 RJ + 2 loads for RJ nodes, all issued
+// before any is consumed. Addresses are a workgroup-uniform plane base plus a 32-bit lane byte
+// offset (a plane is < 2^31 bytes). Rows past the last interior row read the next plane or the
+// arrays' padding rows and are not used.
+template <int RJ>
+struct PackedRows {
+    float oj[RJ + 2];  // other colour, plane i, rows j0-1 .. j0+RJ, at kk
+    float om[RJ], op[RJ];  // other colour, planes i-1 / i+1, rows j0 .., at kk
+    float ok[RJ];      // other colour, plane i, the k-neighbour that is not at kk
+    float o[RJ], d[RJ];  // own colour: field and density
+    uint32_t kk, j0, w1;  // w1: byte offset of row j0 at kk within a plane
+};
+
+template <int RJ, int P>
+__device__ __forceinline__ void load_packed_rows(PackedRows<RJ>& R, const float* __restrict__ own,
+                                                 const float* __restrict__ other, const float* __restrict__ dens,
+                                                 int i, uint32_t u, uint32_t nz2_magic, uint32_t nz2, size_t rs)
+{
+    const uint32_t g = __umulhi(u, nz2_magic);
+    R.kk = u - g * nz2;
+    R.j0 = g * RJ + 1;  // <= ny - 2 since g < ceil((ny - 2) / RJ)
+    const uint32_t row = nz2 * 4u;
+    const uint32_t w0 = ((R.j0 - 1) * nz2 + R.kk) * 4u;
+    R.w1 = w0 + row;
+    const char* o_m = (const char*)(other + (i - 1) * rs);
+    const char* o_0 = (const char*)(other + i * rs);
+    const char* o_p = (const char*)(other + (i + 1) * rs);
+    const char* o_k = (const char*)(other + i * rs + (P ? 1 : -1));   // parity P: kk + 1 or kk - 1
+    const char* o_k1 = (const char*)(other + i * rs + (P ? -1 : 1));  // parity 1 - P
+    const char* w_0 = (const char*)(own + i * rs);
+    const char* d_0 = (const char*)(dens + i * rs);
+    auto at = [](const char* base, uint32_t off) { return *(const float*)(base + off); };
+#pragma unroll
+    for (int m = 0; m < RJ + 2; ++m) R.oj[m] = at(o_0, w0 + m * row);
+#pragma unroll
+    for (int r = 0; r < RJ; ++r) {
+        const uint32_t w = R.w1 + r * row;
+        R.om[r] = at(o_m, w);
+        R.op[r] = at(o_p, w);
+        R.ok[r] = at((r & 1) ? o_k1 : o_k, w);
+        R.o[r] = at(w_0, w);
+        R.d[r] = at(d_0, w);
+    }
+}
+
+// Whether row j0 + r holds an interior node of its colour, and that node's neighbour sum in the
+// reference's order (i-1, i+1, j-1, j+1, k-1, k+1).
+template <int RJ, int P>
+__device__ __forceinline__ bool packed_node(const PackedRows<RJ>& R, int r, const SorParams& p, float& nb)
+{
+    const int par = P ^ (r & 1);
+    const int k = 2 * (int)R.kk + par;
+    if ((int)R.j0 + r > p.ny - 2 || k < 1 || k >= p.nz - 1) return false;
+    const float km = par ? R.oj[r + 1] : R.ok[r];  // other at kk - 1 + par
+    const float kp = par ? R.ok[r] : R.oj[r + 1];  // other at kk + par
+    nb = R.om[r] + R.op[r] + R.oj[r] + R.oj[r + 2] + km + kp;
+    return true;
+}
+
+// The colour pass on RJ rows (kernels/field_kernels.py:185-212, same expressions and order).
+template <int RJ, int P>
+__device__ __forceinline__ void packed_update(const SorParams& p, float* __restrict__ own,
+                                              const float* __restrict__ other, const float* __restrict__ dens,
+                                              int i, uint32_t u, uint32_t nz2_magic, int q_finite)
+{
+    const uint32_t nz2 = (uint32_t)packed_nz2(p.nz);
+    const size_t rs = (size_t)p.ny * nz2;
+    PackedRows<RJ> R;
+    load_packed_rows<RJ, P>(R, own, other, dens, i, u, nz2_magic, nz2, rs);
+    char* w_0 = (char*)(own + i * rs);
+#pragma unroll
+    for (int r = 0; r < RJ; ++r) {
+        float nb;
+        if (!packed_node<RJ, P>(R, r, p, nb)) continue;
+        const float kpo = p.michaelis_k + R.o[r];
+        float gs;
+        // density == 0 makes c == +-0 whenever K + o > 0 and uptake_max is finite: 6 + h*c == 6.
+        if (R.d[r] == 0.0f && kpo > 0.0f && q_finite) {
+            gs = div6(nb);
+        } else {
+            const float c = R.d[r] * p.uptake_max / kpo;
+            gs = nb / (6.0f + p.dx2_over_D * c);
+        }
+        const float v = (1.0f - p.omega) * R.o[r] + p.omega * gs;
+        *(float*)(w_0 + R.w1 + r * nz2 * 4u) = fmaxf(v, 0.0f);
+    }
+}
+
+// max over RJ rows of |r| / max(6 o, reference) (kernels/field_kernels.py:236-243, no production).
+template <int RJ, int P>
+__device__ __forceinline__ float packed_residual(const SorParams& p, const float* __restrict__ own,
+                                                 const float* __restrict__ other, const float* __restrict__ dens,
+                                                 int i, uint32_t u, uint32_t nz2_magic, int q_finite)
+{
+    const uint32_t nz2 = (uint32_t)packed_nz2(p.nz);
+    const size_t rs = (size_t)p.ny * nz2;
+    PackedRows<RJ> R;
+    load_packed_rows<RJ, P>(R, own, other, dens, i, u, nz2_magic, nz2, rs);
+    float m = 0.0f;
+#pragma unroll
+    for (int r = 0; r < RJ; ++r) {
+        float nb;
+        if (!packed_node<RJ, P>(R, r, p, nb)) continue;
+        const float o = R.o[r];
+        const float kpo = p.michaelis_k + o;
+        float c = 0.0f;
+        if (!(R.d[r] == 0.0f && kpo > 0.0f && q_finite)) c = R.d[r] * p.uptake_max / kpo;
+        const float prod = 0.0f;
+        const float res = nb + p.dx2_over_D * prod - (6.0f + p.dx2_over_D * c) * o;
+        m = fmaxf(m, fabsf(res) / fmaxf(6.0f * o, p.reference));
+    }
+    return m;
+}
+
+// Workgroup -> (plane i, lane u) over planes flattened as (row group, packed index): lane u of
+// plane i takes row group g = u / nz2 (rows 1 + g RJ ..) at kk = u - g nz2, so every lane of a
+// wave has a node whatever nz2 is, and a wave's addresses are at most two unit-stride runs. The
+// division is a multiply-high by nz2_magic = floor(2^32 / nz2) + 1, exact for u nz2 < 2^32
+// (checked on the host). Neighbouring planes stay on one XCD.
+__device__ __forceinline__ void packed_lane(int tile, int blocks_per_plane, int& i, int& u)
+{
+    i = tile / blocks_per_plane + 1;
+    u = (tile - (i - 1) * blocks_per_plane) * 256 + (int)threadIdx.x;
+}
+
+// One colour pass over the interior nodes of colour C, in place in `own`; each thread takes RJ
+// consecutive rows of one plane at one packed index.
 template <int C, int RJ>
 __global__ void __launch_bounds__(256) sor_packed_kernel(SorParams p, float* __restrict__ own,
                                                          const float* __restrict__ other,
@@ -76,101 +196,41 @@ __global__ void __launch_bounds__(256) sor_packed_kernel(SorParams p, float* __r
                                                          const int32_t* __restrict__ done, int blocks_per_plane,
                                                          int lanes_per_plane, uint32_t nz2_magic, int q_finite)
 {
+    static_assert(RJ % 2 == 0, "an even row count keeps the parity of a thread's first row uniform");
     if (done && *done) return;
-    const int tile = xcd_tile(blockIdx.x, gridDim.x);
-    const int i = tile / blocks_per_plane + 1;
-    const int u = (tile - (i - 1) * blocks_per_plane) * 256 + (int)threadIdx.x;
+    int i, u;
+    packed_lane(xcd_tile(blockIdx.x, gridDim.x), blocks_per_plane, i, u);
     if (u >= lanes_per_plane) return;
-    const int ny = p.ny, nz = p.nz, nz2 = packed_nz2(nz);
-    const int g = (int)__umulhi((uint32_t)u, nz2_magic);
-    const int kk = u - g * nz2;
-    const int j0 = g * RJ + 1;  // <= ny - 2 since g < ceil((ny - 2) / RJ)
-    const int rs = ny * nz2;    // stride of i in a packed array
-    const int e0 = (i * ny + j0) * nz2 + kk;
-    // Loads: rows past the last interior row read row ny-1 (in the grid; their nodes are not
-    // updated), and the k-neighbour index is clamped into the row.
-    float oj[RJ + 2], om[RJ], op[RJ], ok[RJ], o[RJ], d[RJ];
-    int par[RJ];
-#pragma unroll
-    for (int m = 0; m < RJ + 2; ++m) {
-        const int row = min(j0 - 1 + m, ny - 1);
-        oj[m] = other[e0 + (row - j0) * nz2];
-    }
-#pragma unroll
-    for (int r = 0; r < RJ; ++r) {
-        const int row = min(j0 + r, ny - 1);
-        const int e = e0 + (row - j0) * nz2;
-        par[r] = (C + i + row) & 1;
-        // the k-neighbour that is not `other` at kk: kk - 1 for par 0, kk + 1 for par 1
-        const int kx = par[r] ? min(kk + 1, nz2 - 1) : max(kk - 1, 0);
-        om[r] = other[e - rs];
-        op[r] = other[e + rs];
-        ok[r] = other[e - kk + kx];
-        o[r] = own[e];
-        d[r] = dens[e];
-    }
-#pragma unroll
-    for (int r = 0; r < RJ; ++r) {
-        const int j = j0 + r;
-        const int k = 2 * kk + par[r];
-        if (j >= ny - 1 || k < 1 || k >= nz - 1) continue;
-        const float km = par[r] ? oj[r + 1] : ok[r];  // other at kk - 1 + par
-        const float kp = par[r] ? ok[r] : oj[r + 1];  // other at kk + par
-        const float nb = om[r] + op[r] + oj[r] + oj[r + 2] + km + kp;
-        const float kpo = p.michaelis_k + o[r];
-        float gs;
-        // density == 0 makes c == +-0 whenever K + o > 0 and uptake_max is finite: 6 + h*c == 6.
-        if (d[r] == 0.0f && kpo > 0.0f && q_finite) {
-            gs = div6(nb);
-        } else {
-            const float c = d[r] * p.uptake_max / kpo;
-            gs = nb / (6.0f + p.dx2_over_D * c);
-        }
-        const float v = (1.0f - p.omega) * o[r] + p.omega * gs;
-        own[e0 + r * nz2] = fmaxf(v, 0.0f);
-    }
+    if ((C + i + 1) & 1)
+        packed_update<RJ, 1>(p, own, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
+    else
+        packed_update<RJ, 0>(p, own, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
 }
 
-// |r| / max(6 o, reference) (kernels/field_kernels.py:236-243) over the interior nodes of both
-// colours (blocks [0, tiles) colour 0, [tiles, 2 tiles) colour 1), one maximum per block.
-__global__ void __launch_bounds__(256) residual_packed_kernel(SorParams p, const float* __restrict__ f0,
-                                                              const float* __restrict__ f1,
-                                                              const float* __restrict__ d0,
-                                                              const float* __restrict__ d1,
-                                                              float* __restrict__ partial,
-                                                              const int32_t* __restrict__ done, int nkx, int njy,
-                                                              int q_finite)
+// The residual of both colours: workgroups [0, B) take colour 0, [B, 2B) colour 1 (B = blocks of
+// one colour pass with kResidualRows rows per thread); each stores its maximum in partial[].
+constexpr int kResidualRows = 4;
+
+__global__ void __launch_bounds__(256) residual_packed_kernel(SorParams p, PackedField pf, float* __restrict__ partial,
+                                                              const int32_t* __restrict__ done, int blocks_per_plane,
+                                                              int lanes_per_plane, uint32_t nz2_magic, int q_finite)
 {
     __shared__ float wave_max_lds[4];
     if (done && *done) return;
-    const int tiles = nkx * njy * (p.nx - 2);
-    const int colour = (int)blockIdx.x >= tiles ? 1 : 0;
-    const int tile = (int)blockIdx.x - colour * tiles;
-    const float* own = colour ? f1 : f0;
-    const float* other = colour ? f0 : f1;
-    const float* dens = colour ? d1 : d0;
-    const int ny = p.ny, nz = p.nz, nz2 = packed_nz2(nz);
-    const int kk = (tile % nkx) * 64 + (threadIdx.x & 63);
-    const int j = ((tile / nkx) % njy) * kPackedRows + (threadIdx.x >> 6) + 1;
-    const int i = tile / (nkx * njy) + 1;
+    const int B = blocks_per_plane * (p.nx - 2);
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int colour = t >= B ? 1 : 0;
+    int i, u;
+    packed_lane(t - colour * B, blocks_per_plane, i, u);
     float m = 0.0f;
-    if (j < ny - 1) {
-        const int par = (colour + i + j) & 1;
-        const int k = 2 * kk + par;
-        if (k >= 1 && k < nz - 1) {
-            const int rs = ny * nz2;
-            const int e = (i * ny + j) * nz2 + kk;
-            const float o = own[e];
-            const float d = dens[e];
-            const float kpo = p.michaelis_k + o;
-            float c = 0.0f;
-            if (!(d == 0.0f && kpo > 0.0f && q_finite)) c = d * p.uptake_max / kpo;
-            const float prod = 0.0f;
-            const float nb = other[e - rs] + other[e + rs] + other[e - nz2] + other[e + nz2] + other[e - 1 + par] +
-                             other[e + par];
-            const float r = nb + p.dx2_over_D * prod - (6.0f + p.dx2_over_D * c) * o;
-            m = fabsf(r) / fmaxf(6.0f * o, p.reference);
-        }
+    if (u < lanes_per_plane) {
+        const float* own = pf.f[colour];
+        const float* other = pf.f[colour ^ 1];
+        const float* dens = pf.d[colour];
+        if ((colour + i + 1) & 1)
+            m = packed_residual<kResidualRows, 1>(p, own, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
+        else
+            m = packed_residual<kResidualRows, 0>(p, own, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
     }
     m = wave_max(m);
     if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;
@@ -180,8 +240,6 @@ __global__ void __launch_bounds__(256) residual_packed_kernel(SorParams p, const
 }
 
 // ---- host side ------------------------------------------------------------------------------
-
-int64_t packed_floats(const SorParams& p) { return (int64_t)p.nx * p.ny * ((p.nz + 1) / 2); }
 
 // Lanes of one plane for RJ rows per thread, or -1 when the multiply-high division would not be
 // exact (u nz2 >= 2^32 for some lane u).
@@ -198,18 +256,15 @@ bool packed_supported(const SorParams& p)
            packed_lanes_per_plane(p, 2) >= 0 && packed_floats(p) < ((int64_t)1 << 31);
 }
 
-static void packed_grid(const SorParams& p, int& nkx, int& njy, int& tiles)
+static uint32_t nz2_magic_of(const SorParams& p)
 {
-    nkx = (((p.nz + 1) / 2) + 63) / 64;
-    njy = (p.ny - 2 + kPackedRows - 1) / kPackedRows;
-    tiles = nkx * njy * (p.nx - 2);
+    return (uint32_t)((((uint64_t)1 << 32) / (uint32_t)((p.nz + 1) / 2)) + 1);
 }
 
 int residual_packed_partials(const SorParams& p)
 {
-    int nkx, njy, tiles;
-    packed_grid(p, nkx, njy, tiles);
-    return 2 * tiles;
+    const int64_t lanes = packed_lanes_per_plane(p, kResidualRows);
+    return lanes < 0 ? 0 : 2 * (int)((lanes + 255) / 256) * (p.nx - 2);
 }
 
 hipError_t launch_pack(const SorParams& p, const float* field, const float* density, PackedField& pf, hipStream_t s)
@@ -233,8 +288,7 @@ static hipError_t launch_packed_rows(const SorParams& p, PackedField& pf, const 
 {
     const int64_t lanes = packed_lanes_per_plane(p, RJ);
     if (lanes < 0) return hipErrorInvalidValue;
-    const uint32_t nz2 = (uint32_t)((p.nz + 1) / 2);
-    const uint32_t magic = (uint32_t)((((uint64_t)1 << 32) / nz2) + 1);
+    const uint32_t magic = nz2_magic_of(p);
     const int bpp = (int)((lanes + 255) / 256);
     const int blocks = bpp * (p.nx - 2);
     const int qf = __builtin_isfinite(p.uptake_max) ? 1 : 0;
@@ -257,11 +311,12 @@ hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, cons
 hipError_t launch_residual_packed(const SorParams& p, const PackedField& pf, float* partial, const int32_t* done,
                                   hipStream_t s)
 {
-    int nkx, njy, tiles;
-    packed_grid(p, nkx, njy, tiles);
+    const int64_t lanes = packed_lanes_per_plane(p, kResidualRows);
+    if (lanes < 0) return hipErrorInvalidValue;
+    const int bpp = (int)((lanes + 255) / 256);
     const int qf = __builtin_isfinite(p.uptake_max) ? 1 : 0;
-    residual_packed_kernel<<<2 * tiles, 256, 0, s>>>(p, pf.f[0], pf.f[1], pf.d[0], pf.d[1], partial, done, nkx, njy,
-                                                     qf);
+    residual_packed_kernel<<<2 * bpp * (p.nx - 2), 256, 0, s>>>(p, pf, partial, done, bpp, (int)lanes,
+                                                                 nz2_magic_of(p), qf);
     return hipGetLastError();
 }
 
