@@ -92,7 +92,18 @@ def assert_same_state(serial, batch):
         assert np.array_equal(fa.view(np.uint32), fb.view(np.uint32)), f"{r.label}: field differs"
 
 
-def test_batched_steps_are_bit_exact_per_replica(device):
+def use_schedule(batch, name):
+    """Select the schedule called `name` ('default' keeps the constructor's)."""
+    if name == "default":
+        return
+    from warpbiocell_amd.autotune import batch_schedules
+
+    tile, bpc, _ = next(s for s in batch_schedules() if s[2] == name)
+    batch.set_schedule(tile, bpc)
+
+
+@pytest.mark.parametrize("schedule", ["default", "wide/2", "wide/4"])
+def test_batched_steps_are_bit_exact_per_replica(device, schedule):
     from warpbiocell_amd.simulation.batched import BatchedSweep
 
     steps = 3
@@ -100,6 +111,7 @@ def test_batched_steps_are_bit_exact_per_replica(device):
     batched = [build(s, device) for s in SPECS]
     ref_reports = run_serial(serial, steps)
     batch = BatchedSweep(batched, hash_dim=64)
+    use_schedule(batch, schedule)
     for k in range(steps):
         print(f"batched step {k}", flush=True)
         batch.step(1, use_graph=True)
@@ -119,11 +131,14 @@ def test_batched_steps_are_bit_exact_per_replica(device):
     batch.close()
 
 
-def test_graph_replays_match_eager_steps(device):
+@pytest.mark.parametrize("schedule", ["default", "wide/2"])
+def test_graph_replays_match_eager_steps(device, schedule):
     from warpbiocell_amd.simulation.batched import BatchedSweep
 
     eager = BatchedSweep([build(s, device) for s in SPECS[:3]], hash_dim=64)
     graph = BatchedSweep([build(s, device) for s in SPECS[:3]], hash_dim=64)
+    use_schedule(eager, schedule)
+    use_schedule(graph, schedule)
     for k in range(3):
         print(f"replay {k}", flush=True)
         eager.step(1, use_graph=False)

@@ -93,20 +93,30 @@ def tune_sor(field, density: torch.Tensor, kinetics, omega: float, sweeps: int =
 
 
 def batch_schedules() -> list[tuple[int, int, str]]:
-    """(tile, workgroups per CU, name) of every schedule of the persistent batched solve: each
-    tile shape at 1, 2, ... up to the occupancy limit workgroups per CU (0 = the limit)."""
-    out = []
-    for tile in range(_lib.query("wbc_batch_schedule_count")):
+    """(schedule, workgroups per CU, name) of every schedule of the batched solve: the persistent
+    launch with each tile shape at 1, 2, ... up to the occupancy limit workgroups per CU
+    ('copy' / 'pipe' 60 x TJ x TI @ n/CU), and the wide launches ('wide/RJ': one launch per colour
+    pass over every replica, RJ rows per thread)."""
+    count = _lib.query("wbc_batch_schedule_count")
+    shapes = []
+    for tile in range(count):
         tj, ti, most = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()
         _lib.call("wbc_batch_schedule_shape", tile, ctypes.byref(tj), ctypes.byref(ti), ctypes.byref(most))
-        kind = "pipe" if tile >= _lib.query("wbc_batch_schedule_count") // 2 else "copy"
-        for bpc in range(1, most.value + 1):
-            out.append((tile, bpc, f"{kind}60x{tj.value}x{ti.value}@{bpc}/CU"))
+        shapes.append((tile, tj.value, ti.value, most.value))
+    persistent = sum(1 for s in shapes if s[2] != 0)
+    out = []
+    for tile, tj, ti, most in shapes:
+        if ti == 0:
+            out.append((tile, 0, f"wide/{tj}"))
+            continue
+        kind = "pipe" if tile >= persistent // 2 else "copy"
+        for bpc in range(1, most + 1):
+            out.append((tile, bpc, f"{kind}60x{tj}x{ti}@{bpc}/CU"))
     return out
 
 
 def tune_batch(batch, repeats: int = 1) -> dict:
-    """Time the persistent solve of every replica of ``batch`` under every schedule, from the same
+    """Time the batched solve of every replica of ``batch`` under every schedule, from the same
     start fields (restored after each run; the solve's results do not depend on the schedule),
     and select the fastest. Cached per (device, replica grids and cell counts)."""
     key = ("batch", str(batch.device), tuple((r.oxygen.geometry.shape, r.population.count) for r in batch.replicas))

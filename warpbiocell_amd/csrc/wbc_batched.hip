@@ -22,8 +22,20 @@
 // generation (release/acquire fences at agent scope, relaxed polling). Every wait is bounded and a
 // timeout raises an error flag that makes every workgroup leave the kernel. The grid never
 // exceeds the number of workgroups that fit on the device at once.
+//
+// Wide solve (the other schedule family). Every replica's field and density are packed by colour
+// (the layout of wbc_sor_packed.hip) and each colour pass of a sweep is ONE launch over the planes
+// of every replica still active: workgroup b belongs to the replica whose block range holds it
+// (block offsets per replica), and the workgroups of a replica that has finished leave at once.
+// After the sweeps that end a check of some replica, one launch computes the residual partials of
+// the replicas checking there and one workgroup per replica reduces them, records the report and
+// latches the replica's done flag, which stops its later sweeps on the device; it also writes the
+// flag into mapped host memory, from which the host learns, a few chunks behind the device, when
+// every replica has finished. A global sync is a kernel boundary (cheaper on this device than a
+// grid barrier), and the launch count per sweep is that of one grid whatever the replica count.
 #include <hip/hip_runtime.h>
 
+#include <algorithm>
 #include <climits>
 #include <new>
 #include <vector>
@@ -31,6 +43,7 @@
 #include "wbc.h"
 #include "wbc_cellops.hpp"
 #include "wbc_scan.hpp"
+#include "wbc_sor_packed.hpp"
 #include "wbc_sor_tile.hpp"
 
 namespace wbc {
@@ -574,12 +587,181 @@ __global__ void __launch_bounds__(256) batch_contact_kernel(const ReplicaDev* __
     position[3 * i + 2] = P.z + vz * d.dt_mech;
 }
 
+// ---- wide solve ------------------------------------------------------------------------------
+struct WideRep {
+    SorParams p;          // the replica's solve (density unused: the packed arrays below)
+    PackedField pf;       // the replica's colour-packed field and density
+    int32_t bpp, lanes;   // colour pass: workgroups and lanes per plane at the schedule's rows per thread
+    int32_t rbpp, rlanes; // residual: the same at kResidualRows rows per thread
+    uint32_t magic;       // nz2_magic_of(p)
+    int32_t q_finite;
+    int32_t max_sweeps, check_every;
+    float tolerance;
+    int32_t pad;
+    int64_t node_offset;  // first node of the replica in the concatenated unpacked arrays
+};
+
+__device__ __forceinline__ bool wide_is_check(const WideRep& w, int32_t q)
+{
+    return q >= 1 && q <= w.max_sweeps && (q % w.check_every == 0 || q == w.max_sweeps);
+}
+
+// The replica whose range [off[r], off[r+1]) holds b (off[0] = 0, off[R] = total).
+template <typename T>
+__device__ __forceinline__ int range_of(const T* __restrict__ off, int R, T b)
+{
+    int lo = 0, hi = R - 1;
+    while (lo < hi) {
+        const int mid = (lo + hi + 1) >> 1;
+        if (off[mid] <= b) lo = mid;
+        else hi = mid - 1;
+    }
+    return lo;
+}
+
+template <int C, int RJ>
+__global__ void __launch_bounds__(256) wide_pass_kernel(const WideRep* __restrict__ wr,
+                                                         const int32_t* __restrict__ blk_off, int R,
+                                                         const int32_t* __restrict__ done)
+{
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int r = range_of(blk_off, R, t);
+    if (done[r]) return;
+    const WideRep& w = wr[r];
+    int i, u;
+    packed_lane(t - blk_off[r], w.bpp, i, u);
+    if (u >= w.lanes) return;
+    float* own = w.pf.f[C];
+    const float* other = w.pf.f[C ^ 1];
+    const float* dens = w.pf.d[C];
+    if ((C + i + 1) & 1)
+        packed_update<RJ, 1>(w.p, own, other, dens, i, (uint32_t)u, w.magic, w.q_finite);
+    else
+        packed_update<RJ, 0>(w.p, own, other, dens, i, (uint32_t)u, w.magic, w.q_finite);
+}
+
+// Residual partials (both colours) of every replica that is active and checks after sweep q: the
+// replica's workgroups [rblk_off[r], rblk_off[r+1]) write one maximum each.
+__global__ void __launch_bounds__(256) wide_residual_kernel(const WideRep* __restrict__ wr,
+                                                             const int32_t* __restrict__ rblk_off, int R,
+                                                             const int32_t* __restrict__ done, int32_t q,
+                                                             float* __restrict__ partial)
+{
+    __shared__ float wave_max_lds[4];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int r = range_of(rblk_off, R, t);
+    const WideRep& w = wr[r];
+    if (done[r] || !wide_is_check(w, q)) return;
+    const int local = t - rblk_off[r];
+    const int B = w.rbpp * (w.p.nx - 2);
+    const int colour = local >= B ? 1 : 0;
+    int i, u;
+    packed_lane(local - colour * B, w.rbpp, i, u);
+    float m = 0.0f;
+    if (u < w.rlanes) {
+        const float* own = w.pf.f[colour];
+        const float* other = w.pf.f[colour ^ 1];
+        const float* dens = w.pf.d[colour];
+        if ((colour + i + 1) & 1)
+            m = packed_residual<kResidualRows, 1>(w.p, own, other, dens, i, (uint32_t)u, w.magic, w.q_finite);
+        else
+            m = packed_residual<kResidualRows, 0>(w.p, own, other, dens, i, (uint32_t)u, w.magic, w.q_finite);
+    }
+    m = wave_max(m);
+    if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partial[t] = fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
+}
+
+// One workgroup per replica: for a replica checking after sweep q, reduce its partials, record
+// sweeps = q and the residual, and when the residual is below the tolerance or q = max_sweeps
+// latch done (later sweeps of the replica do nothing), record convergence and add q to its
+// running sweep total. Every replica's done flag is then copied to host_active[r] (mapped host
+// memory): 1 while the replica still sweeps.
+__global__ void __launch_bounds__(256) wide_finish_kernel(const WideRep* __restrict__ wr,
+                                                           const int32_t* __restrict__ rblk_off,
+                                                           int32_t* __restrict__ done, int32_t q,
+                                                           const float* __restrict__ partial,
+                                                           SolveState* __restrict__ st,
+                                                           long long* __restrict__ totals,
+                                                           int32_t* __restrict__ host_active)
+{
+    __shared__ float wave_max_lds[4];
+    const int r = blockIdx.x;
+    const WideRep& w = wr[r];
+    const bool check = done[r] == 0 && wide_is_check(w, q);
+    if (check) {
+        float m = 0.0f;
+        for (int b = rblk_off[r] + (int)threadIdx.x; b < rblk_off[r + 1]; b += 256) m = fmaxf(m, partial[b]);
+        m = wave_max(m);
+        if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;
+        __syncthreads();
+        if (threadIdx.x == 0) {
+            m = fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
+            const bool conv = m < w.tolerance;
+            st[r].sweeps = q;
+            st[r].residual = m;
+            if (conv || q == w.max_sweeps) {
+                st[r].converged = conv ? 1 : 0;
+                st[r].finished_phase = q;
+                totals[r] += q;
+                done[r] = 1;
+            }
+        }
+    }
+    if (threadIdx.x == 0 && host_active) host_active[r] = done[r] ? 0 : 1;
+}
+
+__global__ void __launch_bounds__(256) wide_reset_kernel(int32_t* __restrict__ done, SolveState* __restrict__ st,
+                                                          int R)
+{
+    const int r = threadIdx.x;
+    if (r < R) {
+        done[r] = 0;
+        st[r].finished_phase = INT_MAX;
+        st[r].sweeps = 0;
+        st[r].residual = __builtin_huge_valf();
+        st[r].converged = 0;
+    }
+}
+
+// Concatenated unpacked field and density -> every replica's colour-packed arrays (kUnpack:
+// packed field -> unpacked field).
+template <bool kUnpack>
+__global__ void __launch_bounds__(256) wide_pack_kernel(const WideRep* __restrict__ wr,
+                                                         const int64_t* __restrict__ node_off, int R, int64_t nodes,
+                                                         float* __restrict__ field, const float* __restrict__ density)
+{
+    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nodes; t += (int64_t)gridDim.x * 256) {
+        const int r = range_of(node_off, R, t);
+        const WideRep& w = wr[r];
+        const int64_t local = t - node_off[r];
+        const int nz = w.p.nz, ny = w.p.ny;
+        const int k = (int)(local % nz);
+        const int64_t ij = local / nz;
+        const int j = (int)(ij % ny), i = (int)(ij / ny);
+        const int64_t e = ij * packed_nz2(nz) + (k >> 1);
+        const int colour = (i + j + k) & 1;
+        if (kUnpack) {
+            field[t] = w.pf.f[colour][e];
+        } else {
+            w.pf.f[colour][e] = field[t];
+            w.pf.d[colour][e] = density[t];
+        }
+    }
+}
+
 // ---- schedules ------------------------------------------------------------------------------
 struct BatchTile {
     int tj, ti;
 };
 constexpr BatchTile kBatchTiles[] = {{4, 4}, {4, 8}, {8, 4}, {8, 8}};
 constexpr int kBatchTileCount = sizeof(kBatchTiles) / sizeof(kBatchTiles[0]);
+// Wide schedules (index 2 kBatchTileCount + w): rows per thread of the colour-pass launches.
+constexpr int kWideRows[] = {2, 4};
+constexpr int kWideCount = sizeof(kWideRows) / sizeof(kWideRows[0]);
+constexpr int kScheduleCount = 2 * kBatchTileCount + kWideCount;
 
 // Schedule table: tile index t < kBatchTileCount runs one tile at a time per workgroup,
 // t + kBatchTileCount the pipelined loop.
@@ -611,10 +793,27 @@ static int tiles_for(int tile, int nx, int ny, int nz)
 // ---- workspace layout -----------------------------------------------------------------------
 struct BatchLayout {
     size_t rep, state, bar, node_off, total_sweeps, partial, acc, density, buf1, buf2, rep_of_slot, cnt, start,
-        scan, key, tmp, id, xyzr, total;
+        scan, key, tmp, id, xyzr, wrep, blk_off, rblk_off, wdone, wpartial, packed, total;
     int64_t slots, nodes, buckets;
-    int32_t partial_stride, active_total;
+    int32_t partial_stride, active_total, wide_res_blocks;
+    bool wide_ok;  // every replica's grid fits the colour-packed passes
 };
+
+// Residual workgroups of one replica in the wide solve (both colours).
+static int64_t wide_res_blocks_of(const SorParams& p)
+{
+    const int64_t lanes = packed_lanes_per_plane(p, kResidualRows);
+    return lanes < 0 ? -1 : 2 * ((lanes + 255) / 256) * (p.nx - 2);
+}
+
+static SorParams grid_params(const wbc_replica& d)
+{
+    SorParams p{};
+    p.nx = d.nx;
+    p.ny = d.ny;
+    p.nz = d.nz;
+    return p;
+}
 
 static bool layout_for(const wbc_replica* r, int32_t R, int32_t dim, BatchLayout& L)
 {
@@ -642,6 +841,18 @@ static bool layout_for(const wbc_replica* r, int32_t R, int32_t dim, BatchLayout
         if (s > max_tiles) max_tiles = s;
     }
     tiles = max_tiles;
+    int64_t res_blocks = 0, packed_bytes = 0;
+    bool wide_ok = true;
+    for (int i = 0; i < R; ++i) {
+        const SorParams g = grid_params(r[i]);
+        const int64_t rb = wide_res_blocks_of(g);
+        if (rb < 0 || packed_lanes_per_plane(g, kWideRows[0]) < 0 || packed_floats(g) >= ((int64_t)1 << 31))
+            wide_ok = false;
+        res_blocks += rb < 0 ? 0 : rb;
+        packed_bytes += 4 * (int64_t)((packed_floats(g) * sizeof(float) + 255) / 256 * 256);
+    }
+    if (res_blocks >= ((int64_t)1 << 30)) wide_ok = false;
+    if (!wide_ok) packed_bytes = res_blocks = 0;
     const int64_t buckets = (int64_t)R * dim * dim * dim;
     if (slots >= ((int64_t)1 << 31) || buckets >= ((int64_t)1 << 31) - 1 || tiles >= ((int64_t)1 << 30)) return false;
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
@@ -664,7 +875,15 @@ static bool layout_for(const wbc_replica* r, int32_t R, int32_t dim, BatchLayout
     L.tmp = off;          off += up(sizeof(int32_t) * slots);
     L.id = off;           off += up(sizeof(int32_t) * slots);
     L.xyzr = off;         off += up(sizeof(float4) * slots);
+    L.wrep = off;         off += up(sizeof(WideRep) * R);
+    L.blk_off = off;      off += up(sizeof(int32_t) * (R + 1));
+    L.rblk_off = off;     off += up(sizeof(int32_t) * (R + 1));
+    L.wdone = off;        off += up(sizeof(int32_t) * R);
+    L.wpartial = off;     off += up(sizeof(float) * res_blocks);
+    L.packed = off;       off += up((size_t)packed_bytes);
     L.total = off;
+    L.wide_res_blocks = (int32_t)res_blocks;
+    L.wide_ok = wide_ok;
     L.slots = slots;
     L.nodes = nodes;
     L.buckets = buckets;
@@ -686,20 +905,66 @@ struct wbc_batch {
     char* ws = nullptr;
     int device = 0;
     hipStream_t capture_stream = nullptr;
-    hipGraph_t graph = nullptr;
+    hipGraph_t graph = nullptr;  // the whole step (persistent schedules)
     hipGraphExec_t exec = nullptr;
+    hipGraph_t graph_pre = nullptr, graph_post = nullptr;  // around the host-driven wide solve
+    hipGraphExec_t exec_pre = nullptr, exec_post = nullptr;
+    // wide solve
+    std::vector<WideRep> wide;         // per replica, for the current rows per thread
+    std::vector<int32_t> checkpoints;  // sweep counts after which some replica checks
+    int32_t wide_rows = 0, wide_blocks = 0;
+    int32_t* flags_host = nullptr;     // mapped: checkpoints.size() x R activity flags
+    int32_t* flags_dev = nullptr;
+    hipEvent_t ev[8] = {};
 };
+
+constexpr int kWideLag = 2;  // chunks queued ahead of the host's check (below the 8 events)
+
+static bool is_wide(const wbc_batch* b) { return b->tile >= 2 * kBatchTileCount; }
 
 static void drop_graph(wbc_batch* b)
 {
     if (b->exec) (void)hipGraphExecDestroy(b->exec);
     if (b->graph) (void)hipGraphDestroy(b->graph);
-    b->exec = nullptr;
-    b->graph = nullptr;
+    if (b->exec_pre) (void)hipGraphExecDestroy(b->exec_pre);
+    if (b->graph_pre) (void)hipGraphDestroy(b->graph_pre);
+    if (b->exec_post) (void)hipGraphExecDestroy(b->exec_post);
+    if (b->graph_post) (void)hipGraphDestroy(b->graph_post);
+    b->exec = b->exec_pre = b->exec_post = nullptr;
+    b->graph = b->graph_pre = b->graph_post = nullptr;
+}
+
+// The wide descriptors and block offsets for `rows` rows per thread, uploaded to the workspace.
+static hipError_t wide_configure(wbc_batch* b, int rows)
+{
+    const int R = b->R;
+    std::vector<int32_t> blk(R + 1), rblk(R + 1);
+    int64_t acc = 0, racc = 0;
+    for (int r = 0; r < R; ++r) {
+        WideRep& w = b->wide[r];
+        const int64_t lanes = packed_lanes_per_plane(w.p, rows);
+        if (lanes < 0) return hipErrorInvalidValue;
+        w.lanes = (int32_t)lanes;
+        w.bpp = (int32_t)((lanes + 255) / 256);
+        blk[r] = (int32_t)acc;
+        rblk[r] = (int32_t)racc;
+        acc += (int64_t)w.bpp * (w.p.nx - 2);
+        racc += wide_res_blocks_of(w.p);
+    }
+    if (acc >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
+    blk[R] = (int32_t)acc;
+    rblk[R] = (int32_t)racc;
+    WBC_TRY_ERR(hipMemcpy(b->ws + b->L.wrep, b->wide.data(), sizeof(WideRep) * R, hipMemcpyHostToDevice));
+    WBC_TRY_ERR(hipMemcpy(b->ws + b->L.blk_off, blk.data(), sizeof(int32_t) * (R + 1), hipMemcpyHostToDevice));
+    WBC_TRY_ERR(hipMemcpy(b->ws + b->L.rblk_off, rblk.data(), sizeof(int32_t) * (R + 1), hipMemcpyHostToDevice));
+    b->wide_rows = rows;
+    b->wide_blocks = (int32_t)acc;
+    return hipSuccess;
 }
 
 static hipError_t batch_grid(wbc_batch* b)
 {
+    if (is_wide(b)) return hipSuccess;  // the wide launches size their own grids
     int cus = 0, per_cu = 0;
     WBC_TRY_ERR(hipDeviceGetAttribute(&cus, hipDeviceAttributeMultiprocessorCount, b->device));
     WBC_TRY_ERR(hipOccupancyMaxActiveBlocksPerMultiprocessor(&per_cu, batch_kernel_fn(b->tile), 256, 0));
@@ -740,7 +1005,70 @@ static hipError_t enqueue_solve(wbc_batch* b, hipStream_t s)
     return hipGetLastError();
 }
 
-static hipError_t enqueue_step(wbc_batch* b, hipStream_t s)
+// The wide solve of every replica from its current field: pack, chunks of sweeps up to each
+// checkpoint followed by the residual and the per-replica finish, queued up to kWideLag chunks
+// ahead of the host, which stops queueing once a finished chunk reports every replica done;
+// chunks queued after that do nothing. Then unpack into the replicas' fields. Synchronises.
+static hipError_t wide_solve(wbc_batch* b, hipStream_t s)
+{
+    const BatchLayout& L = b->L;
+    char* ws = b->ws;
+    const int R = b->R;
+    const WideRep* wr = (const WideRep*)(ws + L.wrep);
+    const int32_t* blk_off = (const int32_t*)(ws + L.blk_off);
+    const int32_t* rblk_off = (const int32_t*)(ws + L.rblk_off);
+    int32_t* done = (int32_t*)(ws + L.wdone);
+    float* partial = (float*)(ws + L.wpartial);
+    SolveState* st = (SolveState*)(ws + L.state);
+    long long* totals = (long long*)(ws + L.total_sweeps);
+    const int64_t* node_off = (const int64_t*)(ws + L.node_off);
+    const float* density = (const float*)(ws + L.density);
+    wide_reset_kernel<<<1, 64, 0, s>>>(done, st, R);
+    WBC_TRY_ERR(hipGetLastError());
+    if (L.nodes > 0) {
+        wide_pack_kernel<false><<<2048, 256, 0, s>>>(wr, node_off, R, L.nodes, b->field, density);
+        WBC_TRY_ERR(hipGetLastError());
+    }
+    const int32_t chunks = (int32_t)b->checkpoints.size();
+    int32_t queued = 0, checked = 0, swept = 0;
+    while (true) {
+        while (queued < chunks && queued - checked <= kWideLag) {
+            const int32_t q = b->checkpoints[queued];
+            for (; swept < q; ++swept) {
+                if (b->wide_rows == 2) {
+                    wide_pass_kernel<0, 2><<<b->wide_blocks, 256, 0, s>>>(wr, blk_off, R, done);
+                    wide_pass_kernel<1, 2><<<b->wide_blocks, 256, 0, s>>>(wr, blk_off, R, done);
+                } else {
+                    wide_pass_kernel<0, 4><<<b->wide_blocks, 256, 0, s>>>(wr, blk_off, R, done);
+                    wide_pass_kernel<1, 4><<<b->wide_blocks, 256, 0, s>>>(wr, blk_off, R, done);
+                }
+                WBC_TRY_ERR(hipGetLastError());
+            }
+            wide_residual_kernel<<<L.wide_res_blocks, 256, 0, s>>>(wr, rblk_off, R, done, q, partial);
+            WBC_TRY_ERR(hipGetLastError());
+            wide_finish_kernel<<<R, 256, 0, s>>>(wr, rblk_off, done, q, partial, st, totals,
+                                                 b->flags_dev + (size_t)queued * R);
+            WBC_TRY_ERR(hipGetLastError());
+            WBC_TRY_ERR(hipEventRecord(b->ev[queued % 8], s));
+            ++queued;
+        }
+        if (checked == queued) break;
+        WBC_TRY_ERR(hipEventSynchronize(b->ev[checked % 8]));
+        const volatile int32_t* f = b->flags_host + (size_t)checked * R;
+        bool any = false;
+        for (int r = 0; r < R; ++r) any = any || f[r] != 0;
+        ++checked;
+        if (!any) break;
+    }
+    if (L.nodes > 0) {
+        wide_pack_kernel<true><<<2048, 256, 0, s>>>(wr, node_off, R, L.nodes, b->field, density);
+        WBC_TRY_ERR(hipGetLastError());
+    }
+    return hipSuccess;
+}
+
+// The step before the solve: deposit_combined (zero, deposit, finalize).
+static hipError_t enqueue_pre(wbc_batch* b, hipStream_t s)
 {
     const BatchLayout& L = b->L;
     char* ws = b->ws;
@@ -758,7 +1086,18 @@ static hipError_t enqueue_step(wbc_batch* b, hipStream_t s)
     batch_finalize_kernel<<<grid_for(L.nodes, 256), 256, 0, s>>>(rep, b->R, (const int64_t*)(ws + L.node_off),
                                                                  L.nodes, (float*)(ws + L.density));
     WBC_TRY_ERR(hipGetLastError());
-    WBC_TRY_ERR(enqueue_solve(b, s));
+    return hipSuccess;
+}
+
+// The step after the solve: sample, lifecycle decisions (no daughters), contact substeps.
+static hipError_t enqueue_post(wbc_batch* b, hipStream_t s)
+{
+    const BatchLayout& L = b->L;
+    char* ws = b->ws;
+    const ReplicaDev* rep = (const ReplicaDev*)(ws + L.rep);
+    const int32_t* rep_of_slot = (const int32_t*)(ws + L.rep_of_slot);
+    const wbc_batch_cells& c = b->cells;
+    const unsigned slot_blocks = grid_for(L.slots, 256);
     if (L.slots == 0) return hipSuccess;
     batch_sample_kernel<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, c.position, c.oxygen_local);
     WBC_TRY_ERR(hipGetLastError());
@@ -788,6 +1127,54 @@ static hipError_t enqueue_step(wbc_batch* b, hipStream_t s)
         WBC_TRY_ERR(hipGetLastError());
     }
     return hipSuccess;
+}
+
+static hipError_t enqueue_step(wbc_batch* b, hipStream_t s)
+{
+    WBC_TRY_ERR(enqueue_pre(b, s));
+    WBC_TRY_ERR(enqueue_solve(b, s));
+    return enqueue_post(b, s);
+}
+
+// Wide-solve state: each replica's descriptor (its colour-packed arrays in the workspace), the
+// checkpoints (every sweep count after which some replica checks), the completion events and the
+// mapped activity flags.
+static hipError_t wide_setup(wbc_batch* b, const wbc_replica* replicas, const std::vector<ReplicaDev>& dev)
+{
+    const int R = b->R;
+    b->wide.assign(R, WideRep{});
+    size_t off = b->L.packed;
+    std::vector<int32_t> qs;
+    for (int r = 0; r < R; ++r) {
+        const wbc_replica& h = replicas[r];
+        WideRep& w = b->wide[r];
+        w.p = dev[r].p;
+        const size_t each = (size_t)((packed_floats(w.p) * sizeof(float) + 255) / 256 * 256);
+        w.pf.f[0] = (float*)(b->ws + off);
+        w.pf.f[1] = (float*)(b->ws + off + each);
+        w.pf.d[0] = (float*)(b->ws + off + 2 * each);
+        w.pf.d[1] = (float*)(b->ws + off + 3 * each);
+        off += 4 * each;
+        const int64_t rl = packed_lanes_per_plane(w.p, kResidualRows);
+        w.rlanes = (int32_t)rl;
+        w.rbpp = (int32_t)((rl + 255) / 256);
+        w.magic = nz2_magic_of(w.p);
+        w.q_finite = __builtin_isfinite(h.uptake_max) ? 1 : 0;
+        w.max_sweeps = h.max_sweeps;
+        w.check_every = h.check_every;
+        w.tolerance = h.tolerance;
+        w.node_offset = h.node_offset;
+        for (int32_t q = h.check_every; q < h.max_sweeps; q += h.check_every) qs.push_back(q);
+        qs.push_back(h.max_sweeps);
+    }
+    std::sort(qs.begin(), qs.end());
+    qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
+    b->checkpoints = qs;
+    for (int e = 0; e < 8; ++e) WBC_TRY_ERR(hipEventCreateWithFlags(&b->ev[e], hipEventDisableTiming));
+    WBC_TRY_ERR(hipHostMalloc((void**)&b->flags_host, sizeof(int32_t) * qs.size() * R,
+                              hipHostMallocMapped | hipHostMallocCoherent));
+    WBC_TRY_ERR(hipHostGetDevicePointer((void**)&b->flags_dev, b->flags_host, 0));
+    return wide_configure(b, kWideRows[0]);
 }
 
 WBC_API size_t wbc_batch_workspace_bytes(const wbc_replica* replicas, int32_t count, int32_t hash_dim)
@@ -890,8 +1277,9 @@ WBC_API int wbc_batch_create(const wbc_replica* replicas, int32_t count, int32_t
     b->ws = ws;
     hipError_t e = hipGetDevice(&b->device);
     if (e == hipSuccess) e = batch_grid(b);
+    if (e == hipSuccess && L.wide_ok) e = wide_setup(b, replicas, dev);
     if (e != hipSuccess) {
-        delete b;
+        wbc_batch_destroy(b);
         return (int)e;
     }
     *out = b;
@@ -903,11 +1291,17 @@ WBC_API size_t wbc_batch_struct_size(int32_t which)
     return which == 0 ? sizeof(wbc_replica) : which == 1 ? sizeof(wbc_batch_cells) : 0;
 }
 
-WBC_API int wbc_batch_schedule_count(void) { return 2 * kBatchTileCount; }
+WBC_API int wbc_batch_schedule_count(void) { return kScheduleCount; }
 
 WBC_API int wbc_batch_schedule_shape(int32_t tile, int32_t* tj, int32_t* ti, int32_t* max_blocks_per_cu)
 {
-    if (tile < 0 || tile >= 2 * kBatchTileCount || !tj || !ti || !max_blocks_per_cu) return (int)hipErrorInvalidValue;
+    if (tile < 0 || tile >= kScheduleCount || !tj || !ti || !max_blocks_per_cu) return (int)hipErrorInvalidValue;
+    if (tile >= 2 * kBatchTileCount) {  // wide: rows per thread, no tile, no occupancy choice
+        *tj = kWideRows[tile - 2 * kBatchTileCount];
+        *ti = 0;
+        *max_blocks_per_cu = 1;
+        return 0;
+    }
     *tj = kBatchTiles[tile % kBatchTileCount].tj;
     *ti = kBatchTiles[tile % kBatchTileCount].ti;
     int per_cu = 0;
@@ -918,7 +1312,15 @@ WBC_API int wbc_batch_schedule_shape(int32_t tile, int32_t* tj, int32_t* ti, int
 
 WBC_API int wbc_batch_set_schedule(wbc_batch_t b, int32_t tile, int32_t blocks_per_cu)
 {
-    if (!b || tile < 0 || tile >= 2 * kBatchTileCount || blocks_per_cu < 0) return (int)hipErrorInvalidValue;
+    if (!b || tile < 0 || tile >= kScheduleCount || blocks_per_cu < 0) return (int)hipErrorInvalidValue;
+    if (tile >= 2 * kBatchTileCount) {
+        if (!b->L.wide_ok) return (int)hipErrorInvalidValue;
+        WBC_TRY(wide_configure(b, kWideRows[tile - 2 * kBatchTileCount]));
+        b->tile = tile;
+        b->blocks_per_cu = 0;
+        drop_graph(b);
+        return 0;
+    }
     const int old_tile = b->tile, old_bpc = b->blocks_per_cu;
     b->tile = tile;
     b->blocks_per_cu = blocks_per_cu;
@@ -933,10 +1335,56 @@ WBC_API int wbc_batch_set_schedule(wbc_batch_t b, int32_t tile, int32_t blocks_p
     return 0;
 }
 
+// Capture `enqueue` on the library's capture stream into (graph, exec).
+static hipError_t capture(wbc_batch* b, hipError_t (*enqueue)(wbc_batch*, hipStream_t), hipGraph_t* graph,
+                          hipGraphExec_t* exec)
+{
+    if (!b->capture_stream) WBC_TRY_ERR(hipStreamCreateWithFlags(&b->capture_stream, hipStreamNonBlocking));
+    WBC_TRY_ERR(hipStreamBeginCapture(b->capture_stream, hipStreamCaptureModeThreadLocal));
+    hipError_t e = enqueue(b, b->capture_stream);
+    hipGraph_t g = nullptr;
+    hipError_t e2 = hipStreamEndCapture(b->capture_stream, &g);
+    if (e != hipSuccess || e2 != hipSuccess) {
+        if (g) (void)hipGraphDestroy(g);
+        return e != hipSuccess ? e : e2;
+    }
+    *graph = g;
+    e = hipGraphInstantiate(exec, g, nullptr, nullptr, 0);
+    if (e != hipSuccess) {
+        (void)hipGraphDestroy(g);
+        *graph = nullptr;
+        *exec = nullptr;
+    }
+    return e;
+}
+
+// A wide step: the parts before and after the solve (as graphs when use_graph), the host-driven
+// wide solve between them.
+static hipError_t wide_steps(wbc_batch* b, int32_t steps, bool use_graph, hipStream_t s)
+{
+    if (use_graph && !b->exec_pre) {
+        hipError_t e = capture(b, enqueue_pre, &b->graph_pre, &b->exec_pre);
+        if (e == hipSuccess) e = capture(b, enqueue_post, &b->graph_post, &b->exec_post);
+        if (e != hipSuccess) {
+            drop_graph(b);
+            return e;
+        }
+    }
+    for (int i = 0; i < steps; ++i) {
+        if (use_graph) WBC_TRY_ERR(hipGraphLaunch(b->exec_pre, s));
+        else WBC_TRY_ERR(enqueue_pre(b, s));
+        WBC_TRY_ERR(wide_solve(b, s));
+        if (use_graph) WBC_TRY_ERR(hipGraphLaunch(b->exec_post, s));
+        else WBC_TRY_ERR(enqueue_post(b, s));
+    }
+    return hipSuccess;
+}
+
 WBC_API int wbc_batch_step(wbc_batch_t b, int32_t steps, int32_t use_graph, hipStream_t stream)
 {
     if (!b || steps < 0) return (int)hipErrorInvalidValue;
     if (steps == 0 && !use_graph) return 0;
+    if (is_wide(b)) return (int)wide_steps(b, steps, use_graph != 0, stream);
     if (!use_graph) {
         for (int i = 0; i < steps; ++i) WBC_TRY(enqueue_step(b, stream));
         return 0;
@@ -965,7 +1413,7 @@ WBC_API int wbc_batch_step(wbc_batch_t b, int32_t steps, int32_t use_graph, hipS
 WBC_API int wbc_batch_solve(wbc_batch_t b, hipStream_t stream)
 {
     if (!b) return (int)hipErrorInvalidValue;
-    return (int)enqueue_solve(b, stream);
+    return (int)(is_wide(b) ? wide_solve(b, stream) : enqueue_solve(b, stream));
 }
 
 WBC_API int wbc_batch_report(wbc_batch_t b, hipStream_t stream, int32_t* sweeps, float* residual, int32_t* converged,
@@ -994,6 +1442,9 @@ WBC_API int wbc_batch_destroy(wbc_batch_t b)
     if (!b) return 0;
     drop_graph(b);
     if (b->capture_stream) (void)hipStreamDestroy(b->capture_stream);
+    for (hipEvent_t e : b->ev)
+        if (e) (void)hipEventDestroy(e);
+    if (b->flags_host) (void)hipHostFree(b->flags_host);
     delete b;
     return 0;
 }

@@ -10,9 +10,12 @@ all of them per call of :meth:`BatchedSweep.step`. A step is, per replica:
 2. the lifecycle decisions;
 3. the contact relaxation substeps.
 
-All replicas' solves run in one persistent kernel launch. The whole step is captured once in a
-hipGraph and replayed. Each replica's state after a step is bit-identical to ``cell_step`` on that
-replica alone, including its solve report.
+The solves of all replicas run under one of two schedule families (``set_schedule``, chosen by
+``autotune.tune_batch``): one persistent kernel launch with grid barriers between sweeps, the whole
+step captured once in a hipGraph and replayed; or wide launches, one per colour pass over every
+replica's colour-packed planes, driven by the host chunk by chunk between two captured graphs
+(the step's parts before and after the solve). Each replica's state after a step is
+bit-identical to ``cell_step`` on that replica alone, including its solve report.
 
 Limits:
 
@@ -211,7 +214,8 @@ class BatchedSweep:
 
     # ---- stepping -------------------------------------------------------------------------
     def step(self, steps: int = 1, use_graph: bool = True) -> None:
-        """Enqueue ``steps`` cell steps of every replica (no host synchronisation)."""
+        """Run ``steps`` cell steps of every replica: enqueued without host synchronisation under a
+        persistent schedule; a wide schedule waits on its solve chunks (a few behind the device)."""
         _lib.call("wbc_batch_step", self._h(), int(steps), 1 if use_graph else 0, self.stream)
 
     def capture(self) -> None:
@@ -227,7 +231,7 @@ class BatchedSweep:
         self.schedule = (int(tile), int(blocks_per_cu))
 
     def report(self) -> list[SolveReport]:
-        """The last solve of every replica (synchronises the stream); raises if the persistent
+        """The last solve of every replica (synchronises the stream); raises if a persistent
         launch's grid barrier timed out."""
         n = len(self.replicas)
         sweeps, residual, conv = (c_int32 * n)(), (c_float * n)(), (c_int32 * n)()
