@@ -601,6 +601,16 @@ struct WideRep {
     int64_t node_offset;  // first node of the replica in the concatenated unpacked arrays
 };
 
+// The replicas one wide launch covers (by value in the kernel arguments): workgroups
+// [off[k], off[k+1]) belong to replica rep[k]. The host drops the replicas it knows have finished,
+// so a launch's grid shrinks as replicas converge; a replica that finished since (the host learns
+// a few chunks late) is stopped by its done flag.
+struct WideGrid {
+    int32_t n;
+    int32_t rep[kMaxReplicas];
+    int32_t off[kMaxReplicas + 1];
+};
+
 __device__ __forceinline__ bool wide_is_check(const WideRep& w, int32_t q)
 {
     return q >= 1 && q <= w.max_sweeps && (q % w.check_every == 0 || q == w.max_sweeps);
@@ -620,16 +630,16 @@ __device__ __forceinline__ int range_of(const T* __restrict__ off, int R, T b)
 }
 
 template <int C, int RJ>
-__global__ void __launch_bounds__(256) wide_pass_kernel(const WideRep* __restrict__ wr,
-                                                         const int32_t* __restrict__ blk_off, int R,
+__global__ void __launch_bounds__(256) wide_pass_kernel(const WideRep* __restrict__ wr, const WideGrid g,
                                                          const int32_t* __restrict__ done)
 {
     const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int r = range_of(blk_off, R, t);
+    const int k = range_of(g.off, g.n, t);
+    const int r = g.rep[k];
     if (done[r]) return;
     const WideRep& w = wr[r];
     int i, u;
-    packed_lane(t - blk_off[r], w.bpp, i, u);
+    packed_lane(t - g.off[k], w.bpp, i, u);
     if (u >= w.lanes) return;
     float* own = w.pf.f[C];
     const float* other = w.pf.f[C ^ 1];
@@ -640,19 +650,20 @@ __global__ void __launch_bounds__(256) wide_pass_kernel(const WideRep* __restric
         packed_update<RJ, 0>(w.p, own, other, dens, i, (uint32_t)u, w.magic, w.q_finite);
 }
 
-// Residual partials (both colours) of every replica that is active and checks after sweep q: the
-// replica's workgroups [rblk_off[r], rblk_off[r+1]) write one maximum each.
-__global__ void __launch_bounds__(256) wide_residual_kernel(const WideRep* __restrict__ wr,
-                                                             const int32_t* __restrict__ rblk_off, int R,
+// Residual partials (both colours) of the replicas in g that are active and check after sweep q:
+// the k-th one's workgroups write one maximum each, into partial[rblk_off[r] ..].
+__global__ void __launch_bounds__(256) wide_residual_kernel(const WideRep* __restrict__ wr, const WideGrid g,
+                                                             const int32_t* __restrict__ rblk_off,
                                                              const int32_t* __restrict__ done, int32_t q,
                                                              float* __restrict__ partial)
 {
     __shared__ float wave_max_lds[4];
     const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int r = range_of(rblk_off, R, t);
+    const int k = range_of(g.off, g.n, t);
+    const int r = g.rep[k];
     const WideRep& w = wr[r];
     if (done[r] || !wide_is_check(w, q)) return;
-    const int local = t - rblk_off[r];
+    const int local = t - g.off[k];
     const int B = w.rbpp * (w.p.nx - 2);
     const int colour = local >= B ? 1 : 0;
     int i, u;
@@ -671,7 +682,8 @@ __global__ void __launch_bounds__(256) wide_residual_kernel(const WideRep* __res
     if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;
     __syncthreads();
     if (threadIdx.x == 0)
-        partial[t] = fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
+        partial[rblk_off[r] + local] =
+            fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
 }
 
 // One workgroup per replica: for a replica checking after sweep q, reduce its partials, record
@@ -793,7 +805,7 @@ static int tiles_for(int tile, int nx, int ny, int nz)
 // ---- workspace layout -----------------------------------------------------------------------
 struct BatchLayout {
     size_t rep, state, bar, node_off, total_sweeps, partial, acc, density, buf1, buf2, rep_of_slot, cnt, start,
-        scan, key, tmp, id, xyzr, wrep, blk_off, rblk_off, wdone, wpartial, packed, total;
+        scan, key, tmp, id, xyzr, wrep, rblk_off, wdone, wpartial, packed, total;
     int64_t slots, nodes, buckets;
     int32_t partial_stride, active_total, wide_res_blocks;
     bool wide_ok;  // every replica's grid fits the colour-packed passes
@@ -876,7 +888,6 @@ static bool layout_for(const wbc_replica* r, int32_t R, int32_t dim, BatchLayout
     L.id = off;           off += up(sizeof(int32_t) * slots);
     L.xyzr = off;         off += up(sizeof(float4) * slots);
     L.wrep = off;         off += up(sizeof(WideRep) * R);
-    L.blk_off = off;      off += up(sizeof(int32_t) * (R + 1));
     L.rblk_off = off;     off += up(sizeof(int32_t) * (R + 1));
     L.wdone = off;        off += up(sizeof(int32_t) * R);
     L.wpartial = off;     off += up(sizeof(float) * res_blocks);
@@ -912,7 +923,7 @@ struct wbc_batch {
     // wide solve
     std::vector<WideRep> wide;         // per replica, for the current rows per thread
     std::vector<int32_t> checkpoints;  // sweep counts after which some replica checks
-    int32_t wide_rows = 0, wide_blocks = 0;
+    int32_t wide_rows = 0;
     int32_t* flags_host = nullptr;     // mapped: checkpoints.size() x R activity flags
     int32_t* flags_dev = nullptr;
     hipEvent_t ev[8] = {};
@@ -934,11 +945,12 @@ static void drop_graph(wbc_batch* b)
     b->graph = b->graph_pre = b->graph_post = nullptr;
 }
 
-// The wide descriptors and block offsets for `rows` rows per thread, uploaded to the workspace.
+// The wide descriptors for `rows` rows per thread and the residual partial offsets, uploaded to
+// the workspace.
 static hipError_t wide_configure(wbc_batch* b, int rows)
 {
     const int R = b->R;
-    std::vector<int32_t> blk(R + 1), rblk(R + 1);
+    std::vector<int32_t> rblk(R + 1);
     int64_t acc = 0, racc = 0;
     for (int r = 0; r < R; ++r) {
         WideRep& w = b->wide[r];
@@ -946,19 +958,15 @@ static hipError_t wide_configure(wbc_batch* b, int rows)
         if (lanes < 0) return hipErrorInvalidValue;
         w.lanes = (int32_t)lanes;
         w.bpp = (int32_t)((lanes + 255) / 256);
-        blk[r] = (int32_t)acc;
         rblk[r] = (int32_t)racc;
         acc += (int64_t)w.bpp * (w.p.nx - 2);
         racc += wide_res_blocks_of(w.p);
     }
     if (acc >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
-    blk[R] = (int32_t)acc;
     rblk[R] = (int32_t)racc;
     WBC_TRY_ERR(hipMemcpy(b->ws + b->L.wrep, b->wide.data(), sizeof(WideRep) * R, hipMemcpyHostToDevice));
-    WBC_TRY_ERR(hipMemcpy(b->ws + b->L.blk_off, blk.data(), sizeof(int32_t) * (R + 1), hipMemcpyHostToDevice));
     WBC_TRY_ERR(hipMemcpy(b->ws + b->L.rblk_off, rblk.data(), sizeof(int32_t) * (R + 1), hipMemcpyHostToDevice));
     b->wide_rows = rows;
-    b->wide_blocks = (int32_t)acc;
     return hipSuccess;
 }
 
@@ -1015,7 +1023,6 @@ static hipError_t wide_solve(wbc_batch* b, hipStream_t s)
     char* ws = b->ws;
     const int R = b->R;
     const WideRep* wr = (const WideRep*)(ws + L.wrep);
-    const int32_t* blk_off = (const int32_t*)(ws + L.blk_off);
     const int32_t* rblk_off = (const int32_t*)(ws + L.rblk_off);
     int32_t* done = (int32_t*)(ws + L.wdone);
     float* partial = (float*)(ws + L.wpartial);
@@ -1030,22 +1037,46 @@ static hipError_t wide_solve(wbc_batch* b, hipStream_t s)
         WBC_TRY_ERR(hipGetLastError());
     }
     const int32_t chunks = (int32_t)b->checkpoints.size();
+    // The launches cover the replicas the host does not yet know to have finished (residual
+    // launches: only those checking after sweep q).
+    std::vector<char> known_done(R, 0);
+    auto grid_of = [&](bool residual, int32_t q, WideGrid& g) {
+        g.n = 0;
+        int32_t acc = 0;
+        for (int r = 0; r < R; ++r) {
+            const WideRep& w = b->wide[r];
+            if (known_done[r]) continue;
+            if (residual && !(q <= w.max_sweeps && (q % w.check_every == 0 || q == w.max_sweeps))) continue;
+            g.rep[g.n] = r;
+            g.off[g.n] = acc;
+            acc += residual ? (int32_t)wide_res_blocks_of(w.p) : w.bpp * (w.p.nx - 2);
+            ++g.n;
+        }
+        g.off[g.n] = acc;
+        return acc;
+    };
+    WideGrid pass{}, res{};
     int32_t queued = 0, checked = 0, swept = 0;
     while (true) {
         while (queued < chunks && queued - checked <= kWideLag) {
             const int32_t q = b->checkpoints[queued];
+            const int32_t pass_blocks = grid_of(false, q, pass);
             for (; swept < q; ++swept) {
+                if (pass_blocks == 0) continue;
                 if (b->wide_rows == 2) {
-                    wide_pass_kernel<0, 2><<<b->wide_blocks, 256, 0, s>>>(wr, blk_off, R, done);
-                    wide_pass_kernel<1, 2><<<b->wide_blocks, 256, 0, s>>>(wr, blk_off, R, done);
+                    wide_pass_kernel<0, 2><<<pass_blocks, 256, 0, s>>>(wr, pass, done);
+                    wide_pass_kernel<1, 2><<<pass_blocks, 256, 0, s>>>(wr, pass, done);
                 } else {
-                    wide_pass_kernel<0, 4><<<b->wide_blocks, 256, 0, s>>>(wr, blk_off, R, done);
-                    wide_pass_kernel<1, 4><<<b->wide_blocks, 256, 0, s>>>(wr, blk_off, R, done);
+                    wide_pass_kernel<0, 4><<<pass_blocks, 256, 0, s>>>(wr, pass, done);
+                    wide_pass_kernel<1, 4><<<pass_blocks, 256, 0, s>>>(wr, pass, done);
                 }
                 WBC_TRY_ERR(hipGetLastError());
             }
-            wide_residual_kernel<<<L.wide_res_blocks, 256, 0, s>>>(wr, rblk_off, R, done, q, partial);
-            WBC_TRY_ERR(hipGetLastError());
+            const int32_t res_blocks = grid_of(true, q, res);
+            if (res_blocks > 0) {
+                wide_residual_kernel<<<res_blocks, 256, 0, s>>>(wr, res, rblk_off, done, q, partial);
+                WBC_TRY_ERR(hipGetLastError());
+            }
             wide_finish_kernel<<<R, 256, 0, s>>>(wr, rblk_off, done, q, partial, st, totals,
                                                  b->flags_dev + (size_t)queued * R);
             WBC_TRY_ERR(hipGetLastError());
@@ -1056,7 +1087,10 @@ static hipError_t wide_solve(wbc_batch* b, hipStream_t s)
         WBC_TRY_ERR(hipEventSynchronize(b->ev[checked % 8]));
         const volatile int32_t* f = b->flags_host + (size_t)checked * R;
         bool any = false;
-        for (int r = 0; r < R; ++r) any = any || f[r] != 0;
+        for (int r = 0; r < R; ++r) {
+            if (f[r] == 0) known_done[r] = 1;
+            any = any || f[r] != 0;
+        }
         ++checked;
         if (!any) break;
     }
