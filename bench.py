@@ -110,6 +110,7 @@ def sor_roofline(ox, geom, device, workload: str, launches: int = 200) -> dict:
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(shape, workload),
             "kernel": KERNEL_OF_KIND[family].format(tj=tj.value, ti=ti.value), "shape": shape,
             "launches_per_sweep": 2 if family == "packed" else 1, "sweep_us": round(per_sweep_s * 1e6, 3),
+            "launch_us": round(per_sweep_s * 1e6 / (2 if family == "packed" else 1), 3),
             "algorithmic_bytes_per_sweep": bytes_per_sweep,
             "achieved_at_12B": round(12 * geom.node_count / per_sweep_s / 1e9, 1),
             "traffic_source": str(COUNTER_TABLE.relative_to(ROOT))}
@@ -274,6 +275,9 @@ def main() -> int:
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=100)
+    ap.add_argument("--sor-shape", default=None,
+                    help="run the SOR sweeps with this shape (autotune.config_shapes() name) instead of the "
+                         "autotuner's choice; the autotuner still times every shape")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (counter calibration)")
     args = ap.parse_args()
@@ -306,6 +310,12 @@ def main() -> int:
 
     tune = tune_sor(ox.field, ox.density, ox.species.kinetics, settings.omega)
     log(f"  autotune: tile {tune['shape']} of {tune['us_per_sweep']} (us per sweep)")
+    if args.sor_shape:
+        from warpbiocell_amd.autotune import config_shapes
+
+        _lib.call("wbc_sor_fused_set_config", config_shapes().index(args.sor_shape))
+        tune = dict(tune, forced=args.sor_shape)
+        log(f"  sor shape forced: {args.sor_shape}")
     torch.cuda.synchronize()
     snap = None if args.no_cpu_baseline else snapshot(pop, ox)
 

@@ -211,13 +211,21 @@ int wbc_adjoint_solve(float omega, int32_t max_sweeps, float tolerance, int32_t 
  *   deposit_combined -> steady-state solve -> sample  (fields/species.py update, oxygen weights)
  *   -> decide_lifecycle (no divisions: division_rate must be 0)
  *   -> `substeps` x (hash-grid build, contact gather, x += v dt)   (relax_contacts, no region).
- * The solves of all replicas run in ONE persistent launch: every phase sweeps every replica that
- * has not yet converged, separated by a grid barrier (bounded wait; a timeout sets the error flag
- * read by wbc_batch_report, it never hangs). Each replica's residual is checked on the reference
- * schedule (every check_every sweeps and at max_sweeps), computed inside the next sweep from the
- * values it stages in LDS; its field, sweep count and residual are bit-identical to a serial
- * wbc_solve_steady_state of that replica. wbc_batch_step captures the whole step in a hipGraph
- * on first use (use_graph = 1) and replays it; use_graph = 0 enqueues the same kernels directly.
+ * The solves of all replicas run under one of two schedule families (wbc_batch_set_schedule):
+ *  - persistent: ONE launch; every phase sweeps every replica that has not yet converged,
+ *    separated by a grid barrier (bounded wait; a timeout sets the error flag read by
+ *    wbc_batch_report, it never hangs); residuals computed inside the next sweep from the values
+ *    it stages in LDS. wbc_batch_step captures the whole step in a hipGraph on first use
+ *    (use_graph = 1) and replays it, without host synchronisation;
+ *  - wide: every replica colour-packed, each colour pass of a sweep ONE launch over the replicas
+ *    still sweeping, a residual launch and a per-replica finish after each checkpoint, which
+ *    latches a finished replica's done flag on the device and reports it to the host through
+ *    mapped memory; the host queues chunks a few checkpoints ahead and stops when every replica is
+ *    done. wbc_batch_step (and wbc_batch_solve) then synchronise; with use_graph = 1 the parts of
+ *    the step before and after the solve are captured graphs.
+ * Either way each replica's residual is checked on the reference schedule (every check_every
+ * sweeps and at max_sweeps) and its field, sweep count and residual are bit-identical to a serial
+ * wbc_solve_steady_state of that replica; use_graph = 0 enqueues the same kernels directly.
  *
  * Packing: replica r owns cell slots [cell_offset, cell_offset + capacity) of every per-cell array
  * (cell_offset = sum of the capacities before it) and field nodes [node_offset, node_offset +
@@ -268,17 +276,19 @@ size_t wbc_batch_workspace_bytes(const wbc_replica* replicas, int32_t count, int
  * counts; `cells`, `field` and `workspace` must outlive the handle. Allocates the handle (host). */
 int wbc_batch_create(const wbc_replica* replicas, int32_t count, int32_t hash_dim, const wbc_batch_cells* cells,
                      float* field, void* workspace, size_t workspace_bytes, wbc_stream_t stream, wbc_batch_t* out);
-/* Schedule of the persistent solve: tile shape `tile` in [0, wbc_batch_schedule_count()) (60 x TJ x TI
- * nodes, see wbc_batch_schedule_shape; the second half of the table runs the same shapes with each
- * workgroup copying its next tile while computing the current one) and workgroups per CU (0: as
- * many as fit). Resets the graph. */
+/* Schedule of the solve, `tile` in [0, wbc_batch_schedule_count()). Persistent entries first: tile
+ * shape 60 x TJ x TI nodes (wbc_batch_schedule_shape; the second half of those runs the same shapes
+ * with each workgroup copying its next tile while computing the current one) with workgroups per
+ * CU (0: as many as fit). Then the wide entries, reported with TJ = rows per thread, TI = 0 and
+ * one occupancy choice (blocks_per_cu ignored); setting one fails when a replica's grid is too
+ * large for the packed passes. Resets the graphs. */
 int wbc_batch_schedule_count(void);
 int wbc_batch_schedule_shape(int32_t tile, int32_t* tj, int32_t* ti, int32_t* max_blocks_per_cu);
 int wbc_batch_set_schedule(wbc_batch_t batch, int32_t tile, int32_t blocks_per_cu);
-/* `steps` cell steps of every replica, enqueued on `stream`. With use_graph = 1 the step is
- * captured on first use; steps = 0 with use_graph = 1 only captures it. */
+/* `steps` cell steps of every replica on `stream`. With use_graph = 1 the step (wide schedules: its
+ * parts around the solve) is captured on first use; steps = 0 with use_graph = 1 only captures. */
 int wbc_batch_step(wbc_batch_t batch, int32_t steps, int32_t use_graph, wbc_stream_t stream);
-/* Only the steady-state solves of every replica (one persistent launch), from the current fields. */
+/* Only the steady-state solves of every replica (under the current schedule), from the current fields. */
 int wbc_batch_solve(wbc_batch_t batch, wbc_stream_t stream);
 /* Synchronizes `stream`, then copies the last solve's report per replica (any pointer may be NULL):
  * sweeps, residual, converged, sweeps summed over every solve since creation; *error is nonzero
