@@ -116,6 +116,17 @@ def sor_roofline(ox, geom, device, workload: str, launches: int = 200) -> dict:
             "traffic_source": str(COUNTER_TABLE.relative_to(ROOT))}
 
 
+def quiesce_gc() -> None:
+    """Run Python's full garbage collection now and move every object that exists to the permanent
+    generation, so that no full collection over the import- and setup-time objects (tens of ms with
+    torch loaded) falls inside a timed region; which step it would hit depends on object counts
+    that differ between a first run (bytecode compiled at import) and later ones."""
+    import gc
+
+    gc.collect()
+    gc.freeze()
+
+
 def adjoint_measurement(device) -> dict:
     """Pattern 3 on its own workload (100k cells, 129^3): one forward solve from a uniform start and
     one adjoint gradient (w.r.t. all per-cell rates, D and O_b) of sum (S - target)^2."""
@@ -183,6 +194,7 @@ def batched_measurement(device, steps: int) -> dict:
     log(f"  batched sweep: {len(ws)} replicas, {sum(w.n for w in ws)} cells, "
         f"{sum(sc.geometry.node_count for sc in batched)} nodes; schedule {tune['schedule']}")
 
+    quiesce_gc()
     t0 = time.perf_counter()
     serial_sweeps = 0
     for sc, cfg in zip(serial, configs):
@@ -320,6 +332,7 @@ def main() -> int:
     snap = None if args.no_cpu_baseline else snapshot(pop, ox)
 
     reports, cell_steps = [], 0
+    quiesce_gc()
     torch.cuda.synchronize()
     t0 = time.perf_counter()
     for _ in range(args.steps):
