@@ -203,6 +203,52 @@ int wbc_adjoint_solve(float omega, int32_t max_sweeps, float tolerance, int32_t 
                       int32_t* sweeps_out, float* residual_out, void* workspace, size_t workspace_bytes,
                       wbc_stream_t stream);
 
+/* ---- population metrics ---------------------------------------------------------------------
+ * Device-side reductions behind metrics/population.py:13-106 (state_counts, population_summary,
+ * oxygen_summary, radial_profile), which in the reference copy the whole position, radius, state
+ * and oxygen arrays and the field to the host at every metrics interval (experiment.py:265-271).
+ * Floating-point sums are taken in double, per workgroup in a fixed tree order and then in
+ * workgroup order, so results repeat bit for bit; minima are order-free; counts are exact. Every
+ * output pointer below is DEVICE memory; the host reads a few numbers back. All take a
+ * workspace of wbc_metrics_workspace_bytes(n, n_bins) bytes (n_bins = 0 when no profile). */
+typedef struct wbc_cell_moments_t {
+    double sum_x, sum_y, sum_z, sum_radius, sum_oxygen_alive;  /* over cells [0, n) / living cells */
+    float min_oxygen_alive;                                     /* +inf when no cell lives */
+    int32_t count, alive, states[4];                            /* n, living cells, per CellState */
+} wbc_cell_moments_t;
+
+typedef struct wbc_grid_moments_t {
+    double sum;      /* over the updated nodes: not on a Dirichlet face, not pinned */
+    float min;
+    int32_t count;
+} wbc_grid_moments_t;
+
+size_t wbc_metrics_workspace_bytes(int32_t n, int32_t n_bins);
+/* count_states (kernels/cell_kernels.py) and the sums behind population_summary's mean position
+ * and radius and oxygen_summary's living-cell oxygen. */
+int wbc_cell_moments(const float* position, const float* radius, const int32_t* cell_state,
+                     const float* oxygen_local, int32_t n, void* workspace, size_t workspace_bytes,
+                     wbc_cell_moments_t* out, wbc_stream_t stream);
+/* r[c] = |x_c - centroid| in float32 as numpy.linalg.norm computes it (differences, squares summed
+ * in component order, square root); *r_max = max r. centroid: 3 floats (device). n >= 1. */
+int wbc_centre_distances(const float* position, int32_t n, const float* centroid, float* r, float* r_max,
+                         void* workspace, size_t workspace_bytes, wbc_stream_t stream);
+/* *out = the rank-th smallest (0-based) of n non-negative floats (radix selection), for the
+ * percentile of population_summary's spheroid_radius. */
+int wbc_select_rank(const float* values, int32_t n, int32_t rank, float* out, void* workspace,
+                    size_t workspace_bytes, wbc_stream_t stream);
+/* oxygen_summary's grid mean and minimum over ScalarField.interior_mask (scalar_field.py:143-154). */
+int wbc_grid_moments(const float* field, const int32_t* fixed, int32_t bc_x, int32_t bc_y, int32_t bc_z, int32_t nx,
+                     int32_t ny, int32_t nz, void* workspace, size_t workspace_bytes, wbc_grid_moments_t* out,
+                     wbc_stream_t stream);
+/* radial_profile (metrics/population.py:83-106): shell b of cell c is numpy.digitize(r[c], edges) - 1
+ * capped at n_bins - 1 (edges: n_bins + 1 doubles, device). counts[5 b] = cells in shell b,
+ * counts[5 b + 1 + s] = those in state s; sums[3 b + k] = sum of value_k over the shell's cells
+ * (value_k may be NULL: sum 0). */
+int wbc_radial_profile(const float* r, const int32_t* cell_state, const float* value0, const float* value1,
+                       const float* value2, int32_t n, const double* edges, int32_t n_bins, int32_t* counts,
+                       double* sums, void* workspace, size_t workspace_bytes, wbc_stream_t stream);
+
 /* ---- batched replica sweep -----------------------------------------------------------------
  * No reference counterpart: the reference runs a parameter sweep (simulation/sweep.py:115-162) as
  * sequential Experiments, each calling cell_step (simulation/simulator.py:58-81) on its own

@@ -486,3 +486,94 @@ def adjoint_f64(positions, grad_S, field, Q, diffusion, michaelis_k, origin, dx,
     nd[:, :, -2] += 1
     grad_Ob = float(g[face].sum() + np.sum(mu[inner] * nd[inner]))
     return grad_q, grad_D, grad_Ob, mu
+
+
+# ---- population metrics (metrics/population.py) ---------------------------------------------------
+# numpy restatements of the reference's host-side metrics, in its expressions and order, on the
+# arrays of cells [0, count): positions (n, 3) float32, radii, states int32, per-cell values float32.
+_DEAD = 3
+_STATE_NAMES = ("proliferative", "quiescent", "hypoxic", "dead")
+
+
+def metrics_state_counts(states: np.ndarray) -> list[int]:
+    """state_counts (metrics/population.py:13-19): cells per state."""
+    return [int(np.count_nonzero(states == s)) for s in range(4)]
+
+
+def metrics_population_summary(positions, radii, states) -> dict:
+    """population_summary without the oxygen part (metrics/population.py:22-45)."""
+    counts = metrics_state_counts(states)
+    x = positions
+    if x.shape[0] == 0:
+        centre_distance = np.zeros(0)
+    else:
+        centre_distance = np.linalg.norm(x - x.mean(axis=0), axis=1)
+    n = int(x.shape[0])
+    return {
+        "cells": n,
+        "living_cells": n - counts[_DEAD],
+        "dead_cells": counts[_DEAD],
+        "proliferative_cells": counts[0],
+        "quiescent_cells": counts[1],
+        "hypoxic_cells": counts[2],
+        "spheroid_radius": float(np.percentile(centre_distance, 99)) if centre_distance.size else 0.0,
+        "mean_radius": float(radii.mean()) if n else 0.0,
+    }
+
+
+def metrics_oxygen_summary(states, oxygen_local, field_values, interior, sweeps: int) -> dict:
+    """oxygen_summary (metrics/population.py:48-64); interior = ScalarField.interior_mask()."""
+    alive = states != _DEAD
+    at_cells = oxygen_local[alive]
+    grid = field_values[interior]
+    return {
+        "oxygen_cells_mean": float(at_cells.mean()) if at_cells.size else float("nan"),
+        "oxygen_cells_min": float(at_cells.min()) if at_cells.size else float("nan"),
+        "oxygen_grid_mean": float(grid.mean()),
+        "oxygen_grid_min": float(grid.min()),
+        "field_sweeps": sweeps,
+    }
+
+
+def metrics_radial_profile(positions, oxygen_local, glucose_local, states, n_bins=None, shell_width=None,
+                           lactate_local=None, r=None) -> dict:
+    """radial_profile (metrics/population.py:83-119). ``r`` replaces the centre distances when given
+    (to bin exactly the distances another implementation computed)."""
+    x = positions
+    if r is None:
+        r = np.linalg.norm(x - x.mean(axis=0), axis=1)
+    o, s = oxygen_local, states
+    r_max = float(r.max()) if r.size else 0.0
+    if shell_width is not None:
+        n_bins = max(1, int(np.ceil((r_max + 1e-6) / shell_width)))
+        edges = shell_width * np.arange(n_bins + 1)
+    else:
+        n_bins = n_bins or 10
+        edges = np.linspace(0.0, r_max + 1e-6, n_bins + 1)
+    which = np.minimum(np.digitize(r, edges) - 1, n_bins - 1)
+    out = {"r_outer": edges[1:], "cells": np.bincount(which, minlength=n_bins)}
+    out["oxygen_mean"] = np.array([o[which == b].mean() if np.any(which == b) else np.nan for b in range(n_bins)])
+    values = {"glucose": glucose_local}
+    if lactate_local is not None:
+        values["lactate"] = lactate_local
+    for name, v in values.items():
+        out[f"{name}_mean"] = np.array([v[which == b].mean() if np.any(which == b) else np.nan for b in range(n_bins)])
+    for k, name in enumerate(_STATE_NAMES):
+        out[f"fraction_{name}"] = np.array(
+            [np.mean(s[which == b] == k) if np.any(which == b) else np.nan for b in range(n_bins)])
+    return out
+
+
+def metrics_shell_radii(profile: dict) -> dict:
+    """shell_radii (metrics/population.py:122-135)."""
+
+    def outermost(fraction):
+        hit = np.where(np.nan_to_num(fraction) >= 0.5)[0]
+        return float(profile["r_outer"][hit.max()]) if hit.size else 0.0
+
+    dead = profile["fraction_dead"]
+    return {
+        "necrotic_radius": outermost(dead),
+        "hypoxic_radius": outermost(profile["fraction_hypoxic"] + dead),
+        "non_proliferative_radius": outermost(1.0 - profile["fraction_proliferative"]),
+    }

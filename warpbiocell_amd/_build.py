@@ -27,7 +27,7 @@ CSRC = PACKAGE_DIR / "csrc"
 INCLUDE = PACKAGE_DIR.parent / "include"
 LIBRARY = PACKAGE_DIR / "libwbc_hip.so"
 SOURCES = ("wbc_scan.hip", "wbc_cells.hip", "wbc_field.hip", "wbc_sor_fused.hip", "wbc_sor_march.hip",
-           "wbc_sor_packed.hip", "wbc_mechanics.hip", "wbc_batched.hip", "wbc_adjoint.hip")
+           "wbc_sor_packed.hip", "wbc_mechanics.hip", "wbc_batched.hip", "wbc_adjoint.hip", "wbc_metrics.hip")
 ARCH = "gfx950"
 
 HIP_FLAGS = [

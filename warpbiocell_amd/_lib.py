@@ -62,6 +62,12 @@ SIGNATURES: dict[str, tuple] = {
     "wbc_adjoint_workspace_bytes": (c_size_t, [I32, I32, I32]),
     "wbc_adjoint_solve": (I32, [F32, I32, F32, I32, F32, F32, F32, P, P, P, P, P, P, P, I64, F32, F32, F32, F32,
                                 I32, I32, I32, P, P, P, P, P, c_size_t, P]),
+    "wbc_metrics_workspace_bytes": (c_size_t, [I32, I32]),
+    "wbc_cell_moments": (I32, [P, P, P, P, I32, P, c_size_t, P, P]),
+    "wbc_centre_distances": (I32, [P, I32, P, P, P, P, c_size_t, P]),
+    "wbc_select_rank": (I32, [P, I32, I32, P, P, c_size_t, P]),
+    "wbc_grid_moments": (I32, [P, P, I32, I32, I32, I32, I32, I32, P, c_size_t, P, P]),
+    "wbc_radial_profile": (I32, [P, P, P, P, P, I32, P, I32, P, P, P, c_size_t, P]),
     "wbc_batch_struct_size": (c_size_t, [I32]),
     "wbc_batch_workspace_bytes": (c_size_t, [P, I32, I32]),
     "wbc_batch_create": (I32, [P, I32, I32, P, P, P, c_size_t, P, P]),
