@@ -192,38 +192,50 @@ def _dtype_suffix(dtype):
     return ("f64", np.float64) if dtype == np.float64 else ("f32", np.float32)
 
 
+def _source_args(source, source_k, dt):
+    """(has_source, source array, source_k): the production factor S/(K_S + S) of a source
+    species (kernels/field_kernels.py:145-150), or none."""
+    if source is None:
+        return 0, None, 1.0
+    return 1, np.ascontiguousarray(source, dtype=dt), source_k
+
+
 def sor_sweep(field: np.ndarray, density: np.ndarray, a: SolveArgs, omega: float, color: int, fixed=None,
-              production=None) -> None:
+              production=None, source=None, source_k: float = 1.0) -> None:
     suf, dt = _dtype_suffix(field.dtype)
     dens = np.ascontiguousarray(density, dtype=dt)
     prod = None if production is None else np.ascontiguousarray(production, dtype=dt)
     fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.int32)
+    hs, src, sk = _source_args(source, source_k, dt)
     getattr(lib(), f"orc_sor_sweep_{suf}")(color, omega, a.dx2_over_D, a.uptake_max, a.michaelis_k, *a.boundary,
-                                            _p(fx), _p(dens), _p(prod), 0, None, 1.0, _p(field), *field.shape)
+                                            _p(fx), _p(dens), _p(prod), hs, _p(src), sk, _p(field), *field.shape)
 
 
-def residual(field: np.ndarray, density: np.ndarray, a: SolveArgs, fixed=None, production=None) -> float:
+def residual(field: np.ndarray, density: np.ndarray, a: SolveArgs, fixed=None, production=None, source=None,
+             source_k: float = 1.0) -> float:
     suf, dt = _dtype_suffix(field.dtype)
     dens = np.ascontiguousarray(density, dtype=dt)
     prod = None if production is None else np.ascontiguousarray(production, dtype=dt)
     fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.int32)
+    hs, src, sk = _source_args(source, source_k, dt)
     return float(getattr(lib(), f"orc_residual_{suf}")(a.dx2_over_D, a.uptake_max, a.michaelis_k, a.reference,
-                                                      *a.boundary, _p(fx), _p(dens), _p(prod), 0, None, 1.0,
+                                                      *a.boundary, _p(fx), _p(dens), _p(prod), hs, _p(src), sk,
                                                       _p(field), *field.shape))
 
 
 def solve(field: np.ndarray, density: np.ndarray, a: SolveArgs, omega: float, max_sweeps: int, tolerance: float,
-          check_every: int, fixed=None, production=None) -> tuple[int, float, bool]:
+          check_every: int, fixed=None, production=None, source=None, source_k: float = 1.0) -> tuple[int, float, bool]:
     """fields/diffusion.py:79-123 in place on ``field`` (float32 or float64)."""
     suf, dt = _dtype_suffix(field.dtype)
     dens = np.ascontiguousarray(density, dtype=dt)
     prod = None if production is None else np.ascontiguousarray(production, dtype=dt)
     fx = None if fixed is None else np.ascontiguousarray(fixed, dtype=np.int32)
+    hs, src, sk = _source_args(source, source_k, dt)
     res = np.zeros(1, dtype=dt)
     conv = np.zeros(1, dtype=np.int32)
     sweeps = getattr(lib(), f"orc_solve_steady_state_{suf}")(
         omega, max_sweeps, tolerance, check_every, a.dx2_over_D, a.uptake_max, a.michaelis_k, a.reference, *a.boundary,
-        _p(fx), _p(dens), _p(prod), 0, None, 1.0, _p(field), *field.shape, _p(res), _p(conv))
+        _p(fx), _p(dens), _p(prod), hs, _p(src), sk, _p(field), *field.shape, _p(res), _p(conv))
     return int(sweeps), float(res[0]), bool(conv[0])
 
 

@@ -174,3 +174,28 @@ def test_colour_pass_touches_only_its_colour(oracle):
     black = (i + j + k) % 2 == 1
     np.testing.assert_array_equal(field[black], before[black])
     assert np.all(field[~black] != before[~black])
+
+
+def test_uniform_production_source_equals_scaled_production(oracle):
+    """A source species scales production by S/(K_S + S) (kernels/field_kernels.py:145-150): with a
+    uniform source the solve equals the one whose production is pre-multiplied by that factor,
+    evaluated as the kernel does (sf = S / (K_S + S), then production * sf, in float32)."""
+    rng = np.random.default_rng(7)
+    shape = (13, 11, 9)
+    density = rng.uniform(0.0, 2e-3, shape).astype(np.float32)
+    production = rng.uniform(0.0, 5.0, shape).astype(np.float32)
+    S, K = np.float32(3.25), np.float32(0.75)
+    sf = np.float32(S / np.float32(K + S))
+    a = oracle.SolveArgs(dx=10.0, diffusion=7.2e5, uptake_max=2.0e6, michaelis_k=0.5, boundary_value=1.0)
+    f1 = np.full(shape, 1.0, np.float32)
+    f2 = f1.copy()
+    r1 = oracle.solve(f1, density, a, 1.7, 400, 1e-6, 10, production=production,
+                      source=np.full(shape, S, np.float32), source_k=float(K))
+    r2 = oracle.solve(f2, density, a, 1.7, 400, 1e-6, 10, production=(production * sf).astype(np.float32))
+    assert r1 == r2
+    np.testing.assert_array_equal(f1, f2)
+    # and the source matters: half the source concentration changes the field
+    f3 = np.full(shape, 1.0, np.float32)
+    oracle.solve(f3, density, a, 1.7, 400, 1e-6, 10, production=production,
+                 source=np.full(shape, S / 2, np.float32), source_k=float(K))
+    assert not np.array_equal(f1, f3)

@@ -47,8 +47,10 @@ def save_checkpoint(path: str | Path, population: CellPopulation, oxygen, time_h
     for name, values in population.extra_local.items():
         arrays[f"{name}_local"] = values[:n].cpu().numpy().copy()
     if oxygen is not None:
-        arrays["field"] = oxygen.numpy()
-        arrays["density"] = oxygen.density.cpu().numpy()
+        fields = getattr(oxygen, "species_fields", None) or {"oxygen": oxygen}
+        for name, species in fields.items():
+            arrays["field" if name == "oxygen" else f"field_{name}"] = species.numpy()
+        arrays["density"] = fields["oxygen"].density.cpu().numpy()
         arrays["grid_origin"] = np.array(oxygen.field.geometry.origin)
         arrays["grid_dx"] = np.array(oxygen.field.geometry.dx)
     arrays["neighbor_count"] = _host(population.neighbor_count, n, np.int32).copy()
@@ -85,10 +87,13 @@ def restore_population(data: dict[str, np.ndarray], device=None) -> CellPopulati
 
 
 def restore_field(data: dict[str, np.ndarray], oxygen) -> None:
-    """Put the saved field (the next solve's warm start) into ``oxygen``; its grid must match."""
+    """Put the saved field(s) (the next solves' warm starts) into ``oxygen`` (an OxygenField or a
+    MetabolicFields: every species by its ``field_<name>`` key); the grid must match."""
     geom = oxygen.field.geometry
     if tuple(data["field"].shape) != tuple(geom.shape) or not np.allclose(data["grid_origin"], geom.origin) \
             or float(data["grid_dx"]) != float(geom.dx):
         raise ValueError("the checkpoint's grid does not match the field's")
-    oxygen.field.assign(data["field"])
+    fields = getattr(oxygen, "species_fields", None) or {"oxygen": oxygen}
+    for name, species in fields.items():
+        species.field.assign(data["field" if name == "oxygen" else f"field_{name}"])
     torch.cuda.synchronize(oxygen.device)
