@@ -166,19 +166,51 @@ int wbc_hashgrid_point_ids(const void* workspace, size_t workspace_bytes, int32_
                            const int32_t** point_id, const int32_t** bucket_start);
 
 /* ---- contact mechanics ---------------------------------------------------------------------
- * Replaces the contact_velocities launch, cells/mechanics.py:149-161 (kernel
+ * Replaces the contact_velocities launch, cells/mechanics.py:82-93 (kernel
  * kernels/mechanics_kernels.py:26-57), over the last build. */
 int wbc_contact_velocities(const void* workspace, size_t workspace_bytes, int32_t dim, int64_t capacity, int32_t n,
                            float cell_size, float query_radius, float rate, float* velocity,
                            int32_t* neighbor_count, wbc_stream_t stream);
-/* Replaces integrate_positions, cells/mechanics.py:171-176 (kernels/mechanics_kernels.py:60-67). */
+/* Replaces integrate_positions, cells/mechanics.py:104-109 (kernels/mechanics_kernels.py:60-67). */
 int wbc_integrate_positions(float* position, const float* velocity, int64_t n, float dt, wbc_stream_t stream);
-/* Replaces relax_contacts, cells/mechanics.py:179-189 without a region: `substeps` x (build,
+/* Replaces relax_contacts, cells/mechanics.py:112-122 without a region: `substeps` x (build,
  * contact gather fused with x += v dt). */
 int wbc_relax_contacts(float* position, const float* radius, int32_t n, float cell_size, int32_t dim,
                        float query_radius, float rate, float dt, int32_t substeps, float* velocity,
                        int32_t* neighbor_count, int64_t capacity, void* workspace, size_t workspace_bytes,
                        wbc_stream_t stream);
+
+/* ---- tissue region --------------------------------------------------------------------------
+ * The device half of TissueRegion (geometry/region.py:24-34): the signed-distance field
+ * (negative inside; nx*ny*nz floats, z fastest) and its node gradient (3 floats per node, the
+ * same node order) on the grid with node (0,0,0) at `origin` and spacing 1 / inv_dx [um].
+ * The kernels pass 1 / geometry.dx rounded to float32 as inv_dx. */
+typedef struct wbc_region_t {
+    const float* sdf;
+    const float* gradient;
+    int32_t nx, ny, nz;
+    float origin[3];
+    float inv_dx;
+} wbc_region_t;
+/* Replaces node_gradient, geometry/region.py:18-21 (numpy.gradient in float64 with edge_order 1,
+ * rounded to float32), run once when a region is built: gradient[3 * node + axis]. */
+int wbc_region_gradient(const float* sdf, int32_t nx, int32_t ny, int32_t nz, double dx, float* gradient,
+                        wbc_stream_t stream);
+/* Replaces the confinement_velocities launch, cells/mechanics.py:95-103 (kernel
+ * kernels/geometry_kernels.py:59-79): velocity[c] -= wall_rate * (sdf + r) / |grad| * grad, at the
+ * trilinear SDF and gradient of cell c, where the cell surface pokes out (sdf + r > 0). */
+int wbc_confinement_velocities(const wbc_region_t* region, float wall_rate, const float* position,
+                               const float* radius, int64_t n, float* velocity, wbc_stream_t stream);
+/* Replaces relax_contacts with a region, cells/mechanics.py:112-122: `substeps` x (build, contact
+ * gather + the confinement push of the cell's start-of-substep position + x += v dt, one kernel). */
+int wbc_relax_contacts_confined(float* position, const float* radius, int32_t n, float cell_size, int32_t dim,
+                                float query_radius, float rate, float dt, int32_t substeps,
+                                const wbc_region_t* region, float wall_rate, float* velocity,
+                                int32_t* neighbor_count, int64_t capacity, void* workspace,
+                                size_t workspace_bytes, wbc_stream_t stream);
+/* Replaces TissueRegion.sample_at, geometry/region.py:75-78 (kernel sample_sdf,
+ * kernels/geometry_kernels.py:46-56): the trilinear SDF at n positions, clamped to the grid. */
+int wbc_sample_sdf(const wbc_region_t* region, const float* position, int64_t n, float* out, wbc_stream_t stream);
 
 /* ---- adjoint of the steady-state solve -----------------------------------------------------
  * No reference counterpart (the reference does not differentiate its solve). Per-cell rates are

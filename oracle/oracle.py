@@ -63,6 +63,7 @@ def lib() -> ctypes.CDLL:
             "orc_contact_velocities": (None, [p, p, i64, f32, i32, p, p, f32, f32, p, p]),
             "orc_integrate_positions": (None, [p, p, i64, f32]),
             "orc_contact_substep": (None, [p, p, i64, f32, i32, f32, f32, f32, p, p, p, p]),
+            "orc_confinement_velocities": (None, [p, p, i32, i32, i32, f32, f32, f32, f32, f32, p, p, i64, p]),
             "orc_scan_inclusive_i32": (None, [p, p, i64]),
             "orc_lifecycle_decide": (None, [f32, f32, f32, f32, i32, f32, f32, f32, f32, i32, i32, f32, f32,
                                             p, p, p, p, p, p, p, p, i64]),
@@ -270,15 +271,71 @@ def contact_velocities(positions, radii, cell_size: float, dim: int, query_radiu
     return vel, cnt
 
 
+def node_gradient(values: np.ndarray, dx: float) -> np.ndarray:
+    """geometry/region.py:18-21: numpy's second-order central differences inside, first-order
+    one-sided at the faces, in float64, rounded to float32; shape (nx, ny, nz, 3)."""
+    grads = np.gradient(np.asarray(values, dtype=np.float64), dx, edge_order=1)
+    return np.ascontiguousarray(np.stack(grads, axis=-1).astype(np.float32))
+
+
+@dataclass
+class OracleRegion:
+    """The device half of TissueRegion (geometry/region.py:24-34) on numpy arrays."""
+
+    sdf: np.ndarray
+    gradient: np.ndarray
+    origin: tuple
+    dx: float
+
+    @classmethod
+    def from_sdf(cls, sdf: np.ndarray, origin, dx: float) -> "OracleRegion":
+        sdf = np.ascontiguousarray(sdf, dtype=np.float32)
+        return cls(sdf, node_gradient(sdf, dx), tuple(float(o) for o in origin), float(dx))
+
+    def args(self):
+        o = self.origin
+        return (_p(self.sdf), _p(self.gradient), *self.sdf.shape, o[0], o[1], o[2], 1.0 / self.dx)
+
+
+def confinement_velocities(region: OracleRegion, wall_rate: float, positions, radii, velocity: np.ndarray) -> None:
+    """kernels/geometry_kernels.py:59-79, in place on ``velocity`` (n, 3) float32."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32)
+    rad = np.ascontiguousarray(radii, dtype=np.float32)
+    assert velocity.dtype == np.float32 and velocity.flags.c_contiguous and velocity.shape == pos.shape
+    lib().orc_confinement_velocities(*region.args(), wall_rate, _p(pos), _p(rad), len(pos), _p(velocity))
+
+
+def sample_sdf(region: OracleRegion, positions) -> np.ndarray:
+    """kernels/geometry_kernels.py:46-56: the same clamp and lerp order as the field sampler
+    (kernels/field_kernels.py:153-182), so the oracle's trilinear sampler restates both."""
+    pos = np.ascontiguousarray(positions, dtype=np.float32)
+    out = np.zeros(len(pos), dtype=np.float32)
+    o = region.origin
+    lib().orc_sample_trilinear(_p(region.sdf), *region.sdf.shape, _p(pos), len(pos), o[0], o[1], o[2],
+                               1.0 / region.dx, _p(out))
+    return out
+
+
 def relax_contacts(positions: np.ndarray, radii: np.ndarray, cell_size: float, dim: int, query_radius: float,
-                   rate: float, dt: float, substeps: int, velocity: np.ndarray, counts: np.ndarray) -> None:
-    """In place on ``positions`` (n, 3) float32."""
+                   rate: float, dt: float, substeps: int, velocity: np.ndarray, counts: np.ndarray,
+                   region: OracleRegion | None = None, wall_rate: float = 0.0) -> None:
+    """In place on ``positions`` (n, 3) float32 (cells/mechanics.py:75-122): per substep build,
+    contact gather, the confinement push when a region is given, x += v dt."""
     n = len(positions)
     start = np.zeros(dim ** 3 + 1, dtype=np.int32)
     ids = np.zeros(max(n, 1), dtype=np.int32)
     for _ in range(substeps):
-        lib().orc_contact_substep(_p(positions), _p(radii), n, cell_size, dim, query_radius, rate, dt, _p(velocity),
-                                  _p(counts), _p(start), _p(ids))
+        if region is None:
+            lib().orc_contact_substep(_p(positions), _p(radii), n, cell_size, dim, query_radius, rate, dt,
+                                      _p(velocity), _p(counts), _p(start), _p(ids))
+            continue
+        if n == 0:
+            return
+        lib().orc_hashgrid_build(_p(positions), n, cell_size, dim, _p(start), _p(ids))
+        lib().orc_contact_velocities(_p(positions), _p(radii), n, cell_size, dim, _p(start), _p(ids), query_radius,
+                                     rate, _p(velocity), _p(counts))
+        lib().orc_confinement_velocities(*region.args(), wall_rate, _p(positions), _p(radii), n, _p(velocity))
+        lib().orc_integrate_positions(_p(positions), _p(velocity), n, dt)
 
 
 def scan_inclusive(flags) -> np.ndarray:

@@ -495,7 +495,51 @@ void orc_integrate_positions(float* position, const float* velocity, int64_t n, 
     for (int64_t i = 0; i < 3 * n; ++i) position[i] = position[i] + velocity[i] * dt;
 }
 
-/* One contact substep (cells/mechanics.py:142-176 without a region); scratch is caller-owned. */
+/* confinement_velocities (kernels/geometry_kernels.py:59-79): the tissue SDF (nx, ny, nz) and its node
+ * gradient (nx, ny, nz, 3) sampled trilinearly at each cell (trilinear_cell :19-28, positions clamped to
+ * the grid; sample_scalar / sample_vector :31-46, x then y then z lerps); where the cell surface pokes
+ * out (p = sdf + r > 0) and the gradient is not degenerate, v -= (wall_rate * p / |g|) * g. */
+void orc_confinement_velocities(const float* sdf, const float* gradient, int32_t nx, int32_t ny, int32_t nz, float ox,
+                                float oy, float oz, float inv_dx, float wall_rate, const float* position,
+                                const float* radius, int64_t n, float* velocity)
+{
+    const int64_t plane = (int64_t)ny * nz;
+#pragma omp parallel for schedule(static)
+    for (int64_t c = 0; c < n; ++c) {
+        float ux = (position[3 * c + 0] - ox) * inv_dx;
+        float uy = (position[3 * c + 1] - oy) * inv_dx;
+        float uz = (position[3 * c + 2] - oz) * inv_dx;
+        ux = fminf(fmaxf(ux, 0.0f), (float)(nx - 1));
+        uy = fminf(fmaxf(uy, 0.0f), (float)(ny - 1));
+        uz = fminf(fmaxf(uz, 0.0f), (float)(nz - 1));
+        int32_t i0 = (int32_t)floorf(ux), j0 = (int32_t)floorf(uy), k0 = (int32_t)floorf(uz);
+        if (i0 > nx - 2) i0 = nx - 2;
+        if (j0 > ny - 2) j0 = ny - 2;
+        if (k0 > nz - 2) k0 = nz - 2;
+        const float fx = ux - (float)i0, fy = uy - (float)j0, fz = uz - (float)k0;
+        float s[4];  /* component 0..2 of the gradient, 3: the SDF */
+        for (int comp = 0; comp < 4; ++comp) {
+#define V(i, j, k) (comp == 3 ? sdf[(int64_t)(i) * plane + (int64_t)(j) * nz + (k)] \
+                              : gradient[((int64_t)(i) * plane + (int64_t)(j) * nz + (k)) * 3 + comp])
+            float c00 = V(i0, j0, k0) * (1.0f - fx) + V(i0 + 1, j0, k0) * fx;
+            float c10 = V(i0, j0 + 1, k0) * (1.0f - fx) + V(i0 + 1, j0 + 1, k0) * fx;
+            float c01 = V(i0, j0, k0 + 1) * (1.0f - fx) + V(i0 + 1, j0, k0 + 1) * fx;
+            float c11 = V(i0, j0 + 1, k0 + 1) * (1.0f - fx) + V(i0 + 1, j0 + 1, k0 + 1) * fx;
+#undef V
+            s[comp] = (c00 * (1.0f - fy) + c10 * fy) * (1.0f - fz) + (c01 * (1.0f - fy) + c11 * fy) * fz;
+        }
+        float penetration = s[3] + radius[c];
+        if (penetration <= 0.0f) continue;
+        float length = sqrtf(s[0] * s[0] + s[1] * s[1] + s[2] * s[2]);
+        if (length < 1.0e-8f) continue;
+        float coef = wall_rate * penetration / length;
+        velocity[3 * c + 0] = velocity[3 * c + 0] - coef * s[0];
+        velocity[3 * c + 1] = velocity[3 * c + 1] - coef * s[1];
+        velocity[3 * c + 2] = velocity[3 * c + 2] - coef * s[2];
+    }
+}
+
+/* One contact substep (cells/mechanics.py:75-109 without a region); scratch is caller-owned. */
 void orc_contact_substep(float* position, const float* radius, int64_t n, float cell_size, int32_t dim,
                          float query_radius, float rate, float dt, float* velocity, int32_t* neighbor_count,
                          int32_t* bucket_start, int32_t* point_id)

@@ -42,12 +42,15 @@ def dirichlet_mask(shape, boundary):
     return mask
 
 
-def steady_state_dense(values0, density, D, q, K, dx, boundary, picard=40, tol=1e-12):
-    """Discrete steady state by a dense linear solve inside a Picard loop on the uptake."""
+def steady_state_dense(values0, density, D, q, K, dx, boundary, picard=40, tol=1e-12, fixed_extra=None):
+    """Discrete steady state by a dense linear solve inside a Picard loop on the uptake; nodes of
+    ``fixed_extra`` (and the Dirichlet faces) keep their ``values0``."""
     v = np.asarray(values0, dtype=np.float64).copy()
     dens = np.asarray(density, dtype=np.float64)
     shape = v.shape
     fixed = dirichlet_mask(shape, boundary)
+    if fixed_extra is not None:
+        fixed = fixed | np.asarray(fixed_extra, dtype=bool)
     unknown = np.argwhere(~fixed)
     index = -np.ones(shape, dtype=int)
     for row, (i, j, k) in enumerate(unknown):

@@ -8,7 +8,7 @@ library or a ROCm GPU is missing.
 from __future__ import annotations
 
 import ctypes
-from ctypes import c_float, c_int32, c_int64, c_size_t, c_void_p
+from ctypes import c_double, c_float, c_int32, c_int64, c_size_t, c_void_p
 from pathlib import Path
 
 import torch
@@ -56,6 +56,11 @@ SIGNATURES: dict[str, tuple] = {
     "wbc_contact_velocities": (I32, [P, c_size_t, I32, I64, I32, F32, F32, F32, P, P, P]),
     "wbc_integrate_positions": (I32, [P, P, I64, F32, P]),
     "wbc_relax_contacts": (I32, [P, P, I32, F32, I32, F32, F32, F32, I32, P, P, I64, P, c_size_t, P]),
+    "wbc_confinement_velocities": (I32, [P, F32, P, P, I64, P, P]),
+    "wbc_relax_contacts_confined": (I32, [P, P, I32, F32, I32, F32, F32, F32, I32, P, F32, P, P, I64, P, c_size_t,
+                                          P]),
+    "wbc_sample_sdf": (I32, [P, P, I64, P, P]),
+    "wbc_region_gradient": (I32, [P, I32, I32, I32, c_double, P, P]),
     "wbc_deposit_rates_i64": (I32, [P, P, I64, F32, F32, F32, F32, F32, I32, I32, I32, P, P]),
     "wbc_fixed_to_float": (I32, [P, I64, F32, P, P]),
     "wbc_sample_adjoint": (I32, [P, P, I64, F32, F32, F32, F32, I32, I32, I32, P, P]),

@@ -83,6 +83,83 @@ __device__ __forceinline__ float sample_cell(const float* __restrict__ field, in
     return c0 * (1.0f - fz) + c1 * fz;
 }
 
+// ---- tissue region ----------------------------------------------------------------------------
+// A signed-distance field (negative inside) and its node gradient on a regular grid: nx*ny*nz
+// floats (z fastest) and 3 floats per node.
+struct RegionDev {
+    const float* sdf;
+    const float* gradient;
+    int32_t nx, ny, nz;
+    float ox, oy, oz, inv_dx;
+};
+
+// Trilinear weights of a grid-space position u = (x - origin) * inv_dx, clamped to the grid
+// (kernels/geometry_kernels.py:16-25): the 8 corner offsets in x, y, z lerp order and the fractions.
+struct RegionCorners {
+    int64_t e[8];
+    float fx, fy, fz;
+};
+
+__device__ __forceinline__ RegionCorners region_corners(const RegionDev& g, float px, float py, float pz)
+{
+    float ux = (px - g.ox) * g.inv_dx;
+    float uy = (py - g.oy) * g.inv_dx;
+    float uz = (pz - g.oz) * g.inv_dx;
+    ux = fminf(fmaxf(ux, 0.0f), (float)(g.nx - 1));
+    uy = fminf(fmaxf(uy, 0.0f), (float)(g.ny - 1));
+    uz = fminf(fmaxf(uz, 0.0f), (float)(g.nz - 1));
+    const int32_t i0 = min((int32_t)floorf(ux), g.nx - 2);
+    const int32_t j0 = min((int32_t)floorf(uy), g.ny - 2);
+    const int32_t k0 = min((int32_t)floorf(uz), g.nz - 2);
+    RegionCorners w;
+    w.fx = ux - (float)i0;
+    w.fy = uy - (float)j0;
+    w.fz = uz - (float)k0;
+    const int64_t plane = (int64_t)g.ny * g.nz, row = g.nz;
+    const int64_t base = i0 * plane + j0 * row + k0;
+    w.e[0] = base;
+    w.e[1] = base + plane;
+    w.e[2] = base + row;
+    w.e[3] = base + plane + row;
+    w.e[4] = base + 1;
+    w.e[5] = base + plane + 1;
+    w.e[6] = base + row + 1;
+    w.e[7] = base + plane + row + 1;
+    return w;
+}
+
+// sample_scalar / sample_vector (kernels/geometry_kernels.py:28-43): x lerps, then y, then z, one
+// component (`comp` of `stride` floats per node) at a time.
+__device__ __forceinline__ float region_lerp(const float* __restrict__ f, int stride, int comp, const RegionCorners& w)
+{
+    const float fx = w.fx, fy = w.fy, fz = w.fz;
+    const float c00 = f[w.e[0] * stride + comp] * (1.0f - fx) + f[w.e[1] * stride + comp] * fx;
+    const float c10 = f[w.e[2] * stride + comp] * (1.0f - fx) + f[w.e[3] * stride + comp] * fx;
+    const float c01 = f[w.e[4] * stride + comp] * (1.0f - fx) + f[w.e[5] * stride + comp] * fx;
+    const float c11 = f[w.e[6] * stride + comp] * (1.0f - fx) + f[w.e[7] * stride + comp] * fx;
+    return (c00 * (1.0f - fy) + c10 * fy) * (1.0f - fz) + (c01 * (1.0f - fy) + c11 * fy) * fz;
+}
+
+// confinement_velocities for one cell at (px, py, pz) of radius r (kernels/geometry_kernels.py:59-79):
+// where the cell surface pokes out of the tissue (p = sdf + r > 0) and the sampled gradient is not
+// degenerate (|g| >= 1e-8), v -= (wall_rate * p / |g|) * g component by component.
+__device__ __forceinline__ void confine(const RegionDev& g, float wall_rate, float px, float py, float pz, float r,
+                                        float& vx, float& vy, float& vz)
+{
+    const RegionCorners w = region_corners(g, px, py, pz);
+    const float penetration = region_lerp(g.sdf, 1, 0, w) + r;
+    if (penetration <= 0.0f) return;
+    const float gx = region_lerp(g.gradient, 3, 0, w);
+    const float gy = region_lerp(g.gradient, 3, 1, w);
+    const float gz = region_lerp(g.gradient, 3, 2, w);
+    const float length = sqrtf(gx * gx + gy * gy + gz * gz);
+    if (length < 1.0e-8f) return;
+    const float coef = wall_rate * penetration / length;
+    vx = vx - coef * gx;
+    vy = vy - coef * gy;
+    vz = vz - coef * gz;
+}
+
 // ---- hash grid --------------------------------------------------------------------------------
 __device__ __forceinline__ int32_t wrap_dim(int32_t c, int32_t dim)
 {

@@ -1,8 +1,9 @@
 // Hash-grid neighbour build and overdamped contact mechanics.
 //
 // Replaces spatial/neighbors.py:33-37 (wp.HashGrid.build over the active prefix),
-// kernels/mechanics_kernels.py:26-57 (contact_velocities), :60-67 (integrate_positions) and the
-// substep driver cells/mechanics.py:142-189.
+// kernels/mechanics_kernels.py:26-57 (contact_velocities), :60-67 (integrate_positions),
+// kernels/geometry_kernels.py:46-79 (sample_sdf, confinement_velocities) and the substep driver
+// cells/mechanics.py:75-122.
 //
 // Build = stable counting sort of the active slots by bucket:
 //   1. key per point, atomic count per bucket;
@@ -15,6 +16,7 @@
 // The bucket of p is floor(p / cell_size) per axis, each reduced modulo dim, linearised with z
 // fastest; a query walks x, then y, then z buckets, and each bucket in slot order. That fixes
 // the FP32 summation order of the gather, so the result does not depend on thread timing.
+#include "wbc.h"
 #include "wbc_scan.hpp"
 #include "wbc_cellops.hpp"
 
@@ -80,13 +82,16 @@ __global__ void __launch_bounds__(256) hg_set_total(int32_t* __restrict__ start,
 }
 
 // Gather of cell t (bucket order): neighbour count within query_radius and the repulsion
-// velocity; optionally x += v * dt into the slot-ordered position array (safe because the
-// gather reads only the bucket-ordered copy).
-template <bool kIntegrate>
+// velocity; with kConfine the tissue-wall push of the cell's start-of-substep position is added
+// (the reference's separate confinement launch reads the same position and radius, and the
+// velocity the gather has just written); with kIntegrate x += v * dt into the slot-ordered
+// position array (safe because the gather reads only the bucket-ordered copy).
+template <bool kIntegrate, bool kConfine>
 __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict__ start,
                                                        const int32_t* __restrict__ id,
                                                        const float4* __restrict__ xyzr, int32_t n, float inv_cell,
                                                        int32_t dim, float query_radius, float rate, float dt,
+                                                       RegionDev region, float wall_rate,
                                                        float* __restrict__ velocity,
                                                        int32_t* __restrict__ neighbor_count,
                                                        float* __restrict__ position)
@@ -98,6 +103,7 @@ __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict_
     int32_t count;
     contact_gather(start, xyzr, t, inv_cell, dim, query_radius, rate, vx, vy, vz, count);
     const float4 P = xyzr[t];
+    if (kConfine) confine(region, wall_rate, P.x, P.y, P.z, P.w, vx, vy, vz);
     velocity[3 * i + 0] = vx;
     velocity[3 * i + 1] = vy;
     velocity[3 * i + 2] = vz;
@@ -106,6 +112,56 @@ __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict_
         position[3 * i + 0] = P.x + vx * dt;
         position[3 * i + 1] = P.y + vy * dt;
         position[3 * i + 2] = P.z + vz * dt;
+    }
+}
+
+__global__ void __launch_bounds__(256) confinement_kernel(RegionDev region, float wall_rate,
+                                                           const float* __restrict__ position,
+                                                           const float* __restrict__ radius, int64_t n,
+                                                           float* __restrict__ velocity)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    float vx = velocity[3 * c + 0], vy = velocity[3 * c + 1], vz = velocity[3 * c + 2];
+    confine(region, wall_rate, position[3 * c + 0], position[3 * c + 1], position[3 * c + 2], radius[c], vx, vy, vz);
+    velocity[3 * c + 0] = vx;
+    velocity[3 * c + 1] = vy;
+    velocity[3 * c + 2] = vz;
+}
+
+__global__ void __launch_bounds__(256) sample_sdf_kernel(RegionDev region, const float* __restrict__ position,
+                                                          int64_t n, float* __restrict__ out)
+{
+    const int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (c >= n) return;
+    const RegionCorners w = region_corners(region, position[3 * c + 0], position[3 * c + 1], position[3 * c + 2]);
+    out[c] = region_lerp(region.sdf, 1, 0, w);
+}
+
+// node_gradient (geometry/region.py:18-21), numpy.gradient(sdf as float64, dx, edge_order=1) per
+// axis: (f[+1] - f[-1]) / (2 dx) inside, (f[1] - f[0]) / dx and (f[-1] - f[-2]) / dx on the faces,
+// in double (correctly rounded subtraction and division, as numpy's), rounded once to float.
+__global__ void __launch_bounds__(256) region_gradient_kernel(const float* __restrict__ sdf, int32_t nx, int32_t ny,
+                                                               int32_t nz, double dx, float* __restrict__ gradient)
+{
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t plane = (int64_t)ny * nz;
+    if (t >= (int64_t)nx * plane) return;
+    const int32_t i = (int32_t)(t / plane);
+    const int32_t j = (int32_t)((t - (int64_t)i * plane) / nz);
+    const int32_t k = (int32_t)(t - (int64_t)i * plane - (int64_t)j * nz);
+    const int32_t idx[3] = {i, j, k}, len[3] = {nx, ny, nz};
+    const int64_t step[3] = {plane, nz, 1};
+    const double two_dx = 2.0 * dx;
+    for (int a = 0; a < 3; ++a) {
+        double g;
+        if (idx[a] == 0)
+            g = ((double)sdf[t + step[a]] - (double)sdf[t]) / dx;
+        else if (idx[a] == len[a] - 1)
+            g = ((double)sdf[t] - (double)sdf[t - step[a]]) / dx;
+        else
+            g = ((double)sdf[t + step[a]] - (double)sdf[t - step[a]]) / two_dx;
+        gradient[3 * t + a] = (float)g;
     }
 }
 
@@ -192,11 +248,35 @@ WBC_API int wbc_hashgrid_point_ids(const void* workspace, size_t workspace_bytes
     return 0;
 }
 
+// The region view of a wbc_region_t (null region: confinement off).
+static bool region_of(const wbc_region_t* r, RegionDev& g)
+{
+    if (!r) return false;
+    g.sdf = r->sdf;
+    g.gradient = r->gradient;
+    g.nx = r->nx;
+    g.ny = r->ny;
+    g.nz = r->nz;
+    g.ox = r->origin[0];
+    g.oy = r->origin[1];
+    g.oz = r->origin[2];
+    g.inv_dx = r->inv_dx;
+    return true;
+}
+
+static bool region_ok(const wbc_region_t* r)
+{
+    return r && r->sdf && r->gradient && r->nx >= 2 && r->ny >= 2 && r->nz >= 2 && r->inv_dx > 0.0f &&
+           (int64_t)r->nx * r->ny * r->nz * 3 < ((int64_t)1 << 40);
+}
+
 static int launch_contact(bool integrate, const void* workspace, size_t workspace_bytes, int32_t dim,
                           int64_t capacity, int32_t n, float cell_size, float query_radius, float rate, float dt,
-                          float* velocity, int32_t* neighbor_count, float* position, hipStream_t stream)
+                          const wbc_region_t* region, float wall_rate, float* velocity, int32_t* neighbor_count,
+                          float* position, hipStream_t stream)
 {
     if (!hg_args_ok(dim, capacity) || n < 0 || n > capacity || !(cell_size > 0.0f)) return (int)hipErrorInvalidValue;
+    if (region && !region_ok(region)) return (int)hipErrorInvalidValue;
     if (n == 0) return 0;
     if (!workspace || !velocity || !neighbor_count || (integrate && !position)) return (int)hipErrorInvalidValue;
     HashGridLayout L = hashgrid_layout(dim, capacity);
@@ -206,12 +286,16 @@ static int launch_contact(bool integrate, const void* workspace, size_t workspac
     const int32_t* id = (const int32_t*)(ws + L.id);
     const float4* xyzr = (const float4*)(ws + L.xyzr);
     float inv_cell = 1.0f / cell_size;
+    RegionDev g{};
+    const bool confined = region_of(region, g);
+    auto go = [&](auto kernel) {
+        kernel<<<grid_for(n, 256), 256, 0, stream>>>(start, id, xyzr, n, inv_cell, dim, query_radius, rate, dt, g,
+                                                     wall_rate, velocity, neighbor_count, position);
+    };
     if (integrate)
-        contact_kernel<true><<<grid_for(n, 256), 256, 0, stream>>>(start, id, xyzr, n, inv_cell, dim, query_radius,
-                                                                   rate, dt, velocity, neighbor_count, position);
+        confined ? go(contact_kernel<true, true>) : go(contact_kernel<true, false>);
     else
-        contact_kernel<false><<<grid_for(n, 256), 256, 0, stream>>>(start, id, xyzr, n, inv_cell, dim, query_radius,
-                                                                    rate, dt, velocity, neighbor_count, position);
+        confined ? go(contact_kernel<false, true>) : go(contact_kernel<false, false>);
     WBC_CHECK_LAUNCH();
     return 0;
 }
@@ -221,7 +305,7 @@ WBC_API int wbc_contact_velocities(const void* workspace, size_t workspace_bytes
                                    int32_t* neighbor_count, hipStream_t stream)
 {
     return launch_contact(false, workspace, workspace_bytes, dim, capacity, n, cell_size, query_radius, rate, 0.0f,
-                          velocity, neighbor_count, nullptr, stream);
+                          nullptr, 0.0f, velocity, neighbor_count, nullptr, stream);
 }
 
 WBC_API int wbc_integrate_positions(float* position, const float* velocity, int64_t n, float dt, hipStream_t stream)
@@ -234,19 +318,77 @@ WBC_API int wbc_integrate_positions(float* position, const float* velocity, int6
     return 0;
 }
 
-// `substeps` x (build -> contact gather -> x += v dt), cells/mechanics.py:142-189 without a region.
-WBC_API int wbc_relax_contacts(float* position, const float* radius, int32_t n, float cell_size, int32_t dim,
-                               float query_radius, float rate, float dt, int32_t substeps, float* velocity,
-                               int32_t* neighbor_count, int64_t capacity, void* workspace, size_t workspace_bytes,
-                               hipStream_t stream)
+static int relax(float* position, const float* radius, int32_t n, float cell_size, int32_t dim, float query_radius,
+                 float rate, float dt, int32_t substeps, const wbc_region_t* region, float wall_rate, float* velocity,
+                 int32_t* neighbor_count, int64_t capacity, void* workspace, size_t workspace_bytes,
+                 hipStream_t stream)
 {
     if (substeps < 0) return (int)hipErrorInvalidValue;
     for (int32_t s = 0; s < substeps; ++s) {
         int rc = wbc_hashgrid_build(position, radius, n, cell_size, dim, capacity, workspace, workspace_bytes, stream);
         if (rc) return rc;
         rc = launch_contact(true, workspace, workspace_bytes, dim, capacity, n, cell_size, query_radius, rate, dt,
-                            velocity, neighbor_count, position, stream);
+                            region, wall_rate, velocity, neighbor_count, position, stream);
         if (rc) return rc;
     }
+    return 0;
+}
+
+// `substeps` x (build -> contact gather -> x += v dt), cells/mechanics.py:112-122 without a region.
+WBC_API int wbc_relax_contacts(float* position, const float* radius, int32_t n, float cell_size, int32_t dim,
+                               float query_radius, float rate, float dt, int32_t substeps, float* velocity,
+                               int32_t* neighbor_count, int64_t capacity, void* workspace, size_t workspace_bytes,
+                               hipStream_t stream)
+{
+    return relax(position, radius, n, cell_size, dim, query_radius, rate, dt, substeps, nullptr, 0.0f, velocity,
+                 neighbor_count, capacity, workspace, workspace_bytes, stream);
+}
+
+// The same with the tissue-wall push of `region` added inside the gather (cells/mechanics.py:95-103).
+WBC_API int wbc_relax_contacts_confined(float* position, const float* radius, int32_t n, float cell_size, int32_t dim,
+                                        float query_radius, float rate, float dt, int32_t substeps,
+                                        const wbc_region_t* region, float wall_rate, float* velocity,
+                                        int32_t* neighbor_count, int64_t capacity, void* workspace,
+                                        size_t workspace_bytes, hipStream_t stream)
+{
+    if (!region_ok(region)) return (int)hipErrorInvalidValue;
+    return relax(position, radius, n, cell_size, dim, query_radius, rate, dt, substeps, region, wall_rate, velocity,
+                 neighbor_count, capacity, workspace, workspace_bytes, stream);
+}
+
+WBC_API int wbc_confinement_velocities(const wbc_region_t* region, float wall_rate, const float* position,
+                                       const float* radius, int64_t n, float* velocity, hipStream_t stream)
+{
+    if (n < 0 || n > INT32_MAX || !region_ok(region)) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!position || !radius || !velocity) return (int)hipErrorInvalidValue;
+    RegionDev g{};
+    region_of(region, g);
+    confinement_kernel<<<grid_for(n, 256), 256, 0, stream>>>(g, wall_rate, position, radius, n, velocity);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+WBC_API int wbc_sample_sdf(const wbc_region_t* region, const float* position, int64_t n, float* out,
+                           hipStream_t stream)
+{
+    if (n < 0 || n > INT32_MAX || !region_ok(region)) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!position || !out) return (int)hipErrorInvalidValue;
+    RegionDev g{};
+    region_of(region, g);
+    sample_sdf_kernel<<<grid_for(n, 256), 256, 0, stream>>>(g, position, n, out);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+WBC_API int wbc_region_gradient(const float* sdf, int32_t nx, int32_t ny, int32_t nz, double dx, float* gradient,
+                                hipStream_t stream)
+{
+    if (nx < 2 || ny < 2 || nz < 2 || !(dx > 0.0) || !sdf || !gradient) return (int)hipErrorInvalidValue;
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    if (nodes * 3 >= ((int64_t)1 << 40)) return (int)hipErrorInvalidValue;
+    region_gradient_kernel<<<grid_for(nodes, 256), 256, 0, stream>>>(sdf, nx, ny, nz, dx, gradient);
+    WBC_CHECK_LAUNCH();
     return 0;
 }

@@ -2,7 +2,8 @@
 
 ``save_checkpoint`` writes the same keys, shapes and dtypes as the reference: every persistent
 cell array of the live cells, ``count``, ``capacity``, ``seed``, ``time_h``, extra per-cell fields
-as ``<name>_local``, the field, the consuming density, ``grid_origin`` and ``grid_dx``, so the
+as ``<name>_local``, the field, the consuming density, ``grid_origin`` and ``grid_dx``, the
+tissue SDF as ``tissue_sdf`` when a region is given, so the
 reference's loaders and post-processing (``load_checkpoint``, io/export.py) read it unchanged.
 One key is added: ``neighbor_count``, the crowding of the last mechanics substep, which the next
 step's lifecycle decisions read (simulation/simulator.py:16-18); with it a run restored by
@@ -34,8 +35,8 @@ def _host(t: torch.Tensor, n: int, dtype) -> np.ndarray:
 def save_checkpoint(path: str | Path, population: CellPopulation, oxygen, time_h: float, region=None,
                     network=None) -> Path:
     """Write ``path`` (compressed ``.npz``) in the reference's layout; returns the path."""
-    if region is not None or network is not None:
-        raise NotImplementedError("tissue regions and gene networks are not part of this build")
+    if network is not None:
+        raise NotImplementedError("gene networks are not part of this build")
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
     n = population.count
@@ -53,6 +54,10 @@ def save_checkpoint(path: str | Path, population: CellPopulation, oxygen, time_h
         arrays["density"] = fields["oxygen"].density.cpu().numpy()
         arrays["grid_origin"] = np.array(oxygen.field.geometry.origin)
         arrays["grid_dx"] = np.array(oxygen.field.geometry.dx)
+    if region is not None:
+        arrays["tissue_sdf"] = region.sdf_numpy()
+        arrays["grid_origin"] = np.array(region.geometry.origin)
+        arrays["grid_dx"] = np.array(region.geometry.dx)
     arrays["neighbor_count"] = _host(population.neighbor_count, n, np.int32).copy()
     np.savez_compressed(path, **arrays)
     return path
@@ -97,3 +102,17 @@ def restore_field(data: dict[str, np.ndarray], oxygen) -> None:
     for name, species in fields.items():
         species.field.assign(data["field" if name == "oxygen" else f"field_{name}"])
     torch.cuda.synchronize(oxygen.device)
+
+
+def restore_region(data: dict[str, np.ndarray], device=None, name: str = "region"):
+    """The TissueRegion a checkpoint saved (``tissue_sdf`` on ``grid_origin`` / ``grid_dx``), or
+    None when it holds none."""
+    if "tissue_sdf" not in data:
+        return None
+    from ..fields.scalar_field import GridGeometry
+    from ..geometry.region import TissueRegion
+
+    sdf = np.asarray(data["tissue_sdf"], dtype=np.float32)
+    geom = GridGeometry(origin=tuple(float(o) for o in data["grid_origin"]), dx=float(data["grid_dx"]),
+                        shape=tuple(int(d) for d in sdf.shape))
+    return TissueRegion(geom, sdf, device=device, name=name)
