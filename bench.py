@@ -159,6 +159,45 @@ def adjoint_measurement(device) -> dict:
             "adjoint_over_forward": round((t2 - t1) / (t1 - t0), 3)}
 
 
+def confinement_measurement(device, repeats: int = 25) -> dict:
+    """The contact pass of the 100k-cell spheroid with and without a tissue region: a sphere just
+    inside the spheroid's outer cells, on the oxygen grid, so the wall pushes the outer shell.
+    Each call runs the step's mechanics (4 substeps of build -> gather (+ push) -> move)."""
+    import numpy as np
+    import torch
+
+    from warpbiocell_amd import workloads as W
+    from warpbiocell_amd.cells.mechanics import ContactParams, make_neighbor_grid, relax_contacts
+    from warpbiocell_amd.cells.state import CellPopulation
+    from warpbiocell_amd.fields.scalar_field import GridGeometry
+    from warpbiocell_amd.geometry import Sphere, TissueRegion
+
+    w = W.spheroid_100k_161()
+    geom = GridGeometry.centered_cube(w.box_um, w.dx_um)
+    r_cells = np.linalg.norm(w.positions, axis=1)
+    region = TissueRegion.from_shape(Sphere(radius=float(np.percentile(r_cells, 95))), geom, device=device)
+    params = ContactParams(stiffness=W.CONTACT_RATE, damping=1.0, max_radius=float(w.radii.max()))
+    dt = W.DT_MECHANICS
+    out = {"workload": w.name, "cells": int(w.n), "substeps_per_call": 4,
+           "region": f"sphere r={float(np.percentile(r_cells, 95)):.1f} um"}
+    for label, reg in (("plain", None), ("confined", region)):
+        pop = CellPopulation.from_numpy(w.positions, w.radii, device=device)
+        grid = make_neighbor_grid(params, device)
+        relax_contacts(pop, grid, params, dt, 4, reg)
+        torch.cuda.synchronize()
+        t0 = time.perf_counter()
+        for _ in range(repeats):
+            relax_contacts(pop, grid, params, dt, 4, reg)
+        torch.cuda.synchronize()
+        out[f"{label}_us_per_substep"] = round((time.perf_counter() - t0) / (4 * repeats) * 1e6, 2)
+        if reg is not None:
+            inside = reg.contains(pop.positions_numpy(), margin=0.0)
+            out["outside_after"] = int((~inside).sum())
+    out["outside_before"] = int((~region.contains(w.positions)).sum())
+    out["overhead"] = round(out["confined_us_per_substep"] / out["plain_us_per_substep"] - 1.0, 4)
+    return out
+
+
 def batched_measurement(device, steps: int) -> dict:
     """Batched sweep on the 15-replica workload: `steps` cell steps of every replica, once
     serially with the plain step (replica after replica, as the reference's sweep runs them, each
@@ -286,6 +325,7 @@ def main() -> int:
     ap.add_argument("--no-roofline", action="store_true")
     ap.add_argument("--no-adjoint", action="store_true")
     ap.add_argument("--no-batched", action="store_true")
+    ap.add_argument("--no-confinement", action="store_true")
     ap.add_argument("--batched-steps", type=int, default=100)
     ap.add_argument("--sor-shape", default=None,
                     help="run the SOR sweeps with this shape (autotune.config_shapes() name) instead of the "
@@ -350,6 +390,9 @@ def main() -> int:
     adj = None if args.no_adjoint else adjoint_measurement(device)
     if adj:
         log(f"  adjoint: {adj}")
+    conf = None if args.no_confinement else confinement_measurement(device)
+    if conf:
+        log(f"  confinement: {conf}")
     bat = None if args.no_batched else batched_measurement(device, args.batched_steps)
     if bat:
         log(f"  batched: {bat}")
@@ -381,6 +424,7 @@ def main() -> int:
         "sor_autotune": tune,
         "adjoint": adj,
         "batched": bat,
+        "confinement": conf,
         "cpu_baseline": cpu,
     }
     print(json.dumps(result), flush=True)

@@ -140,24 +140,54 @@ __device__ __forceinline__ float region_lerp(const float* __restrict__ f, int st
     return (c00 * (1.0f - fy) + c10 * fy) * (1.0f - fz) + (c01 * (1.0f - fy) + c11 * fy) * fz;
 }
 
-// confinement_velocities for one cell at (px, py, pz) of radius r (kernels/geometry_kernels.py:59-79):
-// where the cell surface pokes out of the tissue (p = sdf + r > 0) and the sampled gradient is not
-// degenerate (|g| >= 1e-8), v -= (wall_rate * p / |g|) * g component by component.
-__device__ __forceinline__ void confine(const RegionDev& g, float wall_rate, float px, float py, float pz, float r,
-                                        float& vx, float& vy, float& vz)
+// The SDF at the 8 corners of a cell's voxel, loaded ahead of use: the contact gather issues
+// these loads before its neighbour walk, so that they complete alongside its first loads rather
+// than after it. (Loading the gradient there as well measured no faster: only cells at the
+// tissue surface read it.)
+struct RegionSample {
+    RegionCorners w;
+    float s[8];
+};
+
+__device__ __forceinline__ RegionSample region_sample_corners(const RegionDev& g, float px, float py, float pz)
 {
-    const RegionCorners w = region_corners(g, px, py, pz);
-    const float penetration = region_lerp(g.sdf, 1, 0, w) + r;
+    RegionSample r;
+    r.w = region_corners(g, px, py, pz);
+#pragma unroll
+    for (int c = 0; c < 8; ++c) r.s[c] = g.sdf[r.w.e[c]];
+    return r;
+}
+
+// confinement_velocities for one cell of radius r whose voxel corners and SDF values are `q`
+// (kernels/geometry_kernels.py:59-79): where the cell surface pokes out of the tissue
+// (p = sdf + r > 0) and the sampled gradient is not degenerate (|g| >= 1e-8),
+// v -= (wall_rate * p / |g|) * g component by component. The SDF lerps are sample_scalar's
+// (:28-34).
+__device__ __forceinline__ void confine_corners(const RegionDev& g, float wall_rate, const RegionSample& q, float r,
+                                                float& vx, float& vy, float& vz)
+{
+    const float fx = q.w.fx, fy = q.w.fy, fz = q.w.fz;
+    const float c00 = q.s[0] * (1.0f - fx) + q.s[1] * fx;
+    const float c10 = q.s[2] * (1.0f - fx) + q.s[3] * fx;
+    const float c01 = q.s[4] * (1.0f - fx) + q.s[5] * fx;
+    const float c11 = q.s[6] * (1.0f - fx) + q.s[7] * fx;
+    const float penetration = (c00 * (1.0f - fy) + c10 * fy) * (1.0f - fz) + (c01 * (1.0f - fy) + c11 * fy) * fz + r;
     if (penetration <= 0.0f) return;
-    const float gx = region_lerp(g.gradient, 3, 0, w);
-    const float gy = region_lerp(g.gradient, 3, 1, w);
-    const float gz = region_lerp(g.gradient, 3, 2, w);
+    const float gx = region_lerp(g.gradient, 3, 0, q.w);
+    const float gy = region_lerp(g.gradient, 3, 1, q.w);
+    const float gz = region_lerp(g.gradient, 3, 2, q.w);
     const float length = sqrtf(gx * gx + gy * gy + gz * gz);
     if (length < 1.0e-8f) return;
     const float coef = wall_rate * penetration / length;
     vx = vx - coef * gx;
     vy = vy - coef * gy;
     vz = vz - coef * gz;
+}
+
+__device__ __forceinline__ void confine(const RegionDev& g, float wall_rate, float px, float py, float pz, float r,
+                                        float& vx, float& vy, float& vz)
+{
+    confine_corners(g, wall_rate, region_sample_corners(g, px, py, pz), r, vx, vy, vz);
 }
 
 // ---- hash grid --------------------------------------------------------------------------------

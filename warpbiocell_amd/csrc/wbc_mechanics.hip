@@ -99,11 +99,13 @@ __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict_
     int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int32_t i = id[t];
+    const float4 P = xyzr[t];
+    RegionSample q;
+    if (kConfine) q = region_sample_corners(region, P.x, P.y, P.z);
     float vx, vy, vz;
     int32_t count;
     contact_gather(start, xyzr, t, inv_cell, dim, query_radius, rate, vx, vy, vz, count);
-    const float4 P = xyzr[t];
-    if (kConfine) confine(region, wall_rate, P.x, P.y, P.z, P.w, vx, vy, vz);
+    if (kConfine) confine_corners(region, wall_rate, q, P.w, vx, vy, vz);
     velocity[3 * i + 0] = vx;
     velocity[3 * i + 1] = vy;
     velocity[3 * i + 2] = vz;
