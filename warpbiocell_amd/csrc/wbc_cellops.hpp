@@ -208,10 +208,57 @@ __device__ __forceinline__ int32_t bucket_of(const float* __restrict__ position,
     return (cx * dim + cy) * dim + cz;
 }
 
+// Candidates per load group of the contact gather: a lane issues this many neighbour loads
+// before it uses the first, instead of one load and one wait per candidate. Measured at 100k
+// cells (whole substep, build included): 1 -> 134 us, 2 -> 112 us, 4 -> 110 us, 8 -> 110 us.
+#ifndef WBC_GATHER_GROUP
+#define WBC_GATHER_GROUP 4
+#endif
+constexpr int kGatherGroup = WBC_GATHER_GROUP;
+
+// The candidates u in [s, e) of one bucket run, in slot order (skipping the point itself, slot
+// t): loads in groups of kGatherGroup (the index clamped to e - 1 past the run's end, where the
+// value is not used), the distances and overlaps of the whole group computed unconditionally
+// (so that every load is issued, whole, before the first branch), then the terms in order.
+__device__ __forceinline__ void gather_run(const float4* __restrict__ xyzr, int32_t s, int32_t e, int32_t t,
+                                           const float4& P, float query_radius, float rate, float& vx, float& vy,
+                                           float& vz, int32_t& count)
+{
+    for (int32_t u = s; u < e; u += kGatherGroup) {
+        float4 Q[kGatherGroup];
+#pragma unroll
+        for (int g = 0; g < kGatherGroup; ++g) Q[g] = xyzr[min(u + g, e - 1)];
+        float dx[kGatherGroup], dy[kGatherGroup], dz[kGatherGroup], d[kGatherGroup], overlap[kGatherGroup];
+#pragma unroll
+        for (int g = 0; g < kGatherGroup; ++g) {
+            dx[g] = P.x - Q[g].x;
+            dy[g] = P.y - Q[g].y;
+            dz[g] = P.z - Q[g].z;
+            d[g] = sqrtf(dx[g] * dx[g] + dy[g] * dy[g] + dz[g] * dz[g]);
+            overlap[g] = P.w + Q[g].w - d[g];
+        }
+#pragma unroll
+        for (int g = 0; g < kGatherGroup; ++g) {
+            if (u + g >= e || u + g == t || !(d[g] < query_radius)) continue;
+            count += 1;
+            if (overlap[g] > 0.0f && d[g] > kCoincidentEps) {
+                const float coef = rate * overlap[g] / d[g];
+                vx = vx + coef * dx[g];
+                vy = vy + coef * dy[g];
+                vz = vz + coef * dz[g];
+            }
+        }
+    }
+}
+
 // contact_velocities for the point at bucket-ordered slot t (kernels/mechanics_kernels.py:36-57):
 // buckets x, then y, then z around the point, each in slot order. `start` holds the dim^3 + 1
-// bucket starts of this grid and `xyzr` its bucket-ordered (x, y, z, r); both are indexed
-// relative to `base` (the grid's first slot in a packed array).
+// bucket starts of this grid and `xyzr` its bucket-ordered (x, y, z, r).
+//
+// The walk is a list of runs of consecutive slots: one per (x, y) row when the row's z buckets
+// are consecutive keys (no wrap at dim), else one per bucket. In the common case (the query
+// radius equal to the bucket edge: 3 x 3 rows, no wrap) the 18 run bounds are loaded together
+// up front; otherwise each run's bounds are loaded as the walk reaches it.
 __device__ __forceinline__ void contact_gather(const int32_t* __restrict__ start, const float4* __restrict__ xyzr,
                                                int32_t t, float inv_cell, int32_t dim, float query_radius,
                                                float rate, float& vx, float& vy, float& vz, int32_t& count)
@@ -226,36 +273,38 @@ __device__ __forceinline__ void contact_gather(const int32_t* __restrict__ start
     const int32_t z0 = cell_coord(P.z - query_radius, inv_cell), z1 = cell_coord(P.z + query_radius, inv_cell);
     const int32_t wz0 = wrap_dim(z0, dim);
     const bool z_contiguous = wz0 + (z1 - z0) < dim;  // the z buckets are consecutive keys
+    if (z_contiguous && x1 - x0 == 2 && y1 - y0 == 2) {
+        int32_t S[9], E[9];
+#pragma unroll
+        for (int a = 0; a < 3; ++a)
+#pragma unroll
+            for (int b = 0; b < 3; ++b) {
+                const int32_t row = (wrap_dim(x0 + a, dim) * dim + wrap_dim(y0 + b, dim)) * dim + wz0;
+                S[3 * a + b] = start[row];
+                E[3 * a + b] = start[row + (z1 - z0) + 1];
+            }
+        for (int r = 0; r < 9; ++r) {
+            const int32_t s = S[0], e = E[0];
+#pragma unroll
+            for (int k = 0; k < 8; ++k) {
+                S[k] = S[k + 1];
+                E[k] = E[k + 1];
+            }
+            gather_run(xyzr, s, e, t, P, query_radius, rate, vx, vy, vz, count);
+        }
+        return;
+    }
     for (int32_t cx = x0; cx <= x1; ++cx) {
         for (int32_t cy = y0; cy <= y1; ++cy) {
             const int32_t row = (wrap_dim(cx, dim) * dim + wrap_dim(cy, dim)) * dim;
+            if (z_contiguous) {
+                gather_run(xyzr, start[row + wz0], start[row + wz0 + (z1 - z0) + 1], t, P, query_radius, rate, vx,
+                           vy, vz, count);
+                continue;
+            }
             for (int32_t cz = z0; cz <= z1; ++cz) {
-                int32_t s, e;
-                if (z_contiguous) {
-                    s = start[row + wz0];
-                    e = start[row + wz0 + (z1 - z0) + 1];
-                    cz = z1;  // one contiguous run covers every z bucket of this row
-                } else {
-                    int32_t b = row + wrap_dim(cz, dim);
-                    s = start[b];
-                    e = start[b + 1];
-                }
-                for (int32_t u = s; u < e; ++u) {
-                    if (u == t) continue;
-                    const float4 Q = xyzr[u];
-                    float dx = P.x - Q.x, dy = P.y - Q.y, dz = P.z - Q.z;
-                    float d = sqrtf(dx * dx + dy * dy + dz * dz);
-                    if (d < query_radius) {
-                        count += 1;
-                        float overlap = P.w + Q.w - d;
-                        if (overlap > 0.0f && d > kCoincidentEps) {
-                            float coef = rate * overlap / d;
-                            vx = vx + coef * dx;
-                            vy = vy + coef * dy;
-                            vz = vz + coef * dz;
-                        }
-                    }
-                }
+                const int32_t b = row + wrap_dim(cz, dim);
+                gather_run(xyzr, start[b], start[b + 1], t, P, query_radius, rate, vx, vy, vz, count);
             }
         }
     }
