@@ -573,10 +573,11 @@ __global__ void __launch_bounds__(256) batch_contact_kernel(const ReplicaDev* __
     const int32_t i = id[t];
     const int r = rep_of_slot[i];
     const ReplicaDev& d = rep[r];
+    __shared__ int32_t list[kGatherList * 256];
     float vx, vy, vz;
     int32_t count;
     contact_gather(start + (int64_t)r * dim * dim * dim, xyzr, t, d.inv_cell, dim, d.query_radius, d.contact_rate,
-                   vx, vy, vz, count);
+                   list + threadIdx.x, 256, vx, vy, vz, count);
     const float4 P = xyzr[t];
     velocity[3 * i + 0] = vx;
     velocity[3 * i + 1] = vy;

@@ -209,65 +209,105 @@ __device__ __forceinline__ int32_t bucket_of(const float* __restrict__ position,
 }
 
 // Candidates per load group of the contact gather: a lane issues this many neighbour loads
-// before it uses the first, instead of one load and one wait per candidate. Measured at 100k
-// cells (whole substep, build included): 1 -> 134 us, 2 -> 112 us, 4 -> 110 us, 8 -> 110 us.
-#ifndef WBC_GATHER_GROUP
-#define WBC_GATHER_GROUP 4
-#endif
-constexpr int kGatherGroup = WBC_GATHER_GROUP;
+// before it uses the first, instead of one load and one wait per candidate. LDS slots per lane
+// for the candidates that pass the distance prefilter (32 KB for a 256-thread workgroup).
+// Measured at 100k cells, whole substep (build included), group/slots: 4/16 102 us, 4/32 91,
+// 4/64 91, 8/16 100, 8/24 95, 8/32 88, 8/48 87, 16/32 109, 16/48 102, 2/32 97; one pass without
+// the list: group 4 110 us, group 1 134 us.
+constexpr int kGatherGroup = 8;
+constexpr int kGatherList = 32;
 
-// The candidates u in [s, e) of one bucket run, in slot order (skipping the point itself, slot
-// t): loads in groups of kGatherGroup (the index clamped to e - 1 past the run's end, where the
-// value is not used), the distances and overlaps of the whole group computed unconditionally
-// (so that every load is issued, whole, before the first branch), then the terms in order.
+// The contact terms of the candidates a lane has listed, in list order: the reference's exact
+// test and arithmetic (kernels/mechanics_kernels.py:45-57).
+struct GatherState {
+    float4 P;
+    float query_radius, rate, bound;
+    int32_t* col;     // this lane's LDS list: entry k at col[k * stride]
+    int32_t stride;
+    int32_t listed;
+    float vx, vy, vz;
+    int32_t count;
+};
+
+__device__ __forceinline__ void gather_drain(const float4* __restrict__ xyzr, GatherState& g)
+{
+    for (int32_t k = 0; k < g.listed; k += kGatherGroup) {
+        float4 Q[kGatherGroup];
+#pragma unroll
+        for (int j = 0; j < kGatherGroup; ++j) Q[j] = xyzr[g.col[min(k + j, g.listed - 1) * g.stride]];
+#pragma unroll
+        for (int j = 0; j < kGatherGroup; ++j) {
+            if (k + j >= g.listed) continue;
+            const float dx = g.P.x - Q[j].x, dy = g.P.y - Q[j].y, dz = g.P.z - Q[j].z;
+            const float d = sqrtf(dx * dx + dy * dy + dz * dz);
+            if (!(d < g.query_radius)) continue;
+            g.count += 1;
+            const float overlap = g.P.w + Q[j].w - d;
+            if (overlap > 0.0f && d > kCoincidentEps) {
+                const float coef = g.rate * overlap / d;
+                g.vx = g.vx + coef * dx;
+                g.vy = g.vy + coef * dy;
+                g.vz = g.vz + coef * dz;
+            }
+        }
+    }
+    g.listed = 0;
+}
+
+// The candidates u in [s, e) of one bucket run, in slot order, skipping the point itself (slot t):
+// loads in groups of kGatherGroup (the index clamped to e - 1 past the run's end), and every
+// candidate whose squared distance is at most `bound` appended to the lane's list. `bound` is
+// the float above q*q, so every candidate with sqrt(d^2) < q (the exact test, rounded square root)
+// is listed; the drain applies the exact test. A wave drains its lists whenever one could
+// overflow with the next group, so the terms are summed in walk order.
 __device__ __forceinline__ void gather_run(const float4* __restrict__ xyzr, int32_t s, int32_t e, int32_t t,
-                                           const float4& P, float query_radius, float rate, float& vx, float& vy,
-                                           float& vz, int32_t& count)
+                                           GatherState& g)
 {
     for (int32_t u = s; u < e; u += kGatherGroup) {
         float4 Q[kGatherGroup];
 #pragma unroll
-        for (int g = 0; g < kGatherGroup; ++g) Q[g] = xyzr[min(u + g, e - 1)];
-        float dx[kGatherGroup], dy[kGatherGroup], dz[kGatherGroup], d[kGatherGroup], overlap[kGatherGroup];
+        for (int j = 0; j < kGatherGroup; ++j) Q[j] = xyzr[min(u + j, e - 1)];
 #pragma unroll
-        for (int g = 0; g < kGatherGroup; ++g) {
-            dx[g] = P.x - Q[g].x;
-            dy[g] = P.y - Q[g].y;
-            dz[g] = P.z - Q[g].z;
-            d[g] = sqrtf(dx[g] * dx[g] + dy[g] * dy[g] + dz[g] * dz[g]);
-            overlap[g] = P.w + Q[g].w - d[g];
-        }
-#pragma unroll
-        for (int g = 0; g < kGatherGroup; ++g) {
-            if (u + g >= e || u + g == t || !(d[g] < query_radius)) continue;
-            count += 1;
-            if (overlap[g] > 0.0f && d[g] > kCoincidentEps) {
-                const float coef = rate * overlap[g] / d[g];
-                vx = vx + coef * dx[g];
-                vy = vy + coef * dy[g];
-                vz = vz + coef * dz[g];
+        for (int j = 0; j < kGatherGroup; ++j) {
+            const float dx = g.P.x - Q[j].x, dy = g.P.y - Q[j].y, dz = g.P.z - Q[j].z;
+            const float d2 = dx * dx + dy * dy + dz * dz;
+            if (u + j < e && u + j != t && d2 <= g.bound) {
+                g.col[g.listed * g.stride] = u + j;
+                g.listed += 1;
             }
         }
+        if (__any(g.listed > kGatherList - kGatherGroup)) gather_drain(xyzr, g);
     }
 }
 
 // contact_velocities for the point at bucket-ordered slot t (kernels/mechanics_kernels.py:36-57):
 // buckets x, then y, then z around the point, each in slot order. `start` holds the dim^3 + 1
-// bucket starts of this grid and `xyzr` its bucket-ordered (x, y, z, r).
+// bucket starts of this grid and `xyzr` its bucket-ordered (x, y, z, r); `list` is this lane's
+// LDS column of kGatherList ints, `stride` apart.
 //
 // The walk is a list of runs of consecutive slots: one per (x, y) row when the row's z buckets
 // are consecutive keys (no wrap at dim), else one per bucket. In the common case (the query
 // radius equal to the bucket edge: 3 x 3 rows, no wrap) the 18 run bounds are loaded together
-// up front; otherwise each run's bounds are loaded as the walk reaches it.
+// up front; otherwise each run's bounds are loaded as the walk reaches it. The walk lists the
+// candidates inside the prefilter bound; the drain computes their terms in walk order.
 __device__ __forceinline__ void contact_gather(const int32_t* __restrict__ start, const float4* __restrict__ xyzr,
                                                int32_t t, float inv_cell, int32_t dim, float query_radius,
-                                               float rate, float& vx, float& vy, float& vz, int32_t& count)
+                                               float rate, int32_t* list, int32_t stride, float& vx, float& vy,
+                                               float& vz, int32_t& count)
 {
-    const float4 P = xyzr[t];
-    vx = 0.0f;
-    vy = 0.0f;
-    vz = 0.0f;
-    count = 0;
+    GatherState g;
+    g.P = xyzr[t];
+    g.query_radius = query_radius;
+    g.rate = rate;
+    g.bound = nextafterf(query_radius * query_radius, __builtin_huge_valf());
+    g.col = list;
+    g.stride = stride;
+    g.listed = 0;
+    g.vx = 0.0f;
+    g.vy = 0.0f;
+    g.vz = 0.0f;
+    g.count = 0;
+    const float4 P = g.P;
     const int32_t x0 = cell_coord(P.x - query_radius, inv_cell), x1 = cell_coord(P.x + query_radius, inv_cell);
     const int32_t y0 = cell_coord(P.y - query_radius, inv_cell), y1 = cell_coord(P.y + query_radius, inv_cell);
     const int32_t z0 = cell_coord(P.z - query_radius, inv_cell), z1 = cell_coord(P.z + query_radius, inv_cell);
@@ -290,24 +330,28 @@ __device__ __forceinline__ void contact_gather(const int32_t* __restrict__ start
                 S[k] = S[k + 1];
                 E[k] = E[k + 1];
             }
-            gather_run(xyzr, s, e, t, P, query_radius, rate, vx, vy, vz, count);
+            gather_run(xyzr, s, e, t, g);
         }
-        return;
-    }
-    for (int32_t cx = x0; cx <= x1; ++cx) {
-        for (int32_t cy = y0; cy <= y1; ++cy) {
-            const int32_t row = (wrap_dim(cx, dim) * dim + wrap_dim(cy, dim)) * dim;
-            if (z_contiguous) {
-                gather_run(xyzr, start[row + wz0], start[row + wz0 + (z1 - z0) + 1], t, P, query_radius, rate, vx,
-                           vy, vz, count);
-                continue;
-            }
-            for (int32_t cz = z0; cz <= z1; ++cz) {
-                const int32_t b = row + wrap_dim(cz, dim);
-                gather_run(xyzr, start[b], start[b + 1], t, P, query_radius, rate, vx, vy, vz, count);
+    } else {
+        for (int32_t cx = x0; cx <= x1; ++cx) {
+            for (int32_t cy = y0; cy <= y1; ++cy) {
+                const int32_t row = (wrap_dim(cx, dim) * dim + wrap_dim(cy, dim)) * dim;
+                if (z_contiguous) {
+                    gather_run(xyzr, start[row + wz0], start[row + wz0 + (z1 - z0) + 1], t, g);
+                    continue;
+                }
+                for (int32_t cz = z0; cz <= z1; ++cz) {
+                    const int32_t b = row + wrap_dim(cz, dim);
+                    gather_run(xyzr, start[b], start[b + 1], t, g);
+                }
             }
         }
     }
+    gather_drain(xyzr, g);
+    vx = g.vx;
+    vy = g.vy;
+    vz = g.vz;
+    count = g.count;
 }
 
 __device__ __forceinline__ float oxygen_factor(float o, float hypoxia_threshold, float death_threshold)

@@ -102,9 +102,10 @@ __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict_
     const float4 P = xyzr[t];
     RegionSample q;
     if (kConfine) q = region_sample_corners(region, P.x, P.y, P.z);
+    __shared__ int32_t list[kGatherList * 256];
     float vx, vy, vz;
     int32_t count;
-    contact_gather(start, xyzr, t, inv_cell, dim, query_radius, rate, vx, vy, vz, count);
+    contact_gather(start, xyzr, t, inv_cell, dim, query_radius, rate, list + threadIdx.x, 256, vx, vy, vz, count);
     if (kConfine) confine_corners(region, wall_rate, q, P.w, vx, vy, vz);
     velocity[3 * i + 0] = vx;
     velocity[3 * i + 1] = vy;
