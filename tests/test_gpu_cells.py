@@ -28,7 +28,7 @@ def test_rng_streams_are_bit_exact(oracle, device):
     np.testing.assert_array_equal(pop.rng_numpy(), oracle.init_rng(70001, 77))
 
 
-@pytest.mark.parametrize("n", [1, 63, 64, 65, 4095, 4096, 4097, 100_003, 1_048_577])
+@pytest.mark.parametrize("n", [1, 63, 64, 65, 4095, 4096, 4097, 100_003, 1_048_577, 4_194_304, 4_194_305, 9_000_001])
 def test_inclusive_scan_matches_cumsum(device, n):
     flags = (np.random.default_rng(n).random(n) < 0.37).astype(np.int32)
     inp = torch.from_numpy(flags).to(device)

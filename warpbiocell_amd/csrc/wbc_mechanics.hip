@@ -54,11 +54,13 @@ __global__ void __launch_bounds__(256) hg_key_count(const float* __restrict__ po
     atomicAdd(cnt + b, 1);
 }
 
+// Also sets start[nb] = n, the end of the last bucket (the scan writes start[0, nb)).
 __global__ void __launch_bounds__(256) hg_scatter(const int32_t* __restrict__ key, int32_t n,
-                                                   const int32_t* __restrict__ start, int32_t* __restrict__ cnt,
-                                                   int32_t* __restrict__ tmp)
+                                                   int32_t* __restrict__ start, int32_t* __restrict__ cnt,
+                                                   int32_t* __restrict__ tmp, int64_t nb)
 {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
+    if (i == 0) start[nb] = n;
     if (i >= n) return;
     int32_t b = key[i];
     int32_t left = atomicSub(cnt + b, 1);
@@ -74,11 +76,6 @@ __global__ void __launch_bounds__(256) hg_order(const int32_t* __restrict__ key,
     int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     hg_order_point(key, start, tmp, position, radius, t, id, xyzr);
-}
-
-__global__ void __launch_bounds__(256) hg_set_total(int32_t* __restrict__ start, int64_t nb, int32_t n)
-{
-    start[nb] = n;
 }
 
 // Gather of cell t (bucket order): neighbour count within query_radius and the repulsion
@@ -192,8 +189,7 @@ static hipError_t hashgrid_build(const float* position, const float* radius, int
     hg_key_count<<<grid_for(n, 256), 256, 0, s>>>(position, n, inv_cell, dim, key, cnt);
     WBC_TRY_ERR(hipGetLastError());
     WBC_TRY_ERR(scan_i32(cnt, start, nb, (int32_t*)(ws + L.partial), true, s));
-    hg_set_total<<<1, 1, 0, s>>>(start, nb, n);
-    hg_scatter<<<grid_for(n, 256), 256, 0, s>>>(key, n, start, cnt, tmp);
+    hg_scatter<<<grid_for(n, 256), 256, 0, s>>>(key, n, start, cnt, tmp, nb);
     hg_order<<<grid_for(n, 256), 256, 0, s>>>(key, n, start, tmp, position, radius, (int32_t*)(ws + L.id),
                                               (float4*)(ws + L.xyzr));
     return hipGetLastError();

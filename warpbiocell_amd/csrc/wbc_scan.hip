@@ -60,12 +60,23 @@ __global__ void __launch_bounds__(kScanBlock) scan_partials(int* __restrict__ pa
     if (threadIdx.x == 0) partial[num_tiles] = carry;
 }
 
+// Tile pass. `partial` holds either the tiles' exclusive offsets (after scan_partials) or, with
+// `self_prefix`, the raw tile sums, which each workgroup adds up to its own tile itself (used up to
+// kSelfPrefixTiles tiles, where reading the earlier sums costs less than a launch).
 __global__ void __launch_bounds__(kScanBlock) scan_tiles(const int* __restrict__ in, int* __restrict__ out,
                                                            int64_t n, const int* __restrict__ partial,
-                                                           int exclusive)
+                                                           int exclusive, int self_prefix)
 {
     __shared__ int tile[kScanTile + kScanTile / 32];  // padded against bank conflicts
     __shared__ int wave_sums[kScanBlock / kWave];
+    int offset;
+    if (self_prefix) {
+        int s = 0;
+        for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanBlock) s += partial[k];
+        block_exclusive_scan(s, wave_sums, offset);
+    } else {
+        offset = partial[blockIdx.x];
+    }
     const int64_t base = (int64_t)blockIdx.x * kScanTile;
     auto slot = [](int e) { return e + (e >> 5); };
 #pragma unroll
@@ -83,7 +94,7 @@ __global__ void __launch_bounds__(kScanBlock) scan_tiles(const int* __restrict__
         s += vals[k];
     }
     int total;
-    int run = block_exclusive_scan(s, wave_sums, total) + partial[blockIdx.x];
+    int run = block_exclusive_scan(s, wave_sums, total) + offset;
 #pragma unroll
     for (int k = 0; k < kScanItems; ++k) {
         int v = vals[k];
@@ -109,9 +120,10 @@ hipError_t scan_i32(const int* in, int* out, int64_t n, int* partial, bool exclu
 {
     if (n <= 0) return hipSuccess;
     int tiles = (int)((n + kScanTile - 1) / kScanTile);
+    const int self_prefix = tiles <= kSelfPrefixTiles ? 1 : 0;
     scan_reduce_tiles<<<tiles, kScanBlock, 0, stream>>>(in, n, partial);
-    scan_partials<<<1, kScanBlock, 0, stream>>>(partial, tiles);
-    scan_tiles<<<tiles, kScanBlock, 0, stream>>>(in, out, n, partial, exclusive ? 1 : 0);
+    if (!self_prefix) scan_partials<<<1, kScanBlock, 0, stream>>>(partial, tiles);
+    scan_tiles<<<tiles, kScanBlock, 0, stream>>>(in, out, n, partial, exclusive ? 1 : 0, self_prefix);
     return hipGetLastError();
 }
 

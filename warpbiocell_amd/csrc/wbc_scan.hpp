@@ -7,11 +7,14 @@ namespace wbc {
 constexpr int kScanBlock = 256;
 constexpr int kScanItems = 16;
 constexpr int kScanTile = kScanBlock * kScanItems;
+// Up to this many tiles each tile adds up the earlier tiles' sums itself (two launches per scan);
+// beyond it one workgroup scans the tile sums first (three launches).
+constexpr int kSelfPrefixTiles = 1024;
 
 // Bytes of the tile-sum buffer a scan of n values needs (tiles + 1 ints, 256-byte aligned).
 size_t scan_workspace_bytes(int64_t n);
 
-// out = inclusive (or exclusive) prefix sum of in[0, n); partial[tiles] receives the total.
+// out = inclusive (or exclusive) prefix sum of in[0, n); partial holds the tile sums or offsets.
 hipError_t scan_i32(const int* in, int* out, int64_t n, int* partial, bool exclusive, hipStream_t stream);
 
 }  // namespace wbc
