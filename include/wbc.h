@@ -61,6 +61,16 @@ int wbc_place_daughters(int32_t count, float placement_factor, const int32_t* di
                         int32_t* cell_type, float* age, float* oxygen_local, float* glucose_local,
                         uint32_t* rng_state, float* velocity, int32_t* neighbor_count, int32_t* fate_flags,
                         int64_t n, wbc_stream_t stream);
+/* Replaces commit_divisions, cells/lifecycle.py:135-179, in one call: the inclusive scan of
+ * divide_flag into division_offset (workspace: wbc_scan_workspace_bytes(count)), the daughter
+ * count read back to the host (the step's one host sync; *n_daughters), and, when the daughters
+ * fit in `capacity`, wbc_place_daughters. Daughters that do not fit: nothing is placed, the count
+ * is still returned, and the call returns hipErrorOutOfMemory (the reference's CapacityError). */
+int wbc_commit_divisions(int32_t count, int64_t capacity, float placement_factor, const int32_t* divide_flag,
+                         int32_t* division_offset, void* scan_workspace, size_t workspace_bytes, float* position,
+                         float* radius, int32_t* cell_state, int32_t* cell_type, float* age, float* oxygen_local,
+                         float* glucose_local, uint32_t* rng_state, float* velocity, int32_t* neighbor_count,
+                         int32_t* fate_flags, int32_t* n_daughters, wbc_stream_t stream);
 
 /* ---- cell -> grid deposit ------------------------------------------------------------------
  * Replaces CellDensities.deposit, fields/densities.py:26-43: accumulator.zero_() (use

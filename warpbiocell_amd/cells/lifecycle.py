@@ -9,6 +9,7 @@ with nothing placed when the capacity would be exceeded.
 
 from __future__ import annotations
 
+import ctypes
 from dataclasses import dataclass
 
 from warpbiocell_amd import _lib
@@ -78,28 +79,34 @@ def decide_lifecycle(population, params: LifecycleParams, dt: float) -> None:
               n, p.stream)
 
 
+_HIP_ERROR_OUT_OF_MEMORY = 2
+
+
 def commit_divisions(population, params: LifecycleParams) -> int:
-    """Append the daughters of the cells whose ``divide_flag`` is set; returns their number."""
+    """Append the daughters of the cells whose ``divide_flag`` is set; returns their number.
+
+    One C call: the slot scan, the daughter count read back (the step's single host sync), the
+    capacity check and the daughter placement (cells/lifecycle.py:135-179)."""
     n = population.count
     if n == 0:
         return 0
     p = population
     ws = p.scan_workspace()
-    _lib.call("wbc_scan_inclusive_i32", p.divide_flag.data_ptr(), p.division_offset.data_ptr(), n, ws.data_ptr(),
-              ws.numel(), p.stream)
-    n_daughters = int(p.division_offset[n - 1].item())  # the single host sync
-    if n_daughters == 0:
-        return 0
-    if n + n_daughters > p.capacity:
+    out = ctypes.c_int32(0)
+    rc = _lib.load().wbc_commit_divisions(
+        n, p.capacity, params.placement_factor, p.divide_flag.data_ptr(), p.division_offset.data_ptr(),
+        ws.data_ptr(), ws.numel(), p.position.data_ptr(), p.radius.data_ptr(), p.cell_state.data_ptr(),
+        p.cell_type.data_ptr(), p.age.data_ptr(), p.oxygen_local.data_ptr(), p.glucose_local.data_ptr(),
+        p.rng_state.data_ptr(), p.velocity.data_ptr(), p.neighbor_count.data_ptr(), p.fate_flags.data_ptr(),
+        ctypes.byref(out), p.stream)
+    n_daughters = int(out.value)
+    if rc == _HIP_ERROR_OUT_OF_MEMORY and n + n_daughters > p.capacity:
         raise CapacityError(
             f"{n_daughters} divisions would bring the population to {n + n_daughters} cells, "
             f"above the allocated capacity of {p.capacity}"
         )
-    _lib.call("wbc_place_daughters", n, params.placement_factor, p.divide_flag.data_ptr(),
-              p.division_offset.data_ptr(), p.position.data_ptr(), p.radius.data_ptr(), p.cell_state.data_ptr(),
-              p.cell_type.data_ptr(), p.age.data_ptr(), p.oxygen_local.data_ptr(), p.glucose_local.data_ptr(),
-              p.rng_state.data_ptr(), p.velocity.data_ptr(), p.neighbor_count.data_ptr(), p.fate_flags.data_ptr(),
-              n, p.stream)
+    if rc != 0:
+        raise RuntimeError(f"wbc_commit_divisions failed with hipError_t {rc}")
     p.count = n + n_daughters
     return n_daughters
 

@@ -62,6 +62,35 @@ __global__ void __launch_bounds__(256) place_daughters_kernel(
     // rng_state[j] was seeded for every slot when the population was created; left untouched.
 }
 
+// The total of an inclusive scan, offset[n - 1], into mapped host memory.
+__global__ void scan_total_kernel(const int32_t* __restrict__ offset, int64_t n, int32_t* __restrict__ host_out)
+{
+    if (threadIdx.x == 0) host_out[0] = offset[n - 1];
+}
+
+// One mapped, coherent host int and one event per device, for the division count.
+struct CountSlot {
+    int32_t* host = nullptr;
+    int32_t* dev = nullptr;
+    hipEvent_t ev = nullptr;
+};
+static CountSlot g_count_slot[16];
+
+static hipError_t count_slot(CountSlot** out)
+{
+    int d = 0;
+    WBC_TRY_ERR(hipGetDevice(&d));
+    if (d < 0 || d >= 16) return hipErrorInvalidDevice;
+    CountSlot& c = g_count_slot[d];
+    if (!c.host) {
+        WBC_TRY_ERR(hipHostMalloc((void**)&c.host, 256, hipHostMallocMapped | hipHostMallocCoherent));
+        WBC_TRY_ERR(hipHostGetDevicePointer((void**)&c.dev, c.host, 0));
+        WBC_TRY_ERR(hipEventCreateWithFlags(&c.ev, hipEventDisableTiming));
+    }
+    *out = &c;
+    return hipSuccess;
+}
+
 }  // namespace wbc
 
 using namespace wbc;
@@ -116,4 +145,36 @@ WBC_API int wbc_place_daughters(int32_t count, float placement_factor, const int
         oxygen_local, glucose_local, rng_state, velocity, neighbor_count, fate_flags, n);
     WBC_CHECK_LAUNCH();
     return 0;
+}
+
+// commit_divisions (cells/lifecycle.py:135-179) in one call: the inclusive scan of divide_flag
+// into division_offset, the daughter count read back through mapped host memory (the step's one
+// host sync), the capacity check, and place_daughters. *n_daughters receives the count, also when
+// it does not fit: then nothing is placed and the call returns hipErrorOutOfMemory.
+WBC_API int wbc_commit_divisions(int32_t count, int64_t capacity, float placement_factor, const int32_t* divide_flag,
+                                 int32_t* division_offset, void* scan_workspace, size_t workspace_bytes,
+                                 float* position, float* radius, int32_t* cell_state, int32_t* cell_type, float* age,
+                                 float* oxygen_local, float* glucose_local, uint32_t* rng_state, float* velocity,
+                                 int32_t* neighbor_count, int32_t* fate_flags, int32_t* n_daughters,
+                                 hipStream_t stream)
+{
+    if (count < 0 || capacity < count || !n_daughters) return (int)hipErrorInvalidValue;
+    *n_daughters = 0;
+    if (count == 0) return 0;
+    if (!divide_flag || !division_offset || !scan_workspace || workspace_bytes < scan_workspace_bytes(count))
+        return (int)hipErrorInvalidValue;
+    WBC_TRY(scan_i32(divide_flag, division_offset, count, (int32_t*)scan_workspace, false, stream));
+    CountSlot* slot = nullptr;
+    WBC_TRY(count_slot(&slot));
+    scan_total_kernel<<<1, 64, 0, stream>>>(division_offset, count, slot->dev);
+    WBC_CHECK_LAUNCH();
+    WBC_TRY(hipEventRecord(slot->ev, stream));
+    WBC_TRY(hipEventSynchronize(slot->ev));
+    const int32_t d = *(volatile int32_t*)slot->host;
+    *n_daughters = d;
+    if (d == 0) return 0;
+    if ((int64_t)count + d > capacity) return (int)hipErrorOutOfMemory;
+    return wbc_place_daughters(count, placement_factor, divide_flag, division_offset, position, radius, cell_state,
+                               cell_type, age, oxygen_local, glucose_local, rng_state, velocity, neighbor_count,
+                               fate_flags, count, stream);
 }

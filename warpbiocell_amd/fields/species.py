@@ -91,7 +91,6 @@ class SpeciesField:
         self.owns_densities = densities is None
         self.density = torch.zeros(geometry.shape, dtype=torch.float32, device=self.device)
         self.production = torch.zeros(geometry.shape, dtype=torch.float32, device=self.device)
-        self._residual = torch.zeros(1, dtype=torch.float32, device=self.device)
         self.last_report: SolveReport | None = None
 
     @property
@@ -138,7 +137,7 @@ class SpeciesField:
 
     def solve_steady_state(self, source: SpeciesField | None = None) -> SolveReport:
         report = solve_steady_state(
-            self.field, self.density, self.species.kinetics, self.settings, self._residual,
+            self.field, self.density, self.species.kinetics, self.settings, None,
             production=self.production if self.species.produces else None,
             source=source.field.values if source is not None else None,
             source_k=source.species.michaelis_k if source is not None else 1.0,
