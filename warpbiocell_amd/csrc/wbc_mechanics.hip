@@ -82,8 +82,11 @@ __global__ void __launch_bounds__(256) hg_order(const int32_t* __restrict__ key,
 // velocity; with kConfine the tissue-wall push of the cell's start-of-substep position is added
 // (the reference's separate confinement launch reads the same position and radius, and the
 // velocity the gather has just written); with kIntegrate x += v * dt into the slot-ordered
-// position array (safe because the gather reads only the bucket-ordered copy).
-template <bool kIntegrate, bool kConfine>
+// position array (safe because the gather reads only the bucket-ordered copy). With kNextKeys
+// (integrating, before another substep) the moved cell's bucket key and count for the next build
+// are produced here too, from the same floats the position array receives: the build's counts are
+// back at zero after its scatter, and its keys are no longer read.
+template <bool kIntegrate, bool kConfine, bool kNextKeys>
 __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict__ start,
                                                        const int32_t* __restrict__ id,
                                                        const float4* __restrict__ xyzr, int32_t n, float inv_cell,
@@ -91,7 +94,8 @@ __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict_
                                                        RegionDev region, float wall_rate,
                                                        float* __restrict__ velocity,
                                                        int32_t* __restrict__ neighbor_count,
-                                                       float* __restrict__ position)
+                                                       float* __restrict__ position, int32_t* __restrict__ next_key,
+                                                       int32_t* __restrict__ next_cnt)
 {
     int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -109,9 +113,17 @@ __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict_
     velocity[3 * i + 2] = vz;
     neighbor_count[i] = count;
     if (kIntegrate) {
-        position[3 * i + 0] = P.x + vx * dt;
-        position[3 * i + 1] = P.y + vy * dt;
-        position[3 * i + 2] = P.z + vz * dt;
+        const float x = P.x + vx * dt, y = P.y + vy * dt, z = P.z + vz * dt;
+        position[3 * i + 0] = x;
+        position[3 * i + 1] = y;
+        position[3 * i + 2] = z;
+        if (kNextKeys) {
+            const int32_t b = (wrap_dim(cell_coord(x, inv_cell), dim) * dim + wrap_dim(cell_coord(y, inv_cell), dim)) *
+                                  dim +
+                              wrap_dim(cell_coord(z, inv_cell), dim);
+            next_key[i] = b;
+            atomicAdd(next_cnt + b, 1);
+        }
     }
 }
 
@@ -178,16 +190,28 @@ __global__ void __launch_bounds__(256) zero_i32_kernel_mech(int32_t* __restrict_
     if (t < count) p[t] = 0;
 }
 
+static hipError_t hashgrid_build_from_keys(const float* position, const float* radius, int32_t n, int32_t dim,
+                                           char* ws, const HashGridLayout& L, hipStream_t s);
+
 static hipError_t hashgrid_build(const float* position, const float* radius, int32_t n, float inv_cell,
                                  int32_t dim, char* ws, const HashGridLayout& L, hipStream_t s)
+{
+    int32_t* cnt = (int32_t*)(ws + L.cnt);
+    int32_t* key = (int32_t*)(ws + L.key);
+    hg_key_count<<<grid_for(n, 256), 256, 0, s>>>(position, n, inv_cell, dim, key, cnt);
+    WBC_TRY_ERR(hipGetLastError());
+    return hashgrid_build_from_keys(position, radius, n, dim, ws, L, s);
+}
+
+// Steps 2-4 of the build, from keys and counts already in the workspace.
+static hipError_t hashgrid_build_from_keys(const float* position, const float* radius, int32_t n, int32_t dim,
+                                           char* ws, const HashGridLayout& L, hipStream_t s)
 {
     const int64_t nb = (int64_t)dim * dim * dim;
     int32_t* cnt = (int32_t*)(ws + L.cnt);
     int32_t* start = (int32_t*)(ws + L.start);
     int32_t* key = (int32_t*)(ws + L.key);
     int32_t* tmp = (int32_t*)(ws + L.tmp);
-    hg_key_count<<<grid_for(n, 256), 256, 0, s>>>(position, n, inv_cell, dim, key, cnt);
-    WBC_TRY_ERR(hipGetLastError());
     WBC_TRY_ERR(scan_i32(cnt, start, nb, (int32_t*)(ws + L.partial), true, s));
     hg_scatter<<<grid_for(n, 256), 256, 0, s>>>(key, n, start, cnt, tmp, nb);
     hg_order<<<grid_for(n, 256), 256, 0, s>>>(key, n, start, tmp, position, radius, (int32_t*)(ws + L.id),
@@ -269,7 +293,8 @@ static bool region_ok(const wbc_region_t* r)
            (int64_t)r->nx * r->ny * r->nz * 3 < ((int64_t)1 << 40);
 }
 
-static int launch_contact(bool integrate, const void* workspace, size_t workspace_bytes, int32_t dim,
+// next_keys (integrating only): the kernel also produces the next build's keys and counts.
+static int launch_contact(bool integrate, bool next_keys, const void* workspace, size_t workspace_bytes, int32_t dim,
                           int64_t capacity, int32_t n, float cell_size, float query_radius, float rate, float dt,
                           const wbc_region_t* region, float wall_rate, float* velocity, int32_t* neighbor_count,
                           float* position, hipStream_t stream)
@@ -280,21 +305,25 @@ static int launch_contact(bool integrate, const void* workspace, size_t workspac
     if (!workspace || !velocity || !neighbor_count || (integrate && !position)) return (int)hipErrorInvalidValue;
     HashGridLayout L = hashgrid_layout(dim, capacity);
     if (workspace_bytes < L.total) return (int)hipErrorInvalidValue;
-    const char* ws = (const char*)workspace;
+    char* ws = (char*)workspace;
     const int32_t* start = (const int32_t*)(ws + L.start);
     const int32_t* id = (const int32_t*)(ws + L.id);
     const float4* xyzr = (const float4*)(ws + L.xyzr);
+    int32_t* key = (int32_t*)(ws + L.key);
+    int32_t* cnt = (int32_t*)(ws + L.cnt);
     float inv_cell = 1.0f / cell_size;
     RegionDev g{};
     const bool confined = region_of(region, g);
     auto go = [&](auto kernel) {
         kernel<<<grid_for(n, 256), 256, 0, stream>>>(start, id, xyzr, n, inv_cell, dim, query_radius, rate, dt, g,
-                                                     wall_rate, velocity, neighbor_count, position);
+                                                     wall_rate, velocity, neighbor_count, position, key, cnt);
     };
-    if (integrate)
-        confined ? go(contact_kernel<true, true>) : go(contact_kernel<true, false>);
+    if (integrate && next_keys)
+        confined ? go(contact_kernel<true, true, true>) : go(contact_kernel<true, false, true>);
+    else if (integrate)
+        confined ? go(contact_kernel<true, true, false>) : go(contact_kernel<true, false, false>);
     else
-        confined ? go(contact_kernel<false, true>) : go(contact_kernel<false, false>);
+        confined ? go(contact_kernel<false, true, false>) : go(contact_kernel<false, false, false>);
     WBC_CHECK_LAUNCH();
     return 0;
 }
@@ -303,8 +332,8 @@ WBC_API int wbc_contact_velocities(const void* workspace, size_t workspace_bytes
                                    int32_t n, float cell_size, float query_radius, float rate, float* velocity,
                                    int32_t* neighbor_count, hipStream_t stream)
 {
-    return launch_contact(false, workspace, workspace_bytes, dim, capacity, n, cell_size, query_radius, rate, 0.0f,
-                          nullptr, 0.0f, velocity, neighbor_count, nullptr, stream);
+    return launch_contact(false, false, workspace, workspace_bytes, dim, capacity, n, cell_size, query_radius, rate,
+                          0.0f, nullptr, 0.0f, velocity, neighbor_count, nullptr, stream);
 }
 
 WBC_API int wbc_integrate_positions(float* position, const float* velocity, int64_t n, float dt, hipStream_t stream)
@@ -317,17 +346,28 @@ WBC_API int wbc_integrate_positions(float* position, const float* velocity, int6
     return 0;
 }
 
+// Substep s > 0 builds from the keys and counts the previous substep's gather produced, so each
+// substep after the first skips the key pass.
 static int relax(float* position, const float* radius, int32_t n, float cell_size, int32_t dim, float query_radius,
                  float rate, float dt, int32_t substeps, const wbc_region_t* region, float wall_rate, float* velocity,
                  int32_t* neighbor_count, int64_t capacity, void* workspace, size_t workspace_bytes,
                  hipStream_t stream)
 {
     if (substeps < 0) return (int)hipErrorInvalidValue;
+    if (!hg_args_ok(dim, capacity) || n < 0 || n > capacity || !(cell_size > 0.0f)) return (int)hipErrorInvalidValue;
+    if (n == 0 || substeps == 0) return 0;
+    if (!position || !radius || !workspace) return (int)hipErrorInvalidValue;
+    const HashGridLayout L = hashgrid_layout(dim, capacity);
+    if (workspace_bytes < L.total) return (int)hipErrorInvalidValue;
     for (int32_t s = 0; s < substeps; ++s) {
-        int rc = wbc_hashgrid_build(position, radius, n, cell_size, dim, capacity, workspace, workspace_bytes, stream);
+        int rc;
+        if (s == 0)
+            rc = wbc_hashgrid_build(position, radius, n, cell_size, dim, capacity, workspace, workspace_bytes, stream);
+        else
+            rc = (int)hashgrid_build_from_keys(position, radius, n, dim, (char*)workspace, L, stream);
         if (rc) return rc;
-        rc = launch_contact(true, workspace, workspace_bytes, dim, capacity, n, cell_size, query_radius, rate, dt,
-                            region, wall_rate, velocity, neighbor_count, position, stream);
+        rc = launch_contact(true, s + 1 < substeps, workspace, workspace_bytes, dim, capacity, n, cell_size,
+                            query_radius, rate, dt, region, wall_rate, velocity, neighbor_count, position, stream);
         if (rc) return rc;
     }
     return 0;
