@@ -109,3 +109,39 @@ def test_runs_are_bitwise_reproducible(device):
         return pop.positions_numpy()
 
     np.testing.assert_array_equal(run(), run())
+
+
+@pytest.mark.parametrize("factor,dim", [(0.45, 128), (0.7, 128), (1.3, 128), (2.5, 128), (0.7, 7), (1.0, 5)])
+def test_other_bucket_sizes_take_the_general_walk(oracle, device, factor, dim):
+    """Bucket edge != query radius (or wrapping buckets): the walk's per-run path instead of the
+    preloaded 3 x 3 rows; counts and velocities equal the oracle's bits."""
+    rng = np.random.default_rng(int(factor * 100) + dim)
+    pos = spherical_cluster(1500, 8.0, spacing_factor=0.9, jitter=0.3, seed=int(factor * 10))
+    radii = rng.uniform(5.0, 9.0, 1500).astype(np.float32)
+    params = ContactParams(stiffness=1.0, damping=1.0, max_radius=9.0, margin=0.5)
+    q = params.query_radius
+    cell = np.float32(factor * q)
+    pop = CellPopulation.from_numpy(pos, radii, device=device)
+    contact_substep(pop, NeighborGrid(float(cell), dim=dim, device=device), params, dt=0.0)
+    vel, cnt = oracle.contact_velocities(pos, radii, cell, dim, q, params.rate)
+    np.testing.assert_array_equal(pop.neighbor_counts_numpy(), cnt)
+    np.testing.assert_array_equal(pop.velocity.cpu().numpy()[:1500], vel)
+
+
+def test_dense_packing_drains_the_candidate_lists_in_order(oracle, device):
+    """Over 100 neighbours inside the query radius per cell: every lane's candidate list fills and is
+    drained several times per walk; the sums stay in walk order (bit-exact)."""
+    pos = spherical_cluster(2500, 8.0, spacing_factor=0.3, jitter=0.3, seed=3)
+    radii = np.full(2500, 8.0, np.float32)
+    params = ContactParams(stiffness=1.0, damping=1.0, max_radius=8.0, margin=0.5)
+    q = params.query_radius
+    pop = CellPopulation.from_numpy(pos, radii, device=device)
+    contact_substep(pop, make_neighbor_grid(params, device), params, dt=0.0)
+    vel, cnt = oracle.contact_velocities(pos, radii, np.float32(q), 128, q, params.rate)
+    assert np.median(cnt) > 100
+    np.testing.assert_array_equal(pop.neighbor_counts_numpy(), cnt)
+    np.testing.assert_array_equal(pop.velocity.cpu().numpy()[:2500], vel)
+    relax_contacts(pop, make_neighbor_grid(params, device), params, dt=0.01, substeps=5)
+    ref, v5, c5 = pos.copy(), np.zeros_like(pos), np.zeros(2500, np.int32)
+    oracle.relax_contacts(ref, radii, np.float32(q), 128, q, params.rate, 0.01, 5, v5, c5)
+    np.testing.assert_array_equal(pop.positions_numpy(), ref)
