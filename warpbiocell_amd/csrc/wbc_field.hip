@@ -275,6 +275,7 @@ constexpr int kEventPool = 8;
 // one chunk of no-op launches after convergence. Headline step (100k cells, 161^3): 0 -> 2.352 ms,
 // 1 -> 2.283 ms, 2 -> 2.330 ms.
 constexpr int kLag = 1;
+static_assert(kLag >= 1, "the colour-packed loop records chunk c's check while queueing chunk c + 1");
 struct EventPool {
     int device = -1;
     hipEvent_t ev[kEventPool];
@@ -331,13 +332,14 @@ struct SolveLayout {
     size_t state, slots, partial, scratch, packed, packed_each, total;
 };
 
-// Residual partials of either residual kernel, a second field buffer for the ping-pong sweeps
-// and the four colour-packed arrays (field and density per colour) for the packed sweeps.
+// Residual partials of the residual kernel or of the checked colour passes, a second field
+// buffer for the ping-pong sweeps, and five colour-packed arrays for the packed sweeps: field and
+// density per colour and a second red field.
 SolveLayout solve_layout(int32_t chunks, const SorParams& p)
 {
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-    const int32_t partials = residual_partials(p) > residual_packed_partials(p) ? residual_partials(p)
-                                                                                : residual_packed_partials(p);
+    const int32_t packed_partials = 2 * packed_pass_blocks(p, 2);  // the most workgroups of any rows
+    const int32_t partials = residual_partials(p) > packed_partials ? residual_partials(p) : packed_partials;
     const int64_t nodes = (int64_t)p.nx * p.ny * p.nz;
     SolveLayout L;
     size_t off = 0;
@@ -346,7 +348,7 @@ SolveLayout solve_layout(int32_t chunks, const SorParams& p)
     L.partial = off; off += up((size_t)partials * sizeof(float));
     L.scratch = off; off += up((size_t)nodes * sizeof(float));
     L.packed_each = up((size_t)packed_floats(p) * sizeof(float));
-    L.packed = off;  off += 4 * L.packed_each;
+    L.packed = off;  off += 5 * L.packed_each;
     L.total = off;
     return L;
 }
@@ -395,21 +397,44 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
     const int packed_rows = packed_supported(p) ? packed_rows_of(g_fused_config) : 0;
     const bool packed = packed_rows != 0;
     const bool fused = !packed && fused_supported(p);
-    const int32_t partials = packed ? residual_packed_partials(p) : residual_partials(p);
+    // Colour-packed: the residual check is folded into the passes. The black pass that ends chunk
+    // c stores the black terms; the red pass that starts chunk c + 1 stores the red terms of the
+    // red values it reads and writes its result to the other red array, so red[c & 1] keeps the
+    // field chunk c's check describes (the result, if that check converges).
+    const int32_t pass_blocks = packed ? packed_pass_blocks(p, packed_rows) : 0;
+    const int32_t partials = packed ? 2 * pass_blocks : residual_partials(p);
     PackedField pf{{(float*)(ws + L.packed), (float*)(ws + L.packed + L.packed_each)},
                    {(float*)(ws + L.packed + 2 * L.packed_each), (float*)(ws + L.packed + 3 * L.packed_each)}};
+    float* red[2] = {pf.f[0], (float*)(ws + L.packed + 4 * L.packed_each)};
     hipEvent_t* ev;
     WBC_TRY(get_events(&ev));
     float *slots_host, *slots_mapped;
     WBC_TRY(get_host_slots(chunks, &slots_host, &slots_mapped));
     zero_i32_kernel<<<1, 256, 0, stream>>>(state, 4);
     WBC_CHECK_LAUNCH();
-    if (packed) WBC_TRY(launch_pack(p, field, density, pf, stream));
+    if (packed) {
+        WBC_TRY(launch_pack(p, field, density, pf, stream));
+        WBC_TRY(launch_copy_f32(red[0], red[1], packed_floats(p), stream));  // the fixed face values of both
+    }
 
     auto chunk_len = [&](int32_t c) {
         return max_sweeps - c * check_every < check_every ? max_sweeps - c * check_every : check_every;
     };
-    int32_t queued = 0, checked = 0, sweeps = 0, cur = 0;  // buf[cur] holds the field after the queued sweeps
+    int32_t queued = 0, recorded = 0, checked = 0, sweeps = 0, cur = 0;  // buf[cur]: the field after the queued sweeps
+    // Chunk c's check: reduce its partials into slots[c] (and the mapped host copy), then its event.
+    auto queue_check = [&](int32_t c) -> hipError_t {
+        WBC_TRY_ERR(launch_residual_finish(partial, partials, slots, slots_mapped, c, tolerance, state, stream));
+        WBC_TRY_ERR(hipEventRecord(ev[c % kEventPool], stream));
+        recorded = c + 1;
+        return hipSuccess;
+    };
+    // The red pass that opens chunk q (q > 0, or q == chunks after the last chunk) with chunk q-1's
+    // red check, then that check.
+    auto red_check = [&](int32_t q) -> hipError_t {
+        WBC_TRY_ERR(launch_packed_colour(p, 0, packed_rows, 1, red[(q - 1) & 1], red[q & 1], pf.f[1], pf.d[0], state,
+                                         partial + pass_blocks, stream));
+        return queue_check(q - 1);
+    };
     float residual = __builtin_huge_valf();
     bool converged = false;
     int32_t result_buf = 0;
@@ -418,7 +443,15 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
             const int32_t len = chunk_len(queued);
             for (int32_t s = 0; s < len; ++s) {
                 if (packed) {
-                    WBC_TRY(launch_sor_packed(p, pf, packed_rows, state, stream));
+                    float* r = red[queued & 1];
+                    if (s == 0 && queued > 0)
+                        WBC_TRY(red_check(queued));
+                    else
+                        WBC_TRY(launch_packed_colour(p, 0, packed_rows, 0, r, r, pf.f[1], pf.d[0], state, nullptr,
+                                                     stream));
+                    const bool last = s == len - 1;
+                    WBC_TRY(launch_packed_colour(p, 1, packed_rows, last ? 2 : 0, pf.f[1], pf.f[1], r, pf.d[1], state,
+                                                 last ? partial : nullptr, stream));
                 } else if (fused) {
                     WBC_TRY(launch_sor_fused(p, buf[cur], buf[cur ^ 1], state, stream));
                     cur ^= 1;
@@ -427,15 +460,14 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
                     WBC_TRY(launch_sor_color(p, 1, field, state, stream));
                 }
             }
-            if (packed)
-                WBC_TRY(launch_residual_packed(p, pf, partial, state, stream));
-            else
+            if (!packed) {
                 WBC_TRY(launch_residual_partial(p, buf[cur], partial, state, stream));
-            WBC_TRY(launch_residual_finish(partial, partials, slots, slots_mapped, queued, tolerance, state, stream));
-            WBC_TRY(hipEventRecord(ev[queued % kEventPool], stream));
+                WBC_TRY(queue_check(queued));
+            }
             ++queued;
+            if (packed && queued == chunks) WBC_TRY(red_check(chunks));
         }
-        if (checked == queued) break;
+        if (checked == recorded) break;
         WBC_TRY(hipEventSynchronize(ev[checked % kEventPool]));
         residual = ((volatile float*)slots_host)[checked];
         host_residual[checked] = residual;
@@ -447,8 +479,12 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
         }
     }
     // The field after the last executed chunk is in buf[sweeps % 2] (fused path) or in the packed
-    // arrays; chunks queued after convergence did nothing.
-    if (packed) WBC_TRY(launch_unpack(p, pf, field, stream));
+    // arrays, red[(checked - 1) & 1] for the red nodes; chunks queued after convergence did nothing.
+    if (packed) {
+        PackedField result = pf;
+        result.f[0] = red[(checked - 1) & 1];
+        WBC_TRY(launch_unpack(p, result, field, stream));
+    }
     if (fused) result_buf = sweeps & 1;
     if (result_buf != 0) WBC_TRY(launch_copy_f32(buf[1], field, nodes, stream));
     *sweeps_out = sweeps;

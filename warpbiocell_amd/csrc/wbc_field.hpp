@@ -74,14 +74,20 @@ inline int64_t packed_floats_of(int32_t nx, int32_t ny, int32_t nz)
 }
 bool packed_supported(const SorParams& p);
 inline int64_t packed_floats(const SorParams& p) { return packed_floats_of(p.nx, p.ny, p.nz); }
-int residual_packed_partials(const SorParams& p);
 hipError_t launch_pack(const SorParams& p, const float* field, const float* density, PackedField& pf, hipStream_t s);
 hipError_t launch_unpack(const SorParams& p, const PackedField& pf, float* field, hipStream_t s);
 // One sweep (both colour passes); each thread takes `rows` (2, 4 or 8) consecutive rows j.
 hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, const int32_t* done, hipStream_t s);
-// Residual partials of both colours (residual_packed_partials(p) of them, for launch_residual_finish).
-hipError_t launch_residual_packed(const SorParams& p, const PackedField& pf, float* partial, const int32_t* done,
-                                  hipStream_t s);
+// Workgroups of one colour pass at `rows` rows per thread.
+int packed_pass_blocks(const SorParams& p, int rows);
+// One colour pass of the solve loop's colour-packed path with the residual check folded in.
+// check 0: in place in `own`. check 1: the pass also stores the residual terms of the values it
+// reads (one maximum per workgroup in partial[0, packed_pass_blocks)) and writes its new values
+// to `out`, leaving `own` as the field the check describes. check 2: in place, storing the terms
+// of the values it writes.
+hipError_t launch_packed_colour(const SorParams& p, int colour, int rows, int check, float* own, float* out,
+                                const float* other, const float* dens, const int32_t* done, float* partial,
+                                hipStream_t s);
 
 // The sweep shape table (wbc_sor_fused.hip): the entry the solve loop uses, and the colour-packed
 // entries (which the solve loop runs on packed arrays for its whole duration): rows per thread,

@@ -57,53 +57,38 @@ __global__ void __launch_bounds__(256) unpack_kernel(const float* __restrict__ f
 }
 
 // One colour pass over the interior nodes of colour C, in place in `own`; each thread takes RJ
-// consecutive rows of one plane at one packed index.
-template <int C, int RJ>
-__global__ void __launch_bounds__(256) sor_packed_kernel(SorParams p, float* __restrict__ own,
+// consecutive rows of one plane at one packed index. With kRes (see packed_update) the pass also
+// stores its workgroup's maximum residual term in partial[blockIdx.x], and with kRes == 1 it
+// writes the new values to `out`, leaving `own` as the field the residual describes.
+template <int C, int RJ, int kRes>
+__global__ void __launch_bounds__(256) sor_packed_kernel(SorParams p, float* own, float* out,
                                                          const float* __restrict__ other,
                                                          const float* __restrict__ dens,
                                                          const int32_t* __restrict__ done, int blocks_per_plane,
-                                                         int lanes_per_plane, uint32_t nz2_magic, int q_finite)
+                                                         int lanes_per_plane, uint32_t nz2_magic, int q_finite,
+                                                         float* __restrict__ partial)
 {
     static_assert(RJ % 2 == 0, "an even row count keeps the parity of a thread's first row uniform");
+    __shared__ float wave_max_lds[4];
     if (done && *done) return;
     int i, u;
     packed_lane(xcd_tile(blockIdx.x, gridDim.x), blocks_per_plane, i, u);
-    if (u >= lanes_per_plane) return;
-    if ((C + i + 1) & 1)
-        packed_update<RJ, 1>(p, own, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
-    else
-        packed_update<RJ, 0>(p, own, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
-}
-
-// The residual of both colours: workgroups [0, B) take colour 0, [B, 2B) colour 1 (B = blocks of
-// one colour pass with kResidualRows rows per thread); each stores its maximum in partial[].
-__global__ void __launch_bounds__(256) residual_packed_kernel(SorParams p, PackedField pf, float* __restrict__ partial,
-                                                              const int32_t* __restrict__ done, int blocks_per_plane,
-                                                              int lanes_per_plane, uint32_t nz2_magic, int q_finite)
-{
-    __shared__ float wave_max_lds[4];
-    if (done && *done) return;
-    const int B = blocks_per_plane * (p.nx - 2);
-    const int t = xcd_tile(blockIdx.x, gridDim.x);
-    const int colour = t >= B ? 1 : 0;
-    int i, u;
-    packed_lane(t - colour * B, blocks_per_plane, i, u);
+    float* dst = kRes == 1 ? out : own;
     float m = 0.0f;
     if (u < lanes_per_plane) {
-        const float* own = pf.f[colour];
-        const float* other = pf.f[colour ^ 1];
-        const float* dens = pf.d[colour];
-        if ((colour + i + 1) & 1)
-            m = packed_residual<kResidualRows, 1>(p, own, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
+        if ((C + i + 1) & 1)
+            m = packed_update<RJ, 1, kRes>(p, own, dst, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
         else
-            m = packed_residual<kResidualRows, 0>(p, own, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
+            m = packed_update<RJ, 0, kRes>(p, own, dst, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
     }
-    m = wave_max(m);
-    if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;
-    __syncthreads();
-    if (threadIdx.x == 0)
-        partial[blockIdx.x] = fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
+    if (kRes) {
+        m = wave_max(m);
+        if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;
+        __syncthreads();
+        if (threadIdx.x == 0)
+            partial[blockIdx.x] =
+                fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
+    }
 }
 
 // ---- host side ------------------------------------------------------------------------------
@@ -112,12 +97,6 @@ bool packed_supported(const SorParams& p)
 {
     return p.bc_x == 0 && p.bc_y == 0 && p.bc_z == 0 && !p.fixed && !p.production && !p.has_source &&
            packed_lanes_per_plane(p, 2) >= 0 && packed_floats(p) < ((int64_t)1 << 31);
-}
-
-int residual_packed_partials(const SorParams& p)
-{
-    const int64_t lanes = packed_lanes_per_plane(p, kResidualRows);
-    return lanes < 0 ? 0 : 2 * (int)((lanes + 255) / 256) * (p.nx - 2);
 }
 
 hipError_t launch_pack(const SorParams& p, const float* field, const float* density, PackedField& pf, hipStream_t s)
@@ -136,19 +115,30 @@ hipError_t launch_unpack(const SorParams& p, const PackedField& pf, float* field
     return hipGetLastError();
 }
 
-template <int RJ>
-static hipError_t launch_packed_rows(const SorParams& p, PackedField& pf, const int32_t* done, hipStream_t s)
+int packed_pass_blocks(const SorParams& p, int rows)
+{
+    const int64_t lanes = packed_lanes_per_plane(p, rows);
+    return lanes < 0 ? 0 : (int)((lanes + 255) / 256) * (p.nx - 2);
+}
+
+template <int C, int RJ, int kRes>
+static hipError_t launch_pass(const SorParams& p, float* own, float* out, const float* other, const float* dens,
+                              const int32_t* done, float* partial, hipStream_t s)
 {
     const int64_t lanes = packed_lanes_per_plane(p, RJ);
     if (lanes < 0) return hipErrorInvalidValue;
-    const uint32_t magic = nz2_magic_of(p);
     const int bpp = (int)((lanes + 255) / 256);
-    const int blocks = bpp * (p.nx - 2);
     const int qf = __builtin_isfinite(p.uptake_max) ? 1 : 0;
-    sor_packed_kernel<0, RJ><<<blocks, 256, 0, s>>>(p, pf.f[0], pf.f[1], pf.d[0], done, bpp, (int)lanes, magic, qf);
-    WBC_TRY_ERR(hipGetLastError());
-    sor_packed_kernel<1, RJ><<<blocks, 256, 0, s>>>(p, pf.f[1], pf.f[0], pf.d[1], done, bpp, (int)lanes, magic, qf);
+    sor_packed_kernel<C, RJ, kRes><<<bpp * (p.nx - 2), 256, 0, s>>>(p, own, out, other, dens, done, bpp, (int)lanes,
+                                                                    nz2_magic_of(p), qf, partial);
     return hipGetLastError();
+}
+
+template <int RJ>
+static hipError_t launch_packed_rows(const SorParams& p, PackedField& pf, const int32_t* done, hipStream_t s)
+{
+    WBC_TRY_ERR((launch_pass<0, RJ, 0>(p, pf.f[0], pf.f[0], pf.f[1], pf.d[0], done, nullptr, s)));
+    return launch_pass<1, RJ, 0>(p, pf.f[1], pf.f[1], pf.f[0], pf.d[1], done, nullptr, s);
 }
 
 hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, const int32_t* done, hipStream_t s)
@@ -161,16 +151,32 @@ hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, cons
     }
 }
 
-hipError_t launch_residual_packed(const SorParams& p, const PackedField& pf, float* partial, const int32_t* done,
-                                  hipStream_t s)
+template <int RJ>
+static hipError_t launch_colour_rows(const SorParams& p, int colour, int check, float* own, float* out,
+                                     const float* other, const float* dens, const int32_t* done, float* partial,
+                                     hipStream_t s)
 {
-    const int64_t lanes = packed_lanes_per_plane(p, kResidualRows);
-    if (lanes < 0) return hipErrorInvalidValue;
-    const int bpp = (int)((lanes + 255) / 256);
-    const int qf = __builtin_isfinite(p.uptake_max) ? 1 : 0;
-    residual_packed_kernel<<<2 * bpp * (p.nx - 2), 256, 0, s>>>(p, pf, partial, done, bpp, (int)lanes,
-                                                                 nz2_magic_of(p), qf);
-    return hipGetLastError();
+    // The loop checks red values as the red pass reads them and black values as the black pass
+    // writes them; the other two combinations are not instantiated.
+    if (colour == 0 && check == 0) return launch_pass<0, RJ, 0>(p, own, own, other, dens, done, nullptr, s);
+    if (colour == 0 && check == 1) return launch_pass<0, RJ, 1>(p, own, out, other, dens, done, partial, s);
+    if (colour == 1 && check == 0) return launch_pass<1, RJ, 0>(p, own, own, other, dens, done, nullptr, s);
+    if (colour == 1 && check == 2) return launch_pass<1, RJ, 2>(p, own, own, other, dens, done, partial, s);
+    return hipErrorInvalidValue;
+}
+
+hipError_t launch_packed_colour(const SorParams& p, int colour, int rows, int check, float* own, float* out,
+                                const float* other, const float* dens, const int32_t* done, float* partial,
+                                hipStream_t s)
+{
+    if (check != 0 && !partial) return hipErrorInvalidValue;
+    if (check == 1 && (!out || out == own)) return hipErrorInvalidValue;
+    switch (rows) {
+    case 2: return launch_colour_rows<2>(p, colour, check, own, out, other, dens, done, partial, s);
+    case 4: return launch_colour_rows<4>(p, colour, check, own, out, other, dens, done, partial, s);
+    case 8: return launch_colour_rows<8>(p, colour, check, own, out, other, dens, done, partial, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 }  // namespace wbc

@@ -73,21 +73,40 @@ __device__ __forceinline__ bool packed_node(const PackedRows<RJ>& R, int r, cons
     return true;
 }
 
-// The colour pass on RJ rows (kernels/field_kernels.py:185-212, same expressions and order).
-template <int RJ, int P>
-__device__ __forceinline__ void packed_update(const SorParams& p, float* __restrict__ own,
-                                              const float* __restrict__ other, const float* __restrict__ dens,
-                                              int i, uint32_t u, uint32_t nz2_magic, int q_finite)
+// The residual term of one node (kernels/field_kernels.py:236-243, no production): |r| / max(6 o,
+// reference) with r = nb - (6 + h c(o)) o and c(o) = 0 where the density is 0 (uptake finite,
+// K + o > 0), d q / (K + o) elsewhere.
+__device__ __forceinline__ float packed_residual_term(const SorParams& p, float nb, float o, float d, int q_finite)
+{
+    const float kpo = p.michaelis_k + o;
+    float c = 0.0f;
+    if (!(d == 0.0f && kpo > 0.0f && q_finite)) c = d * p.uptake_max / kpo;
+    const float prod = 0.0f;
+    const float res = nb + p.dx2_over_D * prod - (6.0f + p.dx2_over_D * c) * o;
+    return fabsf(res) / fmaxf(6.0f * o, p.reference);
+}
+
+// The colour pass on RJ rows (kernels/field_kernels.py:185-212, same expressions and order),
+// writing the new values to `out` (the `own` array itself, or with kRes == 1 a second array of
+// the same colour). kRes also returns the maximum residual term over the rows: 1 = of the values
+// before the update (the field a check reads, when this is the pass after it), 2 = of the values
+// just written (when this is the pass that ends the check sweep).
+template <int RJ, int P, int kRes = 0>
+__device__ __forceinline__ float packed_update(const SorParams& p, const float* own, float* out,
+                                               const float* __restrict__ other, const float* __restrict__ dens,
+                                               int i, uint32_t u, uint32_t nz2_magic, int q_finite)
 {
     const uint32_t nz2 = (uint32_t)packed_nz2(p.nz);
     const size_t rs = (size_t)p.ny * nz2;
     PackedRows<RJ> R;
     load_packed_rows<RJ, P>(R, own, other, dens, i, u, nz2_magic, nz2, rs);
-    char* w_0 = (char*)(own + i * rs);
+    char* w_0 = (char*)(out + i * rs);
+    float m = 0.0f;
 #pragma unroll
     for (int r = 0; r < RJ; ++r) {
         float nb;
         if (!packed_node<RJ, P>(R, r, p, nb)) continue;
+        if (kRes == 1) m = fmaxf(m, packed_residual_term(p, nb, R.o[r], R.d[r], q_finite));
         const float kpo = p.michaelis_k + R.o[r];
         float gs;
         // density == 0 makes c == +-0 whenever K + o > 0 and uptake_max is finite: 6 + h*c == 6.
@@ -97,9 +116,11 @@ __device__ __forceinline__ void packed_update(const SorParams& p, float* __restr
             const float c = R.d[r] * p.uptake_max / kpo;
             gs = nb / (6.0f + p.dx2_over_D * c);
         }
-        const float v = (1.0f - p.omega) * R.o[r] + p.omega * gs;
-        *(float*)(w_0 + R.w1 + r * nz2 * 4u) = fmaxf(v, 0.0f);
+        const float v = fmaxf((1.0f - p.omega) * R.o[r] + p.omega * gs, 0.0f);
+        *(float*)(w_0 + R.w1 + r * nz2 * 4u) = v;
+        if (kRes == 2) m = fmaxf(m, packed_residual_term(p, nb, v, R.d[r], q_finite));
     }
+    return m;
 }
 
 // max over RJ rows of |r| / max(6 o, reference) (kernels/field_kernels.py:236-243, no production).
@@ -117,13 +138,7 @@ __device__ __forceinline__ float packed_residual(const SorParams& p, const float
     for (int r = 0; r < RJ; ++r) {
         float nb;
         if (!packed_node<RJ, P>(R, r, p, nb)) continue;
-        const float o = R.o[r];
-        const float kpo = p.michaelis_k + o;
-        float c = 0.0f;
-        if (!(R.d[r] == 0.0f && kpo > 0.0f && q_finite)) c = R.d[r] * p.uptake_max / kpo;
-        const float prod = 0.0f;
-        const float res = nb + p.dx2_over_D * prod - (6.0f + p.dx2_over_D * c) * o;
-        m = fmaxf(m, fabsf(res) / fmaxf(6.0f * o, p.reference));
+        m = fmaxf(m, packed_residual_term(p, nb, R.o[r], R.d[r], q_finite));
     }
     return m;
 }
