@@ -646,9 +646,9 @@ __global__ void __launch_bounds__(256) wide_pass_kernel(const WideRep* __restric
     const float* other = w.pf.f[C ^ 1];
     const float* dens = w.pf.d[C];
     if ((C + i + 1) & 1)
-        packed_update<RJ, 1>(w.p, own, own, other, dens, i, (uint32_t)u, w.magic, w.q_finite);
+        packed_update<RJ, 1>(w.p, own, nullptr, other, dens, i, (uint32_t)u, w.magic, w.q_finite);
     else
-        packed_update<RJ, 0>(w.p, own, own, other, dens, i, (uint32_t)u, w.magic, w.q_finite);
+        packed_update<RJ, 0>(w.p, own, nullptr, other, dens, i, (uint32_t)u, w.magic, w.q_finite);
 }
 
 // Residual partials (both colours) of the replicas in g that are active and check after sweep q:

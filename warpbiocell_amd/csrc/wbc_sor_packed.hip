@@ -61,7 +61,8 @@ __global__ void __launch_bounds__(256) unpack_kernel(const float* __restrict__ f
 // stores its workgroup's maximum residual term in partial[blockIdx.x], and with kRes == 1 it
 // writes the new values to `out`, leaving `own` as the field the residual describes.
 template <int C, int RJ, int kRes>
-__global__ void __launch_bounds__(256) sor_packed_kernel(SorParams p, float* own, float* out,
+__global__ void __launch_bounds__(256) sor_packed_kernel(SorParams p, float* __restrict__ own,
+                                                         float* __restrict__ out,
                                                          const float* __restrict__ other,
                                                          const float* __restrict__ dens,
                                                          const int32_t* __restrict__ done, int blocks_per_plane,
@@ -73,13 +74,13 @@ __global__ void __launch_bounds__(256) sor_packed_kernel(SorParams p, float* own
     if (done && *done) return;
     int i, u;
     packed_lane(xcd_tile(blockIdx.x, gridDim.x), blocks_per_plane, i, u);
-    float* dst = kRes == 1 ? out : own;
+    if (!kRes && u >= lanes_per_plane) return;  // with a check every thread reaches the reduction
     float m = 0.0f;
     if (u < lanes_per_plane) {
         if ((C + i + 1) & 1)
-            m = packed_update<RJ, 1, kRes>(p, own, dst, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
+            m = packed_update<RJ, 1, kRes>(p, own, out, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
         else
-            m = packed_update<RJ, 0, kRes>(p, own, dst, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
+            m = packed_update<RJ, 0, kRes>(p, own, out, other, dens, i, (uint32_t)u, nz2_magic, q_finite);
     }
     if (kRes) {
         m = wave_max(m);
@@ -137,8 +138,8 @@ static hipError_t launch_pass(const SorParams& p, float* own, float* out, const 
 template <int RJ>
 static hipError_t launch_packed_rows(const SorParams& p, PackedField& pf, const int32_t* done, hipStream_t s)
 {
-    WBC_TRY_ERR((launch_pass<0, RJ, 0>(p, pf.f[0], pf.f[0], pf.f[1], pf.d[0], done, nullptr, s)));
-    return launch_pass<1, RJ, 0>(p, pf.f[1], pf.f[1], pf.f[0], pf.d[1], done, nullptr, s);
+    WBC_TRY_ERR((launch_pass<0, RJ, 0>(p, pf.f[0], nullptr, pf.f[1], pf.d[0], done, nullptr, s)));
+    return launch_pass<1, RJ, 0>(p, pf.f[1], nullptr, pf.f[0], pf.d[1], done, nullptr, s);
 }
 
 hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, const int32_t* done, hipStream_t s)
@@ -158,10 +159,10 @@ static hipError_t launch_colour_rows(const SorParams& p, int colour, int check, 
 {
     // The loop checks red values as the red pass reads them and black values as the black pass
     // writes them; the other two combinations are not instantiated.
-    if (colour == 0 && check == 0) return launch_pass<0, RJ, 0>(p, own, own, other, dens, done, nullptr, s);
+    if (colour == 0 && check == 0) return launch_pass<0, RJ, 0>(p, own, nullptr, other, dens, done, nullptr, s);
     if (colour == 0 && check == 1) return launch_pass<0, RJ, 1>(p, own, out, other, dens, done, partial, s);
-    if (colour == 1 && check == 0) return launch_pass<1, RJ, 0>(p, own, own, other, dens, done, nullptr, s);
-    if (colour == 1 && check == 2) return launch_pass<1, RJ, 2>(p, own, own, other, dens, done, partial, s);
+    if (colour == 1 && check == 0) return launch_pass<1, RJ, 0>(p, own, nullptr, other, dens, done, nullptr, s);
+    if (colour == 1 && check == 2) return launch_pass<1, RJ, 2>(p, own, nullptr, other, dens, done, partial, s);
     return hipErrorInvalidValue;
 }
 

@@ -92,7 +92,8 @@ __device__ __forceinline__ float packed_residual_term(const SorParams& p, float 
 // before the update (the field a check reads, when this is the pass after it), 2 = of the values
 // just written (when this is the pass that ends the check sweep).
 template <int RJ, int P, int kRes = 0>
-__device__ __forceinline__ float packed_update(const SorParams& p, const float* own, float* out,
+__device__ __forceinline__ float packed_update(const SorParams& p, const float* __restrict__ own,
+                                               float* __restrict__ out,
                                                const float* __restrict__ other, const float* __restrict__ dens,
                                                int i, uint32_t u, uint32_t nz2_magic, int q_finite)
 {
@@ -100,7 +101,8 @@ __device__ __forceinline__ float packed_update(const SorParams& p, const float* 
     const size_t rs = (size_t)p.ny * nz2;
     PackedRows<RJ> R;
     load_packed_rows<RJ, P>(R, own, other, dens, i, u, nz2_magic, nz2, rs);
-    char* w_0 = (char*)(out + i * rs);
+    // In place (kRes != 1) the stores go through `own` itself; `out` is then not used.
+    char* w_0 = (char*)((kRes == 1 ? out : const_cast<float*>(own)) + i * rs);
     float m = 0.0f;
 #pragma unroll
     for (int r = 0; r < RJ; ++r) {
