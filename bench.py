@@ -49,7 +49,7 @@ def setup(workload_name: str, device):
 
 KERNEL_OF_KIND = {"tile": "sor_fused_kernel<{tj}, {ti}>", "march": "sor_march_kernel<{tj}, {ti}>",
                   "pipe": "sor_pipe_kernel<{tj}, {ti}>", "copy": "sor_copy_kernel<{tj}, {ti}>",
-                  "packed": "sor_packed_kernel<C, {tj}>"}
+                  "packed": "sor_packed_kernel<C, {tj}, 0>"}
 COUNTER_TABLE = ROOT / "profiles" / "sor_counters_spheroid_100k_161.md"
 
 
