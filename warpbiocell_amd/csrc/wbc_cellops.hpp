@@ -256,9 +256,11 @@ __device__ __forceinline__ void gather_drain(const float4* __restrict__ xyzr, Ga
 
 // The candidates u in [s, e) of one bucket run, in slot order, skipping the point itself (slot t):
 // loads in groups of kGatherGroup (the index clamped to e - 1 past the run's end), and every
-// candidate whose squared distance is at most `bound` appended to the lane's list. `bound` is
-// the float above q*q, so every candidate with sqrt(d^2) < q (the exact test, rounded square root)
-// is listed; the drain applies the exact test. A wave drains its lists whenever one could
+// candidate whose squared distance (here with fused multiply-adds) is at most `bound` appended to
+// the lane's list. A candidate the exact test keeps has sqrt(s) < q for the unfused sum s, so
+// s < q^2 in real arithmetic; s and the fused sum are each within a few units in the last place
+// of the real d^2, so the fused sum is below q^2 (1 + 2^-18) = `bound`: every kept candidate is
+// listed, and the drain applies the exact test. A wave drains its lists whenever one could
 // overflow with the next group, so the terms are summed in walk order.
 __device__ __forceinline__ void gather_run(const float4* __restrict__ xyzr, int32_t s, int32_t e, int32_t t,
                                            GatherState& g)
@@ -270,7 +272,7 @@ __device__ __forceinline__ void gather_run(const float4* __restrict__ xyzr, int3
 #pragma unroll
         for (int j = 0; j < kGatherGroup; ++j) {
             const float dx = g.P.x - Q[j].x, dy = g.P.y - Q[j].y, dz = g.P.z - Q[j].z;
-            const float d2 = dx * dx + dy * dy + dz * dz;
+            const float d2 = fmaf(dz, dz, fmaf(dy, dy, dx * dx));
             if (u + j < e && u + j != t && d2 <= g.bound) {
                 g.col[g.listed * g.stride] = u + j;
                 g.listed += 1;
@@ -299,7 +301,7 @@ __device__ __forceinline__ void contact_gather(const int32_t* __restrict__ start
     g.P = xyzr[t];
     g.query_radius = query_radius;
     g.rate = rate;
-    g.bound = nextafterf(query_radius * query_radius, __builtin_huge_valf());
+    g.bound = query_radius * query_radius * (1.0f + 0x1p-18f);
     g.col = list;
     g.stride = stride;
     g.listed = 0;
