@@ -141,20 +141,36 @@ def _solve_loop_matches(oracle, device):
     assert err < 1e-4
 
 
-def test_max_sweeps_and_pipelined_checks(oracle, device):
-    """Budget exhaustion mid-chunk and convergence inside a queued-ahead chunk."""
+@pytest.mark.parametrize("shape_name", ["tile60x8x8", "packed/2", "packed/4", "packed/8"])
+def test_max_sweeps_and_pipelined_checks(oracle, device, shape_name):
+    """Budget exhaustion mid-chunk, a last chunk shorter than the others, one sweep per check and
+    convergence inside a queued-ahead chunk, on the one-launch tile and on the colour-packed loop
+    (whose checks ride in the colour passes and whose red field alternates between two arrays)."""
+    from warpbiocell_amd import _lib
+    from warpbiocell_amd.autotune import config_shapes
+
     geom = GridGeometry.centered_cube(300.0, 20.0)
     kin = UptakeKinetics(7.2e6, 1.4e8, 3.4)
     a = oracle.SolveArgs(dx=20.0, diffusion=7.2e6, uptake_max=1.4e8, michaelis_k=3.4, boundary_value=38.0)
-    for max_sweeps, tol, check in ((25, 1e-9, 10), (2000, 1e-5, 7), (3, 1e-5, 1)):
-        f = ScalarField.create(geom, 38.0, initial_value=0.0, device=device)
-        ref = f.numpy()
-        density = np.zeros(geom.shape, np.float32)
-        rep = Dn.solve_steady_state(f, torch.from_numpy(density).to(device), kin,
-                                    SolverSettings(1.8, max_sweeps, tol, check))
-        exp = oracle.solve(ref, density, a, 1.8, max_sweeps, tol, check)
-        assert (rep.sweeps, np.float32(rep.residual), rep.converged) == (exp[0], np.float32(exp[1]), exp[2])
-        np.testing.assert_array_equal(f.numpy(), ref)
+    rng = np.random.default_rng(11)
+    previous = _lib.query("wbc_sor_fused_get_config")
+    _lib.call("wbc_sor_fused_set_config", config_shapes().index(shape_name))
+    try:
+        for max_sweeps, tol, check, dense in ((25, 1e-9, 10, False), (2000, 1e-5, 7, False), (3, 1e-5, 1, False),
+                                              (1, 1e-9, 1, True), (7, 1e-9, 3, True), (60, 2e-3, 1, True),
+                                              (2000, 1e-6, 10, True), (13, 1e-9, 5, True)):
+            f = ScalarField.create(geom, 38.0, initial_value=0.0, device=device)
+            ref = f.numpy()
+            density = (rng.random(geom.shape) * 2e-4).astype(np.float32) if dense else np.zeros(geom.shape, np.float32)
+            rep = Dn.solve_steady_state(f, torch.from_numpy(density).to(device), kin,
+                                        SolverSettings(1.8, max_sweeps, tol, check))
+            exp = oracle.solve(ref, density, a, 1.8, max_sweeps, tol, check)
+            case = (shape_name, max_sweeps, tol, check, dense)
+            assert (rep.sweeps, np.float32(rep.residual), rep.converged) == \
+                   (exp[0], np.float32(exp[1]), exp[2]), case
+            np.testing.assert_array_equal(f.numpy(), ref, err_msg=str(case))
+    finally:
+        _lib.call("wbc_sor_fused_set_config", previous)
 
 
 def test_cosh_profile_on_the_gpu(device):
