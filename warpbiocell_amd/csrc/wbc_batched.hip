@@ -559,6 +559,10 @@ __global__ void __launch_bounds__(256) batch_hg_order(const int32_t* __restrict_
     hg_order_point(key, start, tmp, position, radius, t, id, xyzr);
 }
 
+// With kNextKeys (before another substep) the moved cell's key and count for the next build are
+// produced here as well, from the floats the position array receives (the build's counts are back
+// at zero after its scatter, and its keys are no longer read).
+template <bool kNextKeys>
 __global__ void __launch_bounds__(256) batch_contact_kernel(const ReplicaDev* __restrict__ rep,
                                                             const int32_t* __restrict__ rep_of_slot,
                                                             const int32_t* __restrict__ start,
@@ -566,7 +570,9 @@ __global__ void __launch_bounds__(256) batch_contact_kernel(const ReplicaDev* __
                                                             const float4* __restrict__ xyzr, int32_t n, int32_t dim,
                                                             float* __restrict__ velocity,
                                                             int32_t* __restrict__ neighbor_count,
-                                                            float* __restrict__ position)
+                                                            float* __restrict__ position,
+                                                            int32_t* __restrict__ next_key,
+                                                            int32_t* __restrict__ next_cnt)
 {
     const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -583,9 +589,18 @@ __global__ void __launch_bounds__(256) batch_contact_kernel(const ReplicaDev* __
     velocity[3 * i + 1] = vy;
     velocity[3 * i + 2] = vz;
     neighbor_count[i] = count;
-    position[3 * i + 0] = P.x + vx * d.dt_mech;
-    position[3 * i + 1] = P.y + vy * d.dt_mech;
-    position[3 * i + 2] = P.z + vz * d.dt_mech;
+    const float x = P.x + vx * d.dt_mech, y = P.y + vy * d.dt_mech, z = P.z + vz * d.dt_mech;
+    position[3 * i + 0] = x;
+    position[3 * i + 1] = y;
+    position[3 * i + 2] = z;
+    if (kNextKeys) {
+        const int32_t b = r * dim * dim * dim +
+                          (wrap_dim(cell_coord(x, d.inv_cell), dim) * dim + wrap_dim(cell_coord(y, d.inv_cell), dim)) *
+                              dim +
+                          wrap_dim(cell_coord(z, d.inv_cell), dim);
+        next_key[i] = b;
+        atomicAdd(next_cnt + b, 1);
+    }
 }
 
 // ---- wide solve ------------------------------------------------------------------------------
@@ -1151,8 +1166,10 @@ static hipError_t enqueue_post(wbc_batch* b, hipStream_t s)
     int32_t* id = (int32_t*)(ws + L.id);
     float4* xyzr = (float4*)(ws + L.xyzr);
     for (int sub = 0; sub < b->substeps; ++sub) {
-        batch_hg_key_count<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, c.position, b->dim, key, cnt);
-        WBC_TRY_ERR(hipGetLastError());
+        if (sub == 0) {  // later substeps' keys come from the previous substep's gather
+            batch_hg_key_count<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, c.position, b->dim, key, cnt);
+            WBC_TRY_ERR(hipGetLastError());
+        }
         WBC_TRY_ERR(scan_i32(cnt, start, L.buckets, (int32_t*)(ws + L.scan), true, s));
         batch_hg_scatter<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, key, start, cnt, tmp,
                                                      (int32_t)L.buckets, L.active_total);
@@ -1160,8 +1177,14 @@ static hipError_t enqueue_post(wbc_batch* b, hipStream_t s)
         batch_hg_order<<<grid_for(L.active_total, 256), 256, 0, s>>>(key, L.active_total, start, tmp, c.position,
                                                                     c.radius, id, xyzr);
         WBC_TRY_ERR(hipGetLastError());
-        batch_contact_kernel<<<grid_for(L.active_total, 256), 256, 0, s>>>(
-            rep, rep_of_slot, start, id, xyzr, L.active_total, b->dim, c.velocity, c.neighbor_count, c.position);
+        if (sub + 1 < b->substeps)
+            batch_contact_kernel<true><<<grid_for(L.active_total, 256), 256, 0, s>>>(
+                rep, rep_of_slot, start, id, xyzr, L.active_total, b->dim, c.velocity, c.neighbor_count, c.position,
+                key, cnt);
+        else
+            batch_contact_kernel<false><<<grid_for(L.active_total, 256), 256, 0, s>>>(
+                rep, rep_of_slot, start, id, xyzr, L.active_total, b->dim, c.velocity, c.neighbor_count, c.position,
+                key, cnt);
         WBC_TRY_ERR(hipGetLastError());
     }
     return hipSuccess;
