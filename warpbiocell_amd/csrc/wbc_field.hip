@@ -413,8 +413,7 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
     zero_i32_kernel<<<1, 256, 0, stream>>>(state, 4);
     WBC_CHECK_LAUNCH();
     if (packed) {
-        WBC_TRY(launch_pack(p, field, density, pf, stream));
-        WBC_TRY(launch_copy_f32(red[0], red[1], packed_floats(p), stream));  // the fixed face values of both
+        WBC_TRY(launch_pack(p, field, density, pf, stream, red[1]));  // both red arrays: their face values
     }
 
     auto chunk_len = [&](int32_t c) {

@@ -74,7 +74,9 @@ inline int64_t packed_floats_of(int32_t nx, int32_t ny, int32_t nz)
 }
 bool packed_supported(const SorParams& p);
 inline int64_t packed_floats(const SorParams& p) { return packed_floats_of(p.nx, p.ny, p.nz); }
-hipError_t launch_pack(const SorParams& p, const float* field, const float* density, PackedField& pf, hipStream_t s);
+// f0_copy (optional): also write the colour-0 field there.
+hipError_t launch_pack(const SorParams& p, const float* field, const float* density, PackedField& pf, hipStream_t s,
+                       float* f0_copy = nullptr);
 hipError_t launch_unpack(const SorParams& p, const PackedField& pf, float* field, hipStream_t s);
 // One sweep (both colour passes); each thread takes `rows` (2, 4 or 8) consecutive rows j.
 hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, const int32_t* done, hipStream_t s);

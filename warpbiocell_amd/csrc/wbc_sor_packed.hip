@@ -20,10 +20,11 @@
 
 namespace wbc {
 
+// f0_copy (optional): a second copy of the colour-0 field.
 __global__ void __launch_bounds__(256) pack_kernel(const float* __restrict__ field, const float* __restrict__ density,
                                                    float* __restrict__ f0, float* __restrict__ f1,
-                                                   float* __restrict__ d0, float* __restrict__ d1, int nx, int ny,
-                                                   int nz)
+                                                   float* __restrict__ d0, float* __restrict__ d1,
+                                                   float* __restrict__ f0_copy, int nx, int ny, int nz)
 {
     const int nz2 = packed_nz2(nz);
     const int64_t nodes = (int64_t)nx * ny * nz;
@@ -35,6 +36,7 @@ __global__ void __launch_bounds__(256) pack_kernel(const float* __restrict__ fie
         if (((i + j + k) & 1) == 0) {
             f0[e] = field[t];
             d0[e] = density[t];
+            if (f0_copy) f0_copy[e] = field[t];
         } else {
             f1[e] = field[t];
             d1[e] = density[t];
@@ -100,11 +102,12 @@ bool packed_supported(const SorParams& p)
            packed_lanes_per_plane(p, 2) >= 0 && packed_floats(p) < ((int64_t)1 << 31);
 }
 
-hipError_t launch_pack(const SorParams& p, const float* field, const float* density, PackedField& pf, hipStream_t s)
+hipError_t launch_pack(const SorParams& p, const float* field, const float* density, PackedField& pf, hipStream_t s,
+                       float* f0_copy)
 {
     const int64_t nodes = (int64_t)p.nx * p.ny * p.nz;
     pack_kernel<<<grid_for(nodes < 2048 * 256 ? nodes : 2048 * 256, 256), 256, 0, s>>>(
-        field, density, pf.f[0], pf.f[1], pf.d[0], pf.d[1], p.nx, p.ny, p.nz);
+        field, density, pf.f[0], pf.f[1], pf.d[0], pf.d[1], f0_copy, p.nx, p.ny, p.nz);
     return hipGetLastError();
 }
 
