@@ -334,7 +334,7 @@ def main() -> int:
                     help="only run a plain 256 MiB device copy 5 times (counter calibration)")
     args = ap.parse_args()
     if args.gpus != 1 or int(os.environ.get("WORLD_SIZE", "1")) > 1:
-        print(json.dumps({"error": "this build runs the hot path on one GPU; --gpus above 1 is not supported",
+        print(json.dumps({"error": "bench.py runs the hot path on one GPU; --gpus above 1 is not supported",
                           "n_gpus": args.gpus}))
         return 2
 

@@ -36,7 +36,7 @@ def save_checkpoint(path: str | Path, population: CellPopulation, oxygen, time_h
                     network=None) -> Path:
     """Write ``path`` (compressed ``.npz``) in the reference's layout; returns the path."""
     if network is not None:
-        raise NotImplementedError("gene networks are not part of this build")
+        raise NotImplementedError("gene networks are not part of this package")
     path = Path(path)
     path.parent.mkdir(parents=True, exist_ok=True)
     n = population.count

@@ -154,10 +154,10 @@ def oxygen_summary(population: CellPopulation, oxygen) -> dict[str, float]:
 
 def species_summary(population: CellPopulation, fields) -> dict[str, float]:
     """Mean/min/max at living cells of every species beyond oxygen (empty for a plain OxygenField,
-    the only field this build solves)."""
+    the only field this package solves)."""
     names = [n for n in getattr(fields, "names", []) if n != "oxygen"]
     if names:
-        raise NotImplementedError("species beyond oxygen are not solved by this build")
+        raise NotImplementedError("species beyond oxygen are not solved by this package")
     return {}
 
 
