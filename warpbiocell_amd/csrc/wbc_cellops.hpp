@@ -287,11 +287,12 @@ __device__ __forceinline__ void gather_run(const float4* __restrict__ xyzr, int3
 // bucket starts of this grid and `xyzr` its bucket-ordered (x, y, z, r); `list` is this lane's
 // LDS column of kGatherList ints, `stride` apart.
 //
-// The walk is a list of runs of consecutive slots: one per (x, y) row when the row's z buckets
-// are consecutive keys (no wrap at dim), else one per bucket. In the common case (the query
-// radius equal to the bucket edge: 3 x 3 rows, no wrap) the 18 run bounds are loaded together
-// up front; otherwise each run's bounds are loaded as the walk reaches it. The walk lists the
-// candidates inside the prefilter bound; the drain computes their terms in walk order.
+// The walk is a list of runs of consecutive slots. In the common case (the query radius equal to
+// the bucket edge: 3 x 3 rows) each row is one run, or two where its z buckets wrap at dim, and
+// the 36 run bounds are loaded together up front; otherwise (one run per row when the row's z
+// buckets are consecutive keys, else one per bucket) each run's bounds are loaded as the walk
+// reaches it. The walk lists the candidates inside the prefilter bound; the drain computes their
+// terms in walk order.
 __device__ __forceinline__ void contact_gather(const int32_t* __restrict__ start, const float4* __restrict__ xyzr,
                                                int32_t t, float inv_cell, int32_t dim, float query_radius,
                                                float rate, int32_t* list, int32_t stride, float& vx, float& vy,
@@ -315,24 +316,33 @@ __device__ __forceinline__ void contact_gather(const int32_t* __restrict__ start
     const int32_t z0 = cell_coord(P.z - query_radius, inv_cell), z1 = cell_coord(P.z + query_radius, inv_cell);
     const int32_t wz0 = wrap_dim(z0, dim);
     const bool z_contiguous = wz0 + (z1 - z0) < dim;  // the z buckets are consecutive keys
-    if (z_contiguous && x1 - x0 == 2 && y1 - y0 == 2) {
-        int32_t S[9], E[9];
+    if (x1 - x0 == 2 && y1 - y0 == 2 && z1 - z0 < dim) {
+        // A row's z buckets in walk order, wz0, wz0 + 1, ..., are one run of consecutive keys, or
+        // two when they pass dim - 1 (wz0 .. dim - 1, then 0 ..); the second run is empty otherwise.
+        const int32_t nz = z1 - z0 + 1;
+        const int32_t len_a = min(nz, dim - wz0), len_b = nz - len_a;
+        int32_t S[9], E[9], SB[9], EB[9];
 #pragma unroll
         for (int a = 0; a < 3; ++a)
 #pragma unroll
             for (int b = 0; b < 3; ++b) {
-                const int32_t row = (wrap_dim(x0 + a, dim) * dim + wrap_dim(y0 + b, dim)) * dim + wz0;
-                S[3 * a + b] = start[row];
-                E[3 * a + b] = start[row + (z1 - z0) + 1];
+                const int32_t row = (wrap_dim(x0 + a, dim) * dim + wrap_dim(y0 + b, dim)) * dim;
+                S[3 * a + b] = start[row + wz0];
+                E[3 * a + b] = start[row + wz0 + len_a];
+                SB[3 * a + b] = len_b ? start[row] : 0;
+                EB[3 * a + b] = len_b ? start[row + len_b] : 0;
             }
         for (int r = 0; r < 9; ++r) {
-            const int32_t s = S[0], e = E[0];
+            const int32_t s = S[0], e = E[0], sb = SB[0], eb = EB[0];
 #pragma unroll
             for (int k = 0; k < 8; ++k) {
                 S[k] = S[k + 1];
                 E[k] = E[k + 1];
+                SB[k] = SB[k + 1];
+                EB[k] = EB[k + 1];
             }
             gather_run(xyzr, s, e, t, g);
+            gather_run(xyzr, sb, eb, t, g);
         }
     } else {
         for (int32_t cx = x0; cx <= x1; ++cx) {
