@@ -113,8 +113,9 @@ def test_runs_are_bitwise_reproducible(device):
 
 @pytest.mark.parametrize("factor,dim", [(0.45, 128), (0.7, 128), (1.3, 128), (2.5, 128), (0.7, 7), (1.0, 5)])
 def test_other_bucket_sizes_take_the_general_walk(oracle, device, factor, dim):
-    """Bucket edge != query radius (or wrapping buckets): the walk's per-run path instead of the
-    preloaded 3 x 3 rows; counts and velocities equal the oracle's bits."""
+    """Bucket edge != query radius: the walk's per-run path instead of the preloaded 3 x 3 rows
+    (the (1.0, 5) case keeps the 3 x 3 rows with buckets wrapping in every axis); counts and
+    velocities equal the oracle's bits."""
     rng = np.random.default_rng(int(factor * 100) + dim)
     pos = spherical_cluster(1500, 8.0, spacing_factor=0.9, jitter=0.3, seed=int(factor * 10))
     radii = rng.uniform(5.0, 9.0, 1500).astype(np.float32)
@@ -126,6 +127,33 @@ def test_other_bucket_sizes_take_the_general_walk(oracle, device, factor, dim):
     vel, cnt = oracle.contact_velocities(pos, radii, cell, dim, q, params.rate)
     np.testing.assert_array_equal(pop.neighbor_counts_numpy(), cnt)
     np.testing.assert_array_equal(pop.velocity.cpu().numpy()[:1500], vel)
+
+
+@pytest.mark.parametrize("dim", [3, 4, 6, 128])
+def test_rows_whose_z_buckets_wrap_walk_as_two_runs(oracle, device, dim):
+    """Bucket edge equal to the query radius, cluster centred on the origin: many cells' z search
+    range crosses the dim boundary (for small dim, in every axis). The 3 x 3-row walk takes such a
+    row as two runs. Counts, velocities and three moving substeps equal the oracle's bits."""
+    pos = spherical_cluster(1500, 8.0, spacing_factor=0.9, jitter=0.3, seed=11)
+    radii = np.random.default_rng(dim).uniform(5.0, 9.0, 1500).astype(np.float32)
+    params = ContactParams(stiffness=1.0, damping=1.0, max_radius=9.0, margin=0.5)
+    q = np.float32(params.query_radius)
+    inv = np.float32(1.0) / q
+    lo = np.floor((pos - q) * inv).astype(np.int64)
+    hi = np.floor((pos + q) * inv).astype(np.int64)
+    rows3x3 = (hi[:, 0] - lo[:, 0] == 2) & (hi[:, 1] - lo[:, 1] == 2) & (hi[:, 2] - lo[:, 2] < dim)
+    z_wraps = np.mod(lo[:, 2], dim) + (hi[:, 2] - lo[:, 2]) >= dim
+    assert (rows3x3 & z_wraps).sum() > 50
+    pop = CellPopulation.from_numpy(pos, radii, device=device)
+    contact_substep(pop, NeighborGrid(float(q), dim=dim, device=device), params, dt=0.0)
+    vel, cnt = oracle.contact_velocities(pos, radii, q, dim, q, params.rate)
+    np.testing.assert_array_equal(pop.neighbor_counts_numpy(), cnt)
+    np.testing.assert_array_equal(pop.velocity.cpu().numpy()[:1500], vel)
+    relax_contacts(pop, NeighborGrid(float(q), dim=dim, device=device), params, dt=0.05, substeps=3)
+    ref, v3, c3 = pos.copy(), np.zeros_like(pos), np.zeros(1500, np.int32)
+    oracle.relax_contacts(ref, radii, q, dim, q, params.rate, 0.05, 3, v3, c3)
+    np.testing.assert_array_equal(pop.positions_numpy(), ref)
+    np.testing.assert_array_equal(pop.neighbor_counts_numpy(), c3)
 
 
 def test_dense_packing_drains_the_candidate_lists_in_order(oracle, device):
