@@ -210,12 +210,14 @@ __device__ __forceinline__ int32_t bucket_of(const float* __restrict__ position,
 
 // Candidates per load group of the contact gather: a lane issues this many neighbour loads
 // before it uses the first, instead of one load and one wait per candidate. LDS slots per lane
-// for the candidates that pass the distance prefilter (32 KB for a 256-thread workgroup).
+// for the candidates that pass the distance prefilter (36 KB for a 256-thread workgroup).
 // Measured at 100k cells, whole substep (build included), group/slots: 4/16 102 us, 4/32 91,
 // 4/64 91, 8/16 100, 8/24 95, 8/32 88, 8/48 87, 16/32 109, 16/48 102, 2/32 97; one pass without
-// the list: group 4 110 us, group 1 134 us.
-constexpr int kGatherGroup = 8;
-constexpr int kGatherList = 32;
+// the list: group 4 110 us, group 1 134 us. Again once every lane walked the 3 x 3 rows (wrapped
+// rows as two runs): 8/32 66.9 us, 6/36 66.3, 6/42 66.3, 6/48 66.2, 6/30 66.9, 8/48 66.9,
+// 7/35 67.6, 5/35 67.8, 4/32 68.5, 8/24 73.9, 4/24 76.4.
+constexpr int kGatherGroup = 6;
+constexpr int kGatherList = 36;
 
 // The contact terms of the candidates a lane has listed, in list order: the reference's exact
 // test and arithmetic (kernels/mechanics_kernels.py:45-57).
