@@ -81,6 +81,13 @@ int wbc_fill_zero_i64(int64_t* p, int64_t count, wbc_stream_t stream);
 int wbc_deposit_trilinear_i64(const float* position, const int32_t* cell_state, int64_t n, float ox, float oy,
                               float oz, float inv_dx, int32_t nx, int32_t ny, int32_t nz, int64_t* accumulator,
                               wbc_stream_t stream);
+/* No counterpart in the reference: the same deposit with thread t taking cell order[t], where
+ * order[0, n) is a permutation of [0, n) (such as the hash grid's bucket order after a contact
+ * relaxation). The accumulator equals wbc_deposit_trilinear_i64's to the bit, since int64
+ * addition is exact and commutative; a spatially coherent order halves the kernel's time. */
+int wbc_deposit_trilinear_i64_ordered(const float* position, const int32_t* cell_state, const int32_t* order,
+                                      int64_t n, float ox, float oy, float oz, float inv_dx, int32_t nx, int32_t ny,
+                                      int32_t nz, int64_t* accumulator, wbc_stream_t stream);
 int wbc_density_finalize(const int64_t* accumulator, int64_t nodes, float inv_voxel_volume, float w0, float w1,
                          float w2, float w3, float* combined, float* per_state, wbc_stream_t stream);
 /* Replaces CellDensities.combine, fields/densities.py:45-50 (combine_densities). */

@@ -41,6 +41,54 @@ def test_deposit_accumulator_is_bit_exact_and_order_independent(oracle, device):
     np.testing.assert_array_equal(dens.density.cpu().numpy(), oracle.fixed_point_to_density(expected, geom.dx))
 
 
+def test_deposit_in_any_cell_order_gives_the_same_accumulator(oracle, device):
+    """wbc_deposit_trilinear_i64_ordered visits cells in a given permutation of the live slots:
+    reversed, random and the hash grid's bucket order (recorded by a contact relaxation) all give
+    the oracle's accumulator bit for bit. Adding cells invalidates the recorded order, and the
+    deposit falls back to slot order."""
+    from warpbiocell_amd import _lib
+    from warpbiocell_amd.cells.lifecycle import LifecycleParams, commit_divisions
+    from warpbiocell_amd.cells.mechanics import ContactParams, make_neighbor_grid, relax_contacts
+
+    pos, states, pop = _cluster_with_states(5000, 23, device, outside=41)
+    geom = GridGeometry.centered_cube(600.0, 20.0)
+    dens = CellDensities(geom, device=device)
+    expected = oracle.deposit_accumulator(pos, states, geom.origin, geom.dx, geom.shape)
+    nx, ny, nz = geom.shape
+    origin, inv_dx, _ = dens._scalars()
+    acc = torch.empty_like(dens.accumulator)
+    perms = {"reversed": np.arange(5000)[::-1], "random": np.random.default_rng(4).permutation(5000)}
+    for name, perm in perms.items():
+        order = torch.from_numpy(np.ascontiguousarray(perm, dtype=np.int32)).to(device)
+        acc.zero_()
+        _lib.call("wbc_deposit_trilinear_i64_ordered", pop.position.data_ptr(), pop.cell_state.data_ptr(),
+                  order.data_ptr(), 5000, origin[0], origin[1], origin[2], inv_dx, nx, ny, nz, acc.data_ptr(),
+                  pop.stream)
+        np.testing.assert_array_equal(acc.cpu().numpy(), expected, err_msg=name)
+
+    assert pop.deposit_order() is None
+    params = ContactParams(stiffness=1.0, damping=1.0, max_radius=8.0, margin=0.5)
+    relax_contacts(pop, make_neighbor_grid(params, device), params, dt=0.01, substeps=2)
+    order = pop.deposit_order()
+    assert order is not None
+    assert sorted(order[:5000].cpu().numpy().tolist()) == list(range(5000))
+    assert not np.array_equal(order[:5000].cpu().numpy(), np.arange(5000))
+    moved = pop.positions_numpy()
+    dens.deposit(pop)
+    np.testing.assert_array_equal(dens.accumulator.cpu().numpy(),
+                                  oracle.deposit_accumulator(moved, states, geom.origin, geom.dx, geom.shape))
+
+    pop.divide_flag[:5000] = 0
+    pop.divide_flag[:3] = 1
+    assert commit_divisions(pop, LifecycleParams()) == 3
+    assert pop.deposit_order() is None
+    dens.deposit(pop)
+    grown = pop.positions_numpy()
+    np.testing.assert_array_equal(dens.accumulator.cpu().numpy(),
+                                  oracle.deposit_accumulator(grown, pop.cell_state.cpu().numpy()[:5003], geom.origin,
+                                                             geom.dx, geom.shape))
+
+
 def test_fused_density_combine_is_bit_exact(oracle, device):
     pos, states, pop = _cluster_with_states(3000, 5, device)
     geom = GridGeometry.centered_cube(500.0, 12.5)

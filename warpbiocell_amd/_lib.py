@@ -32,6 +32,7 @@ SIGNATURES: dict[str, tuple] = {
     "wbc_commit_divisions": (I32, [I32, I64, F32, P, P, P, c_size_t, P, P, P, P, P, P, P, P, P, P, P, P, P]),
     "wbc_fill_zero_i64": (I32, [P, I64, P]),
     "wbc_deposit_trilinear_i64": (I32, [P, P, I64, F32, F32, F32, F32, I32, I32, I32, P, P]),
+    "wbc_deposit_trilinear_i64_ordered": (I32, [P, P, P, I64, F32, F32, F32, F32, I32, I32, I32, P, P]),
     "wbc_density_finalize": (I32, [P, I64, F32, F32, F32, F32, F32, P, P, P]),
     "wbc_combine_densities": (I32, [P, I64, F32, F32, F32, F32, P, P]),
     "wbc_sample_trilinear": (I32, [P, I32, I32, I32, P, I64, F32, F32, F32, F32, P, P]),

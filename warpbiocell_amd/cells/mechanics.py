@@ -111,3 +111,6 @@ def relax_contacts(population, grid: NeighborGrid, params: ContactParams, dt: fl
                   params.wall_rate, population.velocity.data_ptr(), population.neighbor_count.data_ptr(),
                   grid.capacity, ws.data_ptr(), ws.numel(), stream)
     grid._built_count = n
+    # The last build's bucket order: the next deposit visits the cells in it (same bits, nearby
+    # atomics).
+    population.record_deposit_order(grid._point_ids_view())

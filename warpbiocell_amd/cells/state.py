@@ -41,6 +41,8 @@ class CellPopulation:
     fate_flags: torch.Tensor  # int32
     extra_local: dict = field(default_factory=dict)
     _scan_workspace: torch.Tensor | None = None
+    _deposit_order: torch.Tensor | None = None  # int32 (capacity,): a permutation of [0, count)
+    _deposit_order_count: int = -1
 
     @classmethod
     def from_numpy(cls, positions: np.ndarray, radii: np.ndarray, capacity: int | None = None, device=None,
@@ -95,6 +97,25 @@ class CellPopulation:
     @property
     def stream(self) -> int:
         return _lib.stream_of(self.device)
+
+    def record_deposit_order(self, order: torch.Tensor) -> None:
+        """Keep a copy of ``order`` (int32, a permutation of the live slots [0, count), such as the
+        hash grid's bucket order after a contact relaxation) as the order the deposit visits cells
+        in. Any permutation gives the same int64 accumulator; a spatially coherent one halves the
+        deposit kernel's time."""
+        n = self.count
+        if order.dtype != torch.int32 or order.numel() != n or order.device != self.device:
+            raise ValueError(f"the order must be {n} int32 values on {self.device}")
+        if self._deposit_order is None:
+            self._deposit_order = torch.empty(self.capacity, dtype=torch.int32, device=self.device)
+        self._deposit_order[:n].copy_(order)
+        self._deposit_order_count = n
+
+    def deposit_order(self) -> torch.Tensor | None:
+        """The recorded order while it covers exactly the live slots (no cells added since), else None."""
+        if self.count > 0 and self._deposit_order_count == self.count:
+            return self._deposit_order
+        return None
 
     def scan_workspace(self) -> torch.Tensor:
         """Tile sums of the division-slot scan, sized for the capacity (allocated once)."""

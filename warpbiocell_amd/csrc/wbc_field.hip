@@ -15,14 +15,19 @@ namespace wbc {
 // Deposit: 8 int64 atomics per cell; integer addition is associative, so the accumulator does
 // not depend on the order in which the atomics land.
 // --------------------------------------------------------------------------------------------
+// Thread t deposits cell order[t] (cell t without an order). A spatially coherent order, such as
+// the hash grid's bucket order, puts a wave's atomics on nearby nodes: 100k cells on a 161^3 grid,
+// 53.1 us in slot order, 29.0-30.0 us in bucket or node order.
 __global__ void __launch_bounds__(256) deposit_trilinear_kernel(const float* __restrict__ position,
-                                                                 const int32_t* __restrict__ cell_state, int64_t n,
+                                                                 const int32_t* __restrict__ cell_state,
+                                                                 const int32_t* __restrict__ order, int64_t n,
                                                                  float ox, float oy, float oz, float inv_dx,
                                                                  int32_t nx, int32_t ny, int32_t nz,
                                                                  unsigned long long* __restrict__ acc)
 {
-    int64_t c = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (c >= n) return;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (t >= n) return;
+    const int64_t c = order ? (int64_t)order[t] : t;
     deposit_cell(position, c, cell_state[c], ox, oy, oz, inv_dx, nx, ny, nz, acc);
 }
 
@@ -168,8 +173,25 @@ WBC_API int wbc_deposit_trilinear_i64(const float* position, const int32_t* cell
     if (n < 0 || n > INT32_MAX || !grid_ok(nx, ny, nz)) return (int)hipErrorInvalidValue;
     if (n == 0) return 0;
     if (!position || !cell_state || !accumulator) return (int)hipErrorInvalidValue;
-    deposit_trilinear_kernel<<<grid_for(n, 256), 256, 0, stream>>>(position, cell_state, n, ox, oy, oz, inv_dx, nx,
-                                                                    ny, nz, (unsigned long long*)accumulator);
+    deposit_trilinear_kernel<<<grid_for(n, 256), 256, 0, stream>>>(position, cell_state, nullptr, n, ox, oy, oz,
+                                                                    inv_dx, nx, ny, nz,
+                                                                    (unsigned long long*)accumulator);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+// The same deposit with thread t taking cell order[t]; order[0, n) must be a permutation of
+// [0, n). The accumulator is the same to the bit (int64 addition is exact and commutative).
+WBC_API int wbc_deposit_trilinear_i64_ordered(const float* position, const int32_t* cell_state,
+                                              const int32_t* order, int64_t n, float ox, float oy, float oz,
+                                              float inv_dx, int32_t nx, int32_t ny, int32_t nz, int64_t* accumulator,
+                                              hipStream_t stream)
+{
+    if (n < 0 || n > INT32_MAX || !grid_ok(nx, ny, nz)) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!position || !cell_state || !order || !accumulator) return (int)hipErrorInvalidValue;
+    deposit_trilinear_kernel<<<grid_for(n, 256), 256, 0, stream>>>(position, cell_state, order, n, ox, oy, oz, inv_dx,
+                                                                    nx, ny, nz, (unsigned long long*)accumulator);
     WBC_CHECK_LAUNCH();
     return 0;
 }

@@ -64,13 +64,17 @@ class NeighborGrid:
 
     def point_ids(self) -> torch.Tensor:
         """Slot ids of the last build in bucket order (``hash_grid_point_id`` order)."""
+        return self._point_ids_view().clone()
+
+    def _point_ids_view(self) -> torch.Tensor:
+        """The same ids as a view of the workspace (valid until the next build)."""
         import ctypes
 
         pid, starts = ctypes.c_void_p(), ctypes.c_void_p()
         _lib.call("wbc_hashgrid_point_ids", self.workspace.data_ptr(), self.workspace.numel(), self.dim,
                   self._capacity, ctypes.byref(pid), ctypes.byref(starts))
         offset = pid.value - self.workspace.data_ptr()
-        return self.workspace[offset: offset + 4 * self._built_count].view(torch.int32).clone()
+        return self.workspace[offset: offset + 4 * self._built_count].view(torch.int32)
 
     def bucket_starts(self) -> torch.Tensor:
         """The dim**3 + 1 bucket starts of the last build."""
