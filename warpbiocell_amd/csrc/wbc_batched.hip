@@ -462,12 +462,23 @@ __global__ void __launch_bounds__(256) batch_zero_i64_kernel(long long* __restri
 
 __device__ __forceinline__ bool slot_active(const ReplicaDev& d, int64_t g) { return g - d.cell_offset < d.count; }
 
+// Once a contact build has run (*order_ready), thread t deposits the cell at bucket-ordered
+// position t of that build, id[t], a permutation of the active slots (the replicas add no cells):
+// the same int64 sums, with a wave's atomics on nearby nodes. Before it, thread t takes slot t.
 __global__ void __launch_bounds__(256) batch_deposit_kernel(const ReplicaDev* __restrict__ rep,
                                                             const int32_t* __restrict__ rep_of_slot, int64_t slots,
+                                                            const int32_t* __restrict__ id,
+                                                            const int32_t* __restrict__ order_ready,
+                                                            int32_t active_total,
                                                             const float* __restrict__ position,
                                                             const int32_t* __restrict__ cell_state)
 {
-    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    int64_t g = t;
+    if (*order_ready) {
+        if (t >= active_total) return;
+        g = id[t];
+    }
     if (g >= slots) return;
     const ReplicaDev& d = rep[rep_of_slot[g]];
     if (!slot_active(d, g)) return;
@@ -552,9 +563,10 @@ __global__ void __launch_bounds__(256) batch_hg_order(const int32_t* __restrict_
                                                       const int32_t* __restrict__ tmp,
                                                       const float* __restrict__ position,
                                                       const float* __restrict__ radius, int32_t* __restrict__ id,
-                                                      float4* __restrict__ xyzr)
+                                                      float4* __restrict__ xyzr, int32_t* __restrict__ order_ready)
 {
     const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    if (t == 0) *order_ready = 1;  // id holds a bucket order of the active slots from here on
     if (t >= n) return;
     hg_order_point(key, start, tmp, position, radius, t, id, xyzr);
 }
@@ -821,7 +833,7 @@ static int tiles_for(int tile, int nx, int ny, int nz)
 // ---- workspace layout -----------------------------------------------------------------------
 struct BatchLayout {
     size_t rep, state, bar, node_off, total_sweeps, partial, acc, density, buf1, buf2, rep_of_slot, cnt, start,
-        scan, key, tmp, id, xyzr, wrep, rblk_off, wdone, wpartial, packed, total;
+        scan, key, tmp, id, order_ready, xyzr, wrep, rblk_off, wdone, wpartial, packed, total;
     int64_t slots, nodes, buckets;
     int32_t partial_stride, active_total, wide_res_blocks;
     bool wide_ok;  // every replica's grid fits the colour-packed passes
@@ -902,6 +914,7 @@ static bool layout_for(const wbc_replica* r, int32_t R, int32_t dim, BatchLayout
     L.key = off;          off += up(sizeof(int32_t) * slots);
     L.tmp = off;          off += up(sizeof(int32_t) * slots);
     L.id = off;           off += up(sizeof(int32_t) * slots);
+    L.order_ready = off;  off += up(sizeof(int32_t));
     L.xyzr = off;         off += up(sizeof(float4) * slots);
     L.wrep = off;         off += up(sizeof(WideRep) * R);
     L.rblk_off = off;     off += up(sizeof(int32_t) * (R + 1));
@@ -1133,7 +1146,9 @@ static hipError_t enqueue_pre(wbc_batch* b, hipStream_t s)
     batch_zero_i64_kernel<<<1024, 256, 0, s>>>((long long*)(ws + L.acc), 4 * L.nodes);
     WBC_TRY_ERR(hipGetLastError());
     if (L.slots > 0) {
-        batch_deposit_kernel<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, c.position, c.cell_state);
+        batch_deposit_kernel<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, (const int32_t*)(ws + L.id),
+                                                         (const int32_t*)(ws + L.order_ready), L.active_total,
+                                                         c.position, c.cell_state);
         WBC_TRY_ERR(hipGetLastError());
     }
     batch_finalize_kernel<<<grid_for(L.nodes, 256), 256, 0, s>>>(rep, b->R, (const int64_t*)(ws + L.node_off),
@@ -1175,7 +1190,8 @@ static hipError_t enqueue_post(wbc_batch* b, hipStream_t s)
                                                      (int32_t)L.buckets, L.active_total);
         WBC_TRY_ERR(hipGetLastError());
         batch_hg_order<<<grid_for(L.active_total, 256), 256, 0, s>>>(key, L.active_total, start, tmp, c.position,
-                                                                    c.radius, id, xyzr);
+                                                                    c.radius, id, xyzr,
+                                                                    (int32_t*)(ws + L.order_ready));
         WBC_TRY_ERR(hipGetLastError());
         if (sub + 1 < b->substeps)
             batch_contact_kernel<true><<<grid_for(L.active_total, 256), 256, 0, s>>>(
@@ -1322,6 +1338,7 @@ WBC_API int wbc_batch_create(const wbc_replica* replicas, int32_t count, int32_t
                                hipMemcpyHostToDevice, stream));
     WBC_TRY(hipMemsetAsync(ws + L.total_sweeps, 0, sizeof(long long) * count, stream));
     WBC_TRY(hipMemsetAsync(ws + L.bar, 0, sizeof(BarrierState), stream));
+    WBC_TRY(hipMemsetAsync(ws + L.order_ready, 0, sizeof(int32_t), stream));
     WBC_TRY(hipMemsetAsync(ws + L.cnt, 0, sizeof(int32_t) * L.buckets, stream));
     WBC_TRY(hipMemsetAsync(ws + L.density, 0, sizeof(float) * L.nodes, stream));
     WBC_TRY(hipStreamSynchronize(stream));  // the host vectors above go out of scope
