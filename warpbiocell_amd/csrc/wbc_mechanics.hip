@@ -54,13 +54,16 @@ __global__ void __launch_bounds__(256) hg_key_count(const float* __restrict__ po
     atomicAdd(cnt + b, 1);
 }
 
-// Also sets start[nb] = n, the end of the last bucket (the scan writes start[0, nb)).
+// Also sets start[nb] = n, the end of the last bucket (the scan writes start[0, nb)), and clears
+// the scan's tile sums (tile_sums[0, tiles]), which the next build's key producer adds into.
 __global__ void __launch_bounds__(256) hg_scatter(const int32_t* __restrict__ key, int32_t n,
                                                    int32_t* __restrict__ start, int32_t* __restrict__ cnt,
-                                                   int32_t* __restrict__ tmp, int64_t nb)
+                                                   int32_t* __restrict__ tmp, int64_t nb,
+                                                   int32_t* __restrict__ tile_sums, int32_t tiles)
 {
     int32_t i = blockIdx.x * blockDim.x + threadIdx.x;
     if (i == 0) start[nb] = n;
+    for (int32_t k = i; k <= tiles; k += gridDim.x * blockDim.x) tile_sums[k] = 0;
     if (i >= n) return;
     int32_t b = key[i];
     int32_t left = atomicSub(cnt + b, 1);
@@ -95,7 +98,8 @@ __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict_
                                                        float* __restrict__ velocity,
                                                        int32_t* __restrict__ neighbor_count,
                                                        float* __restrict__ position, int32_t* __restrict__ next_key,
-                                                       int32_t* __restrict__ next_cnt)
+                                                       int32_t* __restrict__ next_cnt,
+                                                       int32_t* __restrict__ next_tile_sums)
 {
     int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -123,6 +127,17 @@ __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict_
                               wrap_dim(cell_coord(z, inv_cell), dim);
             next_key[i] = b;
             atomicAdd(next_cnt + b, 1);
+            // The next build's scan tile sums: one atomic per distinct tile among the wave's lanes
+            // (bucket-ordered lanes mostly share one), so that build skips its tile-sum pass.
+            const int32_t tile = b / kScanTile;
+            uint64_t pending = __ballot(1);
+            while (pending) {
+                const int leader = __ffsll((unsigned long long)pending) - 1;
+                const int32_t lead_tile = __shfl(tile, leader);
+                const uint64_t same = __ballot(tile == lead_tile);
+                if ((int)__lane_id() == leader) atomicAdd(next_tile_sums + lead_tile, (int32_t)__popcll(same));
+                pending &= ~same;
+            }
         }
     }
 }
@@ -191,7 +206,7 @@ __global__ void __launch_bounds__(256) zero_i32_kernel_mech(int32_t* __restrict_
 }
 
 static hipError_t hashgrid_build_from_keys(const float* position, const float* radius, int32_t n, int32_t dim,
-                                           char* ws, const HashGridLayout& L, hipStream_t s);
+                                           char* ws, const HashGridLayout& L, hipStream_t s, bool presummed);
 
 static hipError_t hashgrid_build(const float* position, const float* radius, int32_t n, float inv_cell,
                                  int32_t dim, char* ws, const HashGridLayout& L, hipStream_t s)
@@ -200,20 +215,26 @@ static hipError_t hashgrid_build(const float* position, const float* radius, int
     int32_t* key = (int32_t*)(ws + L.key);
     hg_key_count<<<grid_for(n, 256), 256, 0, s>>>(position, n, inv_cell, dim, key, cnt);
     WBC_TRY_ERR(hipGetLastError());
-    return hashgrid_build_from_keys(position, radius, n, dim, ws, L, s);
+    return hashgrid_build_from_keys(position, radius, n, dim, ws, L, s, false);
 }
 
-// Steps 2-4 of the build, from keys and counts already in the workspace.
+// Steps 2-4 of the build, from keys and counts already in the workspace; with `presummed` the
+// scan's tile sums are there too (added up by the gather that produced the keys).
 static hipError_t hashgrid_build_from_keys(const float* position, const float* radius, int32_t n, int32_t dim,
-                                           char* ws, const HashGridLayout& L, hipStream_t s)
+                                           char* ws, const HashGridLayout& L, hipStream_t s, bool presummed)
 {
     const int64_t nb = (int64_t)dim * dim * dim;
     int32_t* cnt = (int32_t*)(ws + L.cnt);
     int32_t* start = (int32_t*)(ws + L.start);
     int32_t* key = (int32_t*)(ws + L.key);
     int32_t* tmp = (int32_t*)(ws + L.tmp);
-    WBC_TRY_ERR(scan_i32(cnt, start, nb, (int32_t*)(ws + L.partial), true, s));
-    hg_scatter<<<grid_for(n, 256), 256, 0, s>>>(key, n, start, cnt, tmp, nb);
+    int32_t* tile_sums = (int32_t*)(ws + L.partial);
+    if (presummed)
+        WBC_TRY_ERR(scan_i32_presummed(cnt, start, nb, tile_sums, true, s));
+    else
+        WBC_TRY_ERR(scan_i32(cnt, start, nb, tile_sums, true, s));
+    const int32_t tiles = (int32_t)((nb + kScanTile - 1) / kScanTile);
+    hg_scatter<<<grid_for(n, 256), 256, 0, s>>>(key, n, start, cnt, tmp, nb, tile_sums, tiles);
     hg_order<<<grid_for(n, 256), 256, 0, s>>>(key, n, start, tmp, position, radius, (int32_t*)(ws + L.id),
                                               (float4*)(ws + L.xyzr));
     return hipGetLastError();
@@ -316,7 +337,8 @@ static int launch_contact(bool integrate, bool next_keys, const void* workspace,
     const bool confined = region_of(region, g);
     auto go = [&](auto kernel) {
         kernel<<<grid_for(n, 256), 256, 0, stream>>>(start, id, xyzr, n, inv_cell, dim, query_radius, rate, dt, g,
-                                                     wall_rate, velocity, neighbor_count, position, key, cnt);
+                                                     wall_rate, velocity, neighbor_count, position, key, cnt,
+                                                     (int32_t*)(ws + L.partial));
     };
     if (integrate && next_keys)
         confined ? go(contact_kernel<true, true, true>) : go(contact_kernel<true, false, true>);
@@ -364,7 +386,7 @@ static int relax(float* position, const float* radius, int32_t n, float cell_siz
         if (s == 0)
             rc = wbc_hashgrid_build(position, radius, n, cell_size, dim, capacity, workspace, workspace_bytes, stream);
         else
-            rc = (int)hashgrid_build_from_keys(position, radius, n, dim, (char*)workspace, L, stream);
+            rc = (int)hashgrid_build_from_keys(position, radius, n, dim, (char*)workspace, L, stream, true);
         if (rc) return rc;
         rc = launch_contact(true, s + 1 < substeps, workspace, workspace_bytes, dim, capacity, n, cell_size,
                             query_radius, rate, dt, region, wall_rate, velocity, neighbor_count, position, stream);

@@ -127,6 +127,15 @@ hipError_t scan_i32(const int* in, int* out, int64_t n, int* partial, bool exclu
     return hipGetLastError();
 }
 
+hipError_t scan_i32_presummed(const int* in, int* out, int64_t n, int* partial, bool exclusive, hipStream_t stream)
+{
+    if (n <= 0) return hipSuccess;
+    const int tiles = (int)((n + kScanTile - 1) / kScanTile);
+    if (tiles > kSelfPrefixTiles) return scan_i32(in, out, n, partial, exclusive, stream);
+    scan_tiles<<<tiles, kScanBlock, 0, stream>>>(in, out, n, partial, exclusive ? 1 : 0, 1);
+    return hipGetLastError();
+}
+
 }  // namespace wbc
 
 WBC_API size_t wbc_scan_workspace_bytes(int64_t n) { return wbc::scan_workspace_bytes(n); }

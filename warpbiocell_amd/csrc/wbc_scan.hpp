@@ -17,4 +17,9 @@ size_t scan_workspace_bytes(int64_t n);
 // out = inclusive (or exclusive) prefix sum of in[0, n); partial holds the tile sums or offsets.
 hipError_t scan_i32(const int* in, int* out, int64_t n, int* partial, bool exclusive, hipStream_t stream);
 
+// The same scan when partial[0, tiles) already holds the sums of in's kScanTile-value tiles (added
+// up by the producer of `in`): one launch instead of two. Beyond kSelfPrefixTiles tiles it is
+// scan_i32, which recomputes them.
+hipError_t scan_i32_presummed(const int* in, int* out, int64_t n, int* partial, bool exclusive, hipStream_t stream);
+
 }  // namespace wbc
