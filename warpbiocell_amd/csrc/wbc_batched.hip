@@ -547,10 +547,13 @@ __global__ void __launch_bounds__(256) batch_hg_scatter(const ReplicaDev* __rest
                                                         const int32_t* __restrict__ rep_of_slot, int64_t slots,
                                                         const int32_t* __restrict__ key, int32_t* __restrict__ start,
                                                         int32_t* __restrict__ cnt, int32_t* __restrict__ tmp,
-                                                        int32_t buckets, int32_t active_total)
+                                                        int32_t buckets, int32_t active_total,
+                                                        int32_t* __restrict__ tile_sums, int32_t tiles)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g == 0) start[buckets] = active_total;
+    // clear the scan's tile sums (tiles + 1 ints) for the next build's key producer
+    for (int64_t k = g; k <= tiles; k += (int64_t)gridDim.x * blockDim.x) tile_sums[k] = 0;
     if (g >= slots) return;
     if (!slot_active(rep[rep_of_slot[g]], g)) return;
     const int32_t b = key[g];
@@ -584,7 +587,8 @@ __global__ void __launch_bounds__(256) batch_contact_kernel(const ReplicaDev* __
                                                             int32_t* __restrict__ neighbor_count,
                                                             float* __restrict__ position,
                                                             int32_t* __restrict__ next_key,
-                                                            int32_t* __restrict__ next_cnt)
+                                                            int32_t* __restrict__ next_cnt,
+                                                            int32_t* __restrict__ next_tile_sums)
 {
     const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
@@ -612,6 +616,7 @@ __global__ void __launch_bounds__(256) batch_contact_kernel(const ReplicaDev* __
                           wrap_dim(cell_coord(z, d.inv_cell), dim);
         next_key[i] = b;
         atomicAdd(next_cnt + b, 1);
+        wave_add_tile_counts(next_tile_sums, b);  // the next build skips its tile-sum pass
     }
 }
 
@@ -1185,9 +1190,14 @@ static hipError_t enqueue_post(wbc_batch* b, hipStream_t s)
             batch_hg_key_count<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, c.position, b->dim, key, cnt);
             WBC_TRY_ERR(hipGetLastError());
         }
-        WBC_TRY_ERR(scan_i32(cnt, start, L.buckets, (int32_t*)(ws + L.scan), true, s));
+        int32_t* tile_sums = (int32_t*)(ws + L.scan);
+        if (sub == 0)
+            WBC_TRY_ERR(scan_i32(cnt, start, L.buckets, tile_sums, true, s));
+        else  // the previous substep's gather added up the tile sums
+            WBC_TRY_ERR(scan_i32_presummed(cnt, start, L.buckets, tile_sums, true, s));
+        const int32_t tiles = (int32_t)((L.buckets + kScanTile - 1) / kScanTile);
         batch_hg_scatter<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, key, start, cnt, tmp,
-                                                     (int32_t)L.buckets, L.active_total);
+                                                     (int32_t)L.buckets, L.active_total, tile_sums, tiles);
         WBC_TRY_ERR(hipGetLastError());
         batch_hg_order<<<grid_for(L.active_total, 256), 256, 0, s>>>(key, L.active_total, start, tmp, c.position,
                                                                     c.radius, id, xyzr,
@@ -1196,11 +1206,11 @@ static hipError_t enqueue_post(wbc_batch* b, hipStream_t s)
         if (sub + 1 < b->substeps)
             batch_contact_kernel<true><<<grid_for(L.active_total, 256), 256, 0, s>>>(
                 rep, rep_of_slot, start, id, xyzr, L.active_total, b->dim, c.velocity, c.neighbor_count, c.position,
-                key, cnt);
+                key, cnt, tile_sums);
         else
             batch_contact_kernel<false><<<grid_for(L.active_total, 256), 256, 0, s>>>(
                 rep, rep_of_slot, start, id, xyzr, L.active_total, b->dim, c.velocity, c.neighbor_count, c.position,
-                key, cnt);
+                key, cnt, tile_sums);
         WBC_TRY_ERR(hipGetLastError());
     }
     return hipSuccess;

@@ -127,17 +127,8 @@ __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict_
                               wrap_dim(cell_coord(z, inv_cell), dim);
             next_key[i] = b;
             atomicAdd(next_cnt + b, 1);
-            // The next build's scan tile sums: one atomic per distinct tile among the wave's lanes
-            // (bucket-ordered lanes mostly share one), so that build skips its tile-sum pass.
-            const int32_t tile = b / kScanTile;
-            uint64_t pending = __ballot(1);
-            while (pending) {
-                const int leader = __ffsll((unsigned long long)pending) - 1;
-                const int32_t lead_tile = __shfl(tile, leader);
-                const uint64_t same = __ballot(tile == lead_tile);
-                if ((int)__lane_id() == leader) atomicAdd(next_tile_sums + lead_tile, (int32_t)__popcll(same));
-                pending &= ~same;
-            }
+            // The next build's scan tile sums, so that build skips its tile-sum pass.
+            wave_add_tile_counts(next_tile_sums, b);
         }
     }
 }

@@ -131,8 +131,9 @@ hipError_t scan_i32_presummed(const int* in, int* out, int64_t n, int* partial, 
 {
     if (n <= 0) return hipSuccess;
     const int tiles = (int)((n + kScanTile - 1) / kScanTile);
-    if (tiles > kSelfPrefixTiles) return scan_i32(in, out, n, partial, exclusive, stream);
-    scan_tiles<<<tiles, kScanBlock, 0, stream>>>(in, out, n, partial, exclusive ? 1 : 0, 1);
+    const int self_prefix = tiles <= kSelfPrefixTiles ? 1 : 0;
+    if (!self_prefix) scan_partials<<<1, kScanBlock, 0, stream>>>(partial, tiles);
+    scan_tiles<<<tiles, kScanBlock, 0, stream>>>(in, out, n, partial, exclusive ? 1 : 0, self_prefix);
     return hipGetLastError();
 }
 
