@@ -49,7 +49,7 @@ def setup(workload_name: str, device):
 
 KERNEL_OF_KIND = {"tile": "sor_fused_kernel<{tj}, {ti}>", "march": "sor_march_kernel<{tj}, {ti}>",
                   "pipe": "sor_pipe_kernel<{tj}, {ti}>", "copy": "sor_copy_kernel<{tj}, {ti}>",
-                  "packed": "sor_packed_kernel<C, {tj}, 0>"}
+                  "packed": "sor_packed_kernel<C, {tj}, 0>", "quad": "sor_quad_kernel<C, 0>"}
 COUNTER_TABLE = ROOT / "profiles" / "sor_counters_spheroid_100k_161.md"
 
 
@@ -104,13 +104,14 @@ def sor_roofline(ox, geom, device, workload: str, launches: int = 200) -> dict:
     per_sweep_s = start.elapsed_time(end) / 1e3 / launches
     bytes_per_sweep = 16 * geom.node_count
     achieved = bytes_per_sweep / per_sweep_s / 1e9
-    family = {0: "tile", 1: "march", 2: "pipe", 3: "copy", 4: "packed"}[kind.value]
+    family = {0: "tile", 1: "march", 2: "pipe", 3: "copy", 4: "packed", 5: "quad"}[kind.value]
     shape = shape_name(kind.value, tj.value, ti.value)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(shape, workload),
             "kernel": KERNEL_OF_KIND[family].format(tj=tj.value, ti=ti.value), "shape": shape,
-            "launches_per_sweep": 2 if family == "packed" else 1, "sweep_us": round(per_sweep_s * 1e6, 3),
-            "launch_us": round(per_sweep_s * 1e6 / (2 if family == "packed" else 1), 3),
+            "launches_per_sweep": 2 if family in ("packed", "quad") else 1,
+            "sweep_us": round(per_sweep_s * 1e6, 3),
+            "launch_us": round(per_sweep_s * 1e6 / (2 if family in ("packed", "quad") else 1), 3),
             "algorithmic_bytes_per_sweep": bytes_per_sweep,
             "achieved_at_12B": round(12 * geom.node_count / per_sweep_s / 1e9, 1),
             "traffic_source": str(COUNTER_TABLE.relative_to(ROOT))}

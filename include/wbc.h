@@ -121,11 +121,12 @@ int wbc_sor_sweep_fused(float omega, float dx2_over_D, float uptake_max, float m
  * nodes (k, j, i); kind 1 a 60 x TJ tile marching TI planes along i; kind 2 the 3-D tile on
  * pipelined persistent workgroups; kind 3 the 3-D tile with global->LDS copies; kind 4 the two
  * colour passes on colour-packed arrays (field and density of each colour contiguous per row;
- * Dirichlet faces, no pinned nodes, no production). The autotuner times every entry on the
- * caller's grid and selects one; the solve loop uses the selected entry (kind 4: packing the
- * field once per solve). wbc_sor_sweep_fused_cfg runs entry `cfg` (-1: the selected one) for one
- * sweep; kind 4 is not available there (a lone sweep under that selection takes the 60 x 4 x 8
- * copy tile). wbc_sor_sweeps_cfg runs `sweeps` sweeps of any entry from src into dst through a
+ * Dirichlet faces, no pinned nodes, no production), TJ rows per thread; kind 5 the same passes
+ * with four packed indices of one row per lane (rows padded to a multiple of 4). The autotuner
+ * times every entry on the caller's grid and selects one; the solve loop uses the selected entry
+ * (kinds 4 and 5: packing the field once per solve). wbc_sor_sweep_fused_cfg runs entry `cfg`
+ * (-1: the selected one) for one sweep; kinds 4 and 5 are not available there (a lone sweep under
+ * that selection takes the 60 x 4 x 8 copy tile). wbc_sor_sweeps_cfg runs `sweeps` sweeps of any entry from src into dst through a
  * workspace of wbc_sor_sweeps_workspace_bytes. No reference counterpart: Warp picks launch shapes
  * itself. */
 int wbc_sor_fused_config_count(void);

@@ -234,6 +234,11 @@ def solve(field: np.ndarray, density: np.ndarray, a: SolveArgs, omega: float, ma
     hs, src, sk = _source_args(source, source_k, dt)
     res = np.zeros(1, dtype=dt)
     conv = np.zeros(1, dtype=np.int32)
+    if dt == np.float32:
+        # fields/diffusion.py:120-121 tests the float32 residual against the double tolerance; for
+        # a float32 r that is r < t with t the smallest float32 >= tolerance.
+        t = np.float32(tolerance)
+        tolerance = float(t if float(t) >= tolerance else np.nextafter(t, np.float32(np.inf)))
     sweeps = getattr(lib(), f"orc_solve_steady_state_{suf}")(
         omega, max_sweeps, tolerance, check_every, a.dx2_over_D, a.uptake_max, a.michaelis_k, a.reference, *a.boundary,
         _p(fx), _p(dens), _p(prod), hs, _p(src), sk, _p(field), *field.shape, _p(res), _p(conv))

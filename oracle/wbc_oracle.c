@@ -461,6 +461,12 @@ void orc_contact_velocities(const float* position, const float* radius, int64_t 
         int32_t x0 = orc_cell_coord(xi - query_radius, inv_cell), x1 = orc_cell_coord(xi + query_radius, inv_cell);
         int32_t y0 = orc_cell_coord(yi - query_radius, inv_cell), y1 = orc_cell_coord(yi + query_radius, inv_cell);
         int32_t z0 = orc_cell_coord(zi - query_radius, inv_cell), z1 = orc_cell_coord(zi + query_radius, inv_cell);
+        /* hash_grid_query visits each bucket of the query box; a box wider than dim buckets on an
+         * axis wraps onto buckets already visited, so each axis is cut to dim coordinates (every
+         * bucket once): the neighbour set is then the brute-force one (tests/test_neighbors.py:20-42). */
+        if (x1 > x0 + dim - 1) x1 = x0 + dim - 1;
+        if (y1 > y0 + dim - 1) y1 = y0 + dim - 1;
+        if (z1 > z0 + dim - 1) z1 = z0 + dim - 1;
         for (int32_t cx = x0; cx <= x1; ++cx)
             for (int32_t cy = y0; cy <= y1; ++cy)
                 for (int32_t cz = z0; cz <= z1; ++cz) {

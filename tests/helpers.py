@@ -99,3 +99,31 @@ def pcg_py(v: int) -> int:
     state = (v * 747796405 + 2891336453) % 2 ** 32
     word = (((state >> ((state >> 28) + 4)) ^ state) * 277803737) % 2 ** 32
     return (word >> 22) ^ word
+
+
+def residual_terms_f64(field, density, dx2_over_D, uptake_max, michaelis_k, reference, boundary, fixed=None,
+                       production=None, source=None, source_k=1.0):
+    """Per-node residual terms |r| / max(6 O, O_ref) of kernels/field_kernels.py:215-243 in float64,
+    vectorised over the whole grid, and the mask of nodes the residual runs over (not pinned, not
+    on a Dirichlet face). r = sum of the six mirrored neighbours + dx^2/D P S/(K_S + S)
+    - (6 + dx^2/D rho q / (K + O)) O; the mirror across a face is numpy's reflect padding
+    (value at -1 = value at 1)."""
+    f = np.asarray(field, dtype=np.float64)
+    pad = np.pad(f, 1, mode="reflect")
+    nb = (pad[:-2, 1:-1, 1:-1] + pad[2:, 1:-1, 1:-1] + pad[1:-1, :-2, 1:-1] + pad[1:-1, 2:, 1:-1]
+          + pad[1:-1, 1:-1, :-2] + pad[1:-1, 1:-1, 2:])
+    c = np.asarray(density, dtype=np.float64) * uptake_max / (michaelis_k + f)
+    prod = np.zeros_like(f)
+    if production is not None:
+        prod = np.asarray(production, dtype=np.float64)
+        if source is not None:
+            s = np.asarray(source, dtype=np.float64)
+            prod = prod * (s / (source_k + s))
+    r = nb + dx2_over_D * prod - (6.0 + dx2_over_D * c) * f
+    terms = np.abs(r) / np.maximum(6.0 * f, reference)
+    mask = ~dirichlet_mask(f.shape, boundary)
+    if fixed is not None:
+        mask &= np.asarray(fixed) == 0
+    scale = (np.abs(nb) + np.abs(dx2_over_D * prod) + np.abs((6.0 + dx2_over_D * c) * f)) / np.maximum(6.0 * f,
+                                                                                                     reference)
+    return terms, mask, scale

@@ -52,7 +52,7 @@ def test_deposit_accumulator(device):
     np.testing.assert_array_equal(dens.accumulator.cpu().numpy(), g["accumulator"])
 
 
-@pytest.mark.parametrize("shape_name", ["tile60x8x8", "packed/2"])
+@pytest.mark.parametrize("shape_name", ["tile60x8x8", "packed/2", "packed/quad"])
 def test_steady_state_solve(device, shape_name):
     from warpbiocell_amd.autotune import config_shapes
 

@@ -111,11 +111,13 @@ def test_runs_are_bitwise_reproducible(device):
     np.testing.assert_array_equal(run(), run())
 
 
-@pytest.mark.parametrize("factor,dim", [(0.45, 128), (0.7, 128), (1.3, 128), (2.5, 128), (0.7, 7), (1.0, 5)])
+@pytest.mark.parametrize("factor,dim", [(0.45, 128), (0.7, 128), (1.3, 128), (2.5, 128), (0.7, 7), (1.0, 5),
+                                        (0.3, 3), (0.2, 5)])
 def test_other_bucket_sizes_take_the_general_walk(oracle, device, factor, dim):
     """Bucket edge != query radius: the walk's per-run path instead of the preloaded 3 x 3 rows
-    (the (1.0, 5) case keeps the 3 x 3 rows with buckets wrapping in every axis); counts and
-    velocities equal the oracle's bits."""
+    (the (1.0, 5) case keeps the 3 x 3 rows with buckets wrapping in every axis; (0.3, 3) and
+    (0.2, 5) have query boxes wider than the grid, cut to dim buckets per axis); counts and
+    velocities equal the oracle's bits, and the counts equal brute force."""
     rng = np.random.default_rng(int(factor * 100) + dim)
     pos = spherical_cluster(1500, 8.0, spacing_factor=0.9, jitter=0.3, seed=int(factor * 10))
     radii = rng.uniform(5.0, 9.0, 1500).astype(np.float32)
@@ -127,6 +129,7 @@ def test_other_bucket_sizes_take_the_general_walk(oracle, device, factor, dim):
     vel, cnt = oracle.contact_velocities(pos, radii, cell, dim, q, params.rate)
     np.testing.assert_array_equal(pop.neighbor_counts_numpy(), cnt)
     np.testing.assert_array_equal(pop.velocity.cpu().numpy()[:1500], vel)
+    np.testing.assert_array_equal(cnt, contact_brute_force(pos, radii, params.rate, q)[1])
 
 
 @pytest.mark.parametrize("dim", [3, 4, 6, 128])

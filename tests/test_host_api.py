@@ -86,3 +86,25 @@ def test_spherical_cluster_is_deterministic_and_compact():
     d = np.linalg.norm(a[:, None] - a[None], axis=2)
     np.fill_diagonal(d, np.inf)
     assert d.min() > 0.5 * 0.9 * 16.0
+
+
+@pytest.mark.parametrize("tol", [1e-6, 1e-5, 1e-4, 1e-3, 0.1, 1.0 / 3.0, 0.5, 2.0 ** -30, 3e-38])
+def test_float32_threshold_reproduces_the_double_comparison(tol):
+    """fields/diffusion.py:120-121 compares the float32 residual with the tolerance in double; a
+    float32 r satisfies r < tolerance exactly when r < float32_threshold(tolerance), checked on
+    the 200 float32 values on each side of the tolerance."""
+    from warpbiocell_amd.fields.diffusion import float32_threshold
+
+    t = float32_threshold(tol)
+    assert np.float32(t) == t and t >= tol
+    r = np.float32(tol)
+    lo, hi = r, r
+    for _ in range(200):
+        lo, hi = np.nextafter(lo, np.float32(0)), np.nextafter(hi, np.float32(np.inf))
+    vals = [lo]
+    while vals[-1] < hi:
+        vals.append(np.nextafter(vals[-1], np.float32(np.inf)))
+    for v in vals:
+        assert (float(v) < tol) == (v < np.float32(t)), (v, tol, t)
+    if float(np.float32(tol)) < tol:  # the rounded tolerance would have declined this residual
+        assert float(np.float32(tol)) < t

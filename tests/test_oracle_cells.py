@@ -111,6 +111,22 @@ def test_mixed_radii_and_aliasing_small_dim(oracle):
         np.testing.assert_allclose(vel, v_ref, atol=2e-3)
 
 
+@pytest.mark.parametrize("factor,dim", [(0.3, 3), (0.45, 4), (0.2, 5)])
+def test_query_box_wider_than_the_grid_counts_each_neighbour_once(oracle, factor, dim):
+    """Buckets much smaller than the query radius on a small grid: the query box spans more than
+    dim buckets per axis, so its coordinates wrap onto buckets already visited. Each neighbour
+    is still counted once: counts equal brute force and velocities the float64 pair sums."""
+    rng = np.random.default_rng(dim)
+    pos = spherical_cluster(300, 8.0, spacing_factor=1.0, jitter=0.2, seed=dim)
+    radii = rng.uniform(5.0, 10.0, size=300).astype(np.float32)
+    q = 20.0
+    assert 2 * q / (factor * q) + 1 > dim
+    vel, cnt = oracle.contact_velocities(pos, radii, np.float32(factor * q), dim, q, 10.0)
+    v_ref, c_ref = contact_brute_force(pos, radii, 10.0, q)
+    np.testing.assert_array_equal(cnt, c_ref)
+    np.testing.assert_allclose(vel, v_ref, atol=2e-3)
+
+
 def test_pair_overlap_decays_at_the_analytic_rate(oracle):
     r, overlap0, dt, n = 8.0, 3.0, 0.01, 15
     pos = np.array([[0.0, 0.0, 0.0], [2 * r - overlap0, 0.0, 0.0]], dtype=np.float32)

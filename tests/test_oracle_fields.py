@@ -199,3 +199,69 @@ def test_uniform_production_source_equals_scaled_production(oracle):
     oracle.solve(f3, density, a, 1.7, 400, 1e-6, 10, production=production,
                  source=np.full(shape, S / 2, np.float32), source_k=float(K))
     assert not np.array_equal(f1, f3)
+
+
+def _residual_case(rng, shape, boundary, with_fixed, with_production, with_source):
+    field = rng.uniform(0.0, 40.0, shape)
+    field[rng.random(shape) < 0.05] = 0.0  # depleted nodes: the reference scale takes over
+    density = np.where(rng.random(shape) < 0.5, rng.uniform(0.0, 2e-3, shape), 0.0)
+    fixed = (rng.random(shape) < 0.15).astype(np.int32) if with_fixed else None
+    production = rng.uniform(0.0, 5e3, shape) if with_production else None
+    source = rng.uniform(0.0, 6.0, shape) if with_source else None
+    return field, density, fixed, production, source
+
+
+CASES = [((0, 0, 0), False, False, False), ((0, 1, 0), True, False, False), ((1, 1, 1), False, True, False),
+         ((1, 0, 1), True, True, True), ((0, 0, 1), False, True, True)]
+
+
+@pytest.mark.parametrize("boundary,with_fixed,with_production,with_source", CASES)
+def test_residual_matches_float64_restatement(oracle, boundary, with_fixed, with_production, with_source):
+    """The residual maximum against residual_terms_f64 (an independent vectorised restatement of
+    kernels/field_kernels.py:215-243): 1e-12 relative in float64, within the float32 rounding of
+    the terms' magnitudes in float32."""
+    from tests.helpers import residual_terms_f64
+
+    rng = np.random.default_rng(hash((boundary, with_fixed, with_production, with_source)) % 2 ** 32)
+    shape = (9, 8, 10)
+    field, density, fixed, production, source = _residual_case(rng, shape, boundary, with_fixed, with_production,
+                                                               with_source)
+    a = oracle.SolveArgs(dx=10.0, diffusion=7.2e4, uptake_max=5.5e5, michaelis_k=1.5, boundary_value=20.0,
+                         boundary=boundary)
+    terms, mask, scale = residual_terms_f64(field, density, a.dx2_over_D, a.uptake_max, a.michaelis_k, a.reference,
+                                            boundary, fixed, production, source, 0.7)
+    expected = terms[mask].max()
+    got64 = oracle.residual(field, density, a, fixed=fixed, production=production, source=source, source_k=0.7)
+    assert abs(got64 - expected) <= 1e-12 * expected
+    f32 = lambda v: None if v is None else np.asarray(v, dtype=np.float32)  # noqa: E731
+    got32 = oracle.residual(f32(field), f32(density), a, fixed=fixed, production=f32(production),
+                            source=f32(source), source_k=0.7)
+    terms32, _, _ = residual_terms_f64(f32(field), f32(density), a.dx2_over_D, a.uptake_max, a.michaelis_k,
+                                       a.reference, boundary, fixed, f32(production), f32(source), 0.7)
+    assert abs(got32 - terms32[mask].max()) <= 4e-7 * scale[mask].max()
+
+
+def test_residual_term_of_every_node_matches_float64_restatement(oracle):
+    """One free node at a time (every other node pinned): the oracle's residual is then that node's
+    own term, so each node's term, including the mirrored faces, the reference scale at depleted
+    nodes, production and its source factor, is compared and not only the maximum."""
+    from tests.helpers import residual_terms_f64
+
+    rng = np.random.default_rng(11)
+    shape, boundary = (5, 4, 6), (1, 0, 1)
+    field, density, _, production, source = _residual_case(rng, shape, boundary, False, True, True)
+    a = oracle.SolveArgs(dx=10.0, diffusion=7.2e4, uptake_max=5.5e5, michaelis_k=1.5, boundary_value=20.0,
+                         boundary=boundary)
+    terms, mask, _ = residual_terms_f64(field, density, a.dx2_over_D, a.uptake_max, a.michaelis_k, a.reference,
+                                        boundary, None, production, source, 0.7)
+    checked = 0
+    for t in np.ndindex(shape):
+        fixed = np.ones(shape, dtype=np.int32)
+        fixed[t] = 0
+        got = oracle.residual(field, density, a, fixed=fixed, production=production, source=source, source_k=0.7)
+        if mask[t]:
+            assert abs(got - terms[t]) <= 1e-12 * max(terms[t], 1e-300), t
+            checked += 1
+        else:
+            assert got == 0.0, t  # a node on a Dirichlet face is never part of the residual
+    assert checked == int(mask.sum()) and checked > 0

@@ -28,7 +28,7 @@ from dataclasses import dataclass
 import torch
 
 from warpbiocell_amd import _lib
-from warpbiocell_amd.fields.diffusion import SolverSettings, UptakeKinetics, solve_steady_state
+from warpbiocell_amd.fields.diffusion import SolverSettings, UptakeKinetics, float32_threshold, solve_steady_state
 from warpbiocell_amd.fields.scalar_field import GridGeometry, ScalarField
 
 
@@ -125,7 +125,7 @@ class OxygenAdjointProblem:
         grad_rate = torch.empty(self.n, dtype=torch.float32, device=self.device)
         sums = (ctypes.c_double * 2)()
         sweeps, residual = ctypes.c_int32(), ctypes.c_float()
-        _lib.call("wbc_adjoint_solve", omega, a.max_sweeps, a.tolerance, a.check_every, h, self.q_ref,
+        _lib.call("wbc_adjoint_solve", omega, a.max_sweeps, float32_threshold(a.tolerance), a.check_every, h, self.q_ref,
                   self.michaelis_k, None, self.field.values.data_ptr(), self.rate_density.data_ptr(), None,
                   self._g.data_ptr(), self.mu.data_ptr(), self.positions.data_ptr(), self.n, g.origin[0], g.origin[1],
                   g.origin[2], g.dx, nx, ny, nz, grad_rate.data_ptr(), ctypes.byref(sums), ctypes.byref(sweeps),

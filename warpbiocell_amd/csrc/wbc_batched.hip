@@ -1241,6 +1241,7 @@ static hipError_t wide_setup(wbc_batch* b, const wbc_replica* replicas, const st
         w.pf.f[1] = (float*)(b->ws + off + each);
         w.pf.d[0] = (float*)(b->ws + off + 2 * each);
         w.pf.d[1] = (float*)(b->ws + off + 3 * each);
+        w.pf.pitch = packed_pitch(w.p.nz, kWideRows[0]);  // the row passes' pitch, ceil(nz / 2)
         off += 4 * each;
         const int64_t rl = packed_lanes_per_plane(w.p, kResidualRows);
         w.rlanes = (int32_t)rl;

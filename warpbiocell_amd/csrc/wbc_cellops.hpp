@@ -313,12 +313,19 @@ __device__ __forceinline__ void contact_gather(const int32_t* __restrict__ start
     g.vz = 0.0f;
     g.count = 0;
     const float4 P = g.P;
-    const int32_t x0 = cell_coord(P.x - query_radius, inv_cell), x1 = cell_coord(P.x + query_radius, inv_cell);
-    const int32_t y0 = cell_coord(P.y - query_radius, inv_cell), y1 = cell_coord(P.y + query_radius, inv_cell);
-    const int32_t z0 = cell_coord(P.z - query_radius, inv_cell), z1 = cell_coord(P.z + query_radius, inv_cell);
+    // A query box wider than dim buckets on an axis would reach buckets it has already visited
+    // (coordinates wrap modulo dim) and count their points twice; each axis is cut to dim
+    // consecutive coordinates, which cover every bucket of that axis once, so the neighbour set
+    // is exactly the points within the query radius.
+    const int32_t x0 = cell_coord(P.x - query_radius, inv_cell);
+    const int32_t x1 = min(cell_coord(P.x + query_radius, inv_cell), x0 + dim - 1);
+    const int32_t y0 = cell_coord(P.y - query_radius, inv_cell);
+    const int32_t y1 = min(cell_coord(P.y + query_radius, inv_cell), y0 + dim - 1);
+    const int32_t z0 = cell_coord(P.z - query_radius, inv_cell);
+    const int32_t z1 = min(cell_coord(P.z + query_radius, inv_cell), z0 + dim - 1);
     const int32_t wz0 = wrap_dim(z0, dim);
     const bool z_contiguous = wz0 + (z1 - z0) < dim;  // the z buckets are consecutive keys
-    if (x1 - x0 == 2 && y1 - y0 == 2 && z1 - z0 < dim) {
+    if (x1 - x0 == 2 && y1 - y0 == 2) {
         // A row's z buckets in walk order, wz0, wz0 + 1, ..., are one run of consecutive keys, or
         // two when they pass dim - 1 (wz0 .. dim - 1, then 0 ..); the second run is empty otherwise.
         const int32_t nz = z1 - z0 + 1;

@@ -360,7 +360,8 @@ struct SolveLayout {
 SolveLayout solve_layout(int32_t chunks, const SorParams& p)
 {
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
-    const int32_t packed_partials = 2 * packed_pass_blocks(p, 2);  // the most workgroups of any rows
+    int32_t packed_partials = 2 * packed_pass_blocks(p, 2);  // the most workgroups of the row passes
+    if (2 * packed_pass_blocks(p, kQuadRows) > packed_partials) packed_partials = 2 * packed_pass_blocks(p, kQuadRows);
     const int32_t partials = residual_partials(p) > packed_partials ? residual_partials(p) : packed_partials;
     const int64_t nodes = (int64_t)p.nx * p.ny * p.nz;
     SolveLayout L;
@@ -416,7 +417,8 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
     float* buf[2] = {field, (float*)(ws + L.scratch)};
     // The colour-packed sweeps when selected and applicable (pack once, unpack at the end), else
     // out-of-place fused sweeps on all-Dirichlet grids, else the two colour passes in place.
-    const int packed_rows = packed_supported(p) ? packed_rows_of(g_fused_config) : 0;
+    int packed_rows = packed_supported(p) ? packed_rows_of(g_fused_config) : 0;
+    if (packed_rows && !packed_rows_supported(p, packed_rows)) packed_rows = packed_rows_supported(p, 2) ? 2 : 0;
     const bool packed = packed_rows != 0;
     const bool fused = !packed && fused_supported(p);
     // Colour-packed: the residual check is folded into the passes. The black pass that ends chunk
@@ -426,7 +428,8 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
     const int32_t pass_blocks = packed ? packed_pass_blocks(p, packed_rows) : 0;
     const int32_t partials = packed ? 2 * pass_blocks : residual_partials(p);
     PackedField pf{{(float*)(ws + L.packed), (float*)(ws + L.packed + L.packed_each)},
-                   {(float*)(ws + L.packed + 2 * L.packed_each), (float*)(ws + L.packed + 3 * L.packed_each)}};
+                   {(float*)(ws + L.packed + 2 * L.packed_each), (float*)(ws + L.packed + 3 * L.packed_each)},
+                   packed_pitch(nz, packed_rows)};
     float* red[2] = {pf.f[0], (float*)(ws + L.packed + 4 * L.packed_each)};
     hipEvent_t* ev;
     WBC_TRY(get_events(&ev));

@@ -60,27 +60,42 @@ hipError_t launch_residual_atomic(const SorParams& p, const float* field, float*
 hipError_t launch_copy_f32(const float* a, float* b, int64_t n, hipStream_t s);
 
 // Colour-packed layout (wbc_sor_packed.hip): per colour, one field and one density array of
-// packed_floats(p) floats each: nx ny ceil(nz / 2) nodes and kPackedPadRows rows of padding, which
-// the colour passes read (and never use) for rows past the last interior row. Dirichlet faces, no
-// pinned nodes, no production only.
+// packed_floats(p) floats each, rows of `pitch` floats holding ceil(nz / 2) nodes: the row passes
+// (RJ rows per thread) use pitch ceil(nz / 2), the quad pass (four packed indices per lane, one
+// 16-byte load per array) the same rounded up to a multiple of 4. kPackedPadRows rows of padding
+// follow the last plane; the passes read (and never use) them past the last interior row.
+// Dirichlet faces, no pinned nodes, no production only.
 struct PackedField {
     float* f[2];
     float* d[2];
+    int32_t pitch;
 };
 constexpr int kPackedPadRows = 8;
+// `rows` value that selects the quad pass in the functions below.
+constexpr int kQuadRows = 1;
+inline int32_t packed_pitch(int32_t nz, int rows)
+{
+    const int32_t nz2 = (nz + 1) / 2;
+    return rows == kQuadRows ? (nz2 + 3) / 4 * 4 : nz2;
+}
+// Floats of one packed array, enough for either pitch.
 inline int64_t packed_floats_of(int32_t nx, int32_t ny, int32_t nz)
 {
-    return ((int64_t)nx * ny + kPackedPadRows) * ((nz + 1) / 2);
+    return ((int64_t)nx * ny + kPackedPadRows) * packed_pitch(nz, kQuadRows);
 }
 bool packed_supported(const SorParams& p);
+// Whether the pass with `rows` (2, 4, 8 or kQuadRows) can run on p's grid (its lane indexing fits
+// 32 bits).
+bool packed_rows_supported(const SorParams& p, int rows);
 inline int64_t packed_floats(const SorParams& p) { return packed_floats_of(p.nx, p.ny, p.nz); }
-// f0_copy (optional): also write the colour-0 field there.
+// f0_copy (optional): also write the colour-0 field there. pf.pitch sets the row pitch.
 hipError_t launch_pack(const SorParams& p, const float* field, const float* density, PackedField& pf, hipStream_t s,
                        float* f0_copy = nullptr);
 hipError_t launch_unpack(const SorParams& p, const PackedField& pf, float* field, hipStream_t s);
-// One sweep (both colour passes); each thread takes `rows` (2, 4 or 8) consecutive rows j.
+// One sweep (both colour passes); each thread takes `rows` (2, 4 or 8) consecutive rows j, or with
+// kQuadRows four packed indices of one row (pf.pitch must be packed_pitch(nz, rows)).
 hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, const int32_t* done, hipStream_t s);
-// Workgroups of one colour pass at `rows` rows per thread.
+// Workgroups of one colour pass at `rows`.
 int packed_pass_blocks(const SorParams& p, int rows);
 // One colour pass of the solve loop's colour-packed path with the residual check folded in.
 // check 0: in place in `own`. check 1: the pass also stores the residual terms of the values it
@@ -92,8 +107,8 @@ hipError_t launch_packed_colour(const SorParams& p, int colour, int rows, int ch
                                 hipStream_t s);
 
 // The sweep shape table (wbc_sor_fused.hip): the entry the solve loop uses, and the colour-packed
-// entries (which the solve loop runs on packed arrays for its whole duration): rows per thread,
-// or 0 for an entry that is not colour-packed.
+// entries (which the solve loop runs on packed arrays for its whole duration): rows per thread
+// (kQuadRows for the quad pass), or 0 for an entry that is not colour-packed.
 extern int g_fused_config;
 int packed_rows_of(int cfg);
 

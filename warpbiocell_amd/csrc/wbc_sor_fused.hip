@@ -190,7 +190,9 @@ __global__ void __launch_bounds__(256) copy_f32_kernel(const float* __restrict__
 // kind 0: 3-D tile 60 x TJ x TI (this file); kind 1: 60 x TJ tile marching CI planes along i;
 // kind 2: 3-D tile 60 x TJ x TI, pipelined persistent workgroups (this file); kind 3: 3-D tile
 // 60 x TJ x TI with global->LDS copies (this file); kind 4: two colour passes on colour-packed
-// arrays (wbc_sor_packed.hip), which only the multi-sweep entry point and the solve loop run.
+// arrays, TJ rows per thread (wbc_sor_packed.hip); kind 5: two colour passes on colour-packed
+// arrays, four packed indices of one row per lane (the quad pass, wbc_sor_packed.hpp). Kinds 4
+// and 5 run only in the multi-sweep entry point and the solve loop.
 struct FusedConfig {
     int kind, tj, ti;
 };
@@ -198,12 +200,15 @@ constexpr FusedConfig kFusedConfigs[] = {{0, 4, 4}, {0, 4, 8}, {0, 8, 4}, {0, 8,
                                          {0, 2, 16}, {0, 16, 2}, {1, 12, 8}, {1, 12, 16}, {1, 12, 24},
                                          {1, 28, 8}, {1, 28, 16}, {1, 12, 4}, {2, 4, 4}, {2, 4, 8},
                                          {2, 8, 4}, {2, 8, 8}, {3, 4, 4}, {3, 4, 8}, {3, 8, 4}, {3, 8, 8},
-                                         {3, 2, 8}, {3, 8, 2}, {4, 2, 0}, {4, 4, 0}, {4, 8, 0}};
+                                         {3, 2, 8}, {3, 8, 2}, {4, 2, 0}, {4, 4, 0}, {4, 8, 0},
+                                         {5, 4, 0}};
 constexpr int kFusedConfigCount = sizeof(kFusedConfigs) / sizeof(kFusedConfigs[0]);
 
 int packed_rows_of(int cfg)
 {
-    return (cfg >= 0 && cfg < kFusedConfigCount && kFusedConfigs[cfg].kind == 4) ? kFusedConfigs[cfg].tj : 0;
+    if (cfg < 0 || cfg >= kFusedConfigCount) return 0;
+    if (kFusedConfigs[cfg].kind == 5) return kQuadRows;
+    return kFusedConfigs[cfg].kind == 4 ? kFusedConfigs[cfg].tj : 0;
 }
 int g_fused_config = 3;
 constexpr int kResidualIStride = 8;
@@ -429,10 +434,11 @@ WBC_API int wbc_sor_sweeps_cfg(int32_t cfg, int32_t sweeps, float omega, float d
                 has_source, source, source_k, nx, ny, nz};
     char* ws = (char*)workspace;
     if (const int rows = packed_rows_of(cfg)) {
-        if (!packed_supported(p)) return (int)hipErrorInvalidValue;
+        if (!packed_rows_supported(p, rows)) return (int)hipErrorInvalidValue;
         const size_t bytes = (L.total - L.packed) / 4;
         PackedField pf{{(float*)(ws + L.packed), (float*)(ws + L.packed + bytes)},
-                       {(float*)(ws + L.packed + 2 * bytes), (float*)(ws + L.packed + 3 * bytes)}};
+                       {(float*)(ws + L.packed + 2 * bytes), (float*)(ws + L.packed + 3 * bytes)},
+                       packed_pitch(nz, rows)};
         WBC_TRY(launch_pack(p, src, density, pf, stream));
         for (int32_t s = 0; s < sweeps; ++s) WBC_TRY(launch_sor_packed(p, pf, rows, nullptr, stream));
         return (int)launch_unpack(p, pf, dst, stream);

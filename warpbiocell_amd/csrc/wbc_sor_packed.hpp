@@ -159,6 +159,125 @@ __device__ __forceinline__ void packed_lane(int tile, int blocks_per_plane, int&
     u = (tile - (i - 1) * blocks_per_plane) * 256 + (int)threadIdx.x;
 }
 
+// ---- the quad pass: four packed indices of one row per lane ------------------------------------
+//
+// Lane t of a colour pass takes interior plane i, interior row j and the four packed indices
+// kk0 .. kk0+3 (kk0 = 4 q) of that row, the lanes flattened over (plane, row, q) so that no lane of
+// any wave idles. Every array is read with one 16-byte load per lane: the other colour's rows j-1,
+// j, j+1 of plane i and row j of planes i-1 and i+1, the own field and the density, 7 loads for 4
+// nodes (the row passes above issue 6 RJ + 2 four-byte loads for RJ nodes). The k-neighbours of
+// the four nodes are the other colour's row j at kk0 - 1 + par .. kk0 + 4 + par: the middle four are
+// the lane's own load, the outer two (kk0 - 1 and kk0 + 4) the neighbouring lanes' (a wave shift
+// by one lane); lanes 0 and 63 load theirs. Where the neighbouring lane sits on another row (q = 0
+// or the last q), the value it supplies is one no interior node uses: node kk0 = 0 of parity 0 is
+// k = 0 and node kk0 + 3 at the row end is k >= nz - 1, both on the Dirichlet faces.
+//
+// Addresses are 32-bit byte offsets from the arrays' bases (the host requires the packed array
+// below 4 GiB).
+
+typedef float wbc_f4 __attribute__((ext_vector_type(4)));
+
+// Unsigned division by an invariant divisor d >= 1, exact for every 32-bit numerator: with
+// l = ceil(log2 d), m = floor(2^32 (2^l - d) / d) + 1, t = mulhi(m, n),
+// n / d = (t + ((n - t) >> min(l, 1))) >> max(l - 1, 0) (Granlund & Montgomery, "Division by
+// invariant integers using multiplication", PLDI 1994, figure 4.1).
+struct FastDiv {
+    uint32_t d, m, s1, s2;
+};
+inline FastDiv fast_div_of(uint32_t d)
+{
+    uint32_t l = 0;
+    while (l < 32 && ((uint64_t)1 << l) < d) ++l;
+    FastDiv f;
+    f.d = d;
+    f.m = (uint32_t)(((((uint64_t)1 << l) - d) << 32) / d + 1);
+    f.s1 = l < 1 ? l : 1;
+    f.s2 = l > 1 ? l - 1 : 0;
+    return f;
+}
+__device__ __forceinline__ uint32_t fast_div(uint32_t n, const FastDiv& f)
+{
+    const uint32_t t = __umulhi(f.m, n);
+    return (t + ((n - t) >> f.s1)) >> f.s2;
+}
+
+struct QuadGrid {
+    FastDiv plane, row;  // divisions by the lanes per plane and the groups of 4 per row
+    uint32_t row_bytes, plane_bytes;
+    int32_t total;  // lanes of one pass
+};
+
+__device__ __forceinline__ float lane_from_left(float v, float lane0)  // lane L: lane L-1's v
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lane0), __float_as_int(v), 0x138, 0xf, 0xf, false));
+}
+__device__ __forceinline__ float lane_from_right(float v, float lane63)  // lane L: lane L+1's v
+{
+    return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lane63), __float_as_int(v), 0x130, 0xf, 0xf, false));
+}
+
+// One lane of the quad pass of colour C (kernels/field_kernels.py:185-212, same expressions and
+// order: neighbour sum i-1, i+1, j-1, j+1, k-1, k+1). Writes the four values (unchanged where a
+// slot is not an interior node) to `out` (kRes == 1) or in place, and returns the maximum
+// residual term as packed_update does for kRes 1 and 2. When the solve's convergence latch
+// `done` is set the lane does nothing and returns -1; the latch is read after the lane's loads
+// are issued, so its latency overlaps theirs (0.1-0.2 us per 161^3 sweep).
+template <int C, int kRes>
+__device__ __forceinline__ float quad_update(const SorParams& p, const QuadGrid& g, uint32_t t, bool store,
+                                             const float* __restrict__ own, float* __restrict__ out,
+                                             const float* __restrict__ other, const float* __restrict__ dens,
+                                             const int32_t* __restrict__ done, int q_finite)
+{
+    const int32_t stop = done ? *done : 0;
+    const uint32_t pl = fast_div(t, g.plane);
+    const uint32_t u = t - pl * g.plane.d;
+    const uint32_t row = fast_div(u, g.row);
+    const uint32_t q = u - row * g.row.d;
+    const int i = (int)pl + 1, j = (int)row + 1;
+    const int kk0 = 4 * (int)q;
+    const int par = (C + i + j) & 1;
+    const uint32_t w = (uint32_t)i * g.plane_bytes + (uint32_t)j * g.row_bytes + (uint32_t)kk0 * 4u;
+    const char* ob = (const char*)other;
+    auto at = [](const void* base, uint32_t off) { return *(const wbc_f4*)((const char*)base + off); };
+    const wbc_f4 jm = at(ob, w - g.row_bytes), jc = at(ob, w), jp = at(ob, w + g.row_bytes);
+    const wbc_f4 im = at(ob, w - g.plane_bytes), ip = at(ob, w + g.plane_bytes);
+    const wbc_f4 o = at(own, w), d = at(dens, w);
+    const int lane = (int)__lane_id();
+    float edge = 0.0f;
+    if (lane == 0 || lane == kWave - 1) edge = *(const float*)(ob + w + (lane == 0 ? -4 : 16));
+    if (stop) return -1.0f;
+    const float ext[6] = {lane_from_left(jc.w, edge), jc.x, jc.y, jc.z, jc.w, lane_from_right(jc.x, edge)};
+    const int k0 = 2 * kk0 + par;
+    wbc_f4 val;
+    float m = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        val[e] = o[e];
+        const int k = k0 + 2 * e;
+        if (k < 1 || k > p.nz - 2) continue;
+        const float km = par ? ext[e + 1] : ext[e];      // other at kk0 + e - 1 + par
+        const float kp = par ? ext[e + 2] : ext[e + 1];  // other at kk0 + e + par
+        const float nb = im[e] + ip[e] + jm[e] + jp[e] + km + kp;
+        if (kRes == 1) m = fmaxf(m, packed_residual_term(p, nb, o[e], d[e], q_finite));
+        const float kpo = p.michaelis_k + o[e];
+        float gs;
+        // density == 0 makes c == +-0 whenever K + o > 0 and uptake_max is finite: 6 + h*c == 6.
+        if (d[e] == 0.0f && kpo > 0.0f && q_finite) {
+            gs = div6(nb);
+        } else {
+            const float c = d[e] * p.uptake_max / kpo;
+            gs = nb / (6.0f + p.dx2_over_D * c);
+        }
+        const float v = fmaxf((1.0f - p.omega) * o[e] + p.omega * gs, 0.0f);
+        val[e] = v;
+        if (kRes == 2) m = fmaxf(m, packed_residual_term(p, nb, v, d[e], q_finite));
+    }
+    // The own colour's slots are read and written by this lane only, so rewriting an unchanged
+    // value (face nodes, the row's padding) races with nothing. Non-temporal as in packed_update.
+    if (store) __builtin_nontemporal_store(val, (wbc_f4*)((char*)(kRes == 1 ? out : const_cast<float*>(own)) + w));
+    return m;
+}
+
 // ---- host helpers ---------------------------------------------------------------------------
 
 // Lanes of one plane for RJ rows per thread, or -1 when the multiply-high division would not be
@@ -168,6 +287,20 @@ inline int64_t packed_lanes_per_plane(const SorParams& p, int rj)
     const int64_t nz2 = (p.nz + 1) / 2;
     const int64_t lanes = (int64_t)((p.ny - 2 + rj - 1) / rj) * nz2;
     return lanes * nz2 < ((int64_t)1 << 32) ? lanes : -1;
+}
+
+// The quad pass's lane geometry, or false when its 32-bit indexing does not cover the grid.
+inline bool quad_grid_of(const SorParams& p, QuadGrid& g)
+{
+    const int64_t pitch = packed_pitch(p.nz, kQuadRows);
+    const int64_t groups = pitch / 4, lanes = (int64_t)(p.ny - 2) * groups, total = lanes * (p.nx - 2);
+    if (packed_floats(p) * 4 >= ((int64_t)1 << 32) || total >= INT32_MAX - 256) return false;
+    g.plane = fast_div_of((uint32_t)lanes);
+    g.row = fast_div_of((uint32_t)groups);
+    g.row_bytes = (uint32_t)(pitch * 4);
+    g.plane_bytes = (uint32_t)(pitch * 4 * p.ny);
+    g.total = (int32_t)total;
+    return true;
 }
 
 inline uint32_t nz2_magic_of(const SorParams& p)

@@ -13,6 +13,7 @@ from __future__ import annotations
 import ctypes
 from dataclasses import dataclass
 
+import numpy as np
 import torch
 
 from warpbiocell_amd import _lib
@@ -60,6 +61,20 @@ class SolveReport:
     converged: bool
 
 
+def float32_threshold(tolerance: float) -> float:
+    """The smallest float32 t with t >= ``tolerance``.
+
+    The reference reads the float32 residual back as a Python float and tests ``residual <
+    tolerance`` in double (fields/diffusion.py:120-121). For a float32 r, ``r < tolerance`` holds
+    exactly when ``r < t``: every float32 below t is below ``tolerance``. Rounding ``tolerance`` to
+    the nearest float32 instead would round 1e-6, 1e-5 and 1e-4 down, and a residual equal to that
+    rounded value would converge in the reference but not here."""
+    t = np.float32(tolerance)
+    if float(t) < tolerance:
+        t = np.nextafter(t, np.float32(np.inf))
+    return float(t)
+
+
 def solver_scalars(field: ScalarField, kinetics: UptakeKinetics) -> tuple[float, float]:
     """(dx^2 / D, residual reference scale), computed in double as the reference does."""
     dx2_over_D = field.geometry.dx ** 2 / kinetics.diffusion
@@ -79,7 +94,8 @@ def solve_steady_state(field: ScalarField, density: torch.Tensor, kinetics: Upta
         production = field.fixed_zero_like()
     ws, host = field.solver_workspace(settings.max_sweeps, settings.check_every)
     sweeps, residual, converged = ctypes.c_int32(), ctypes.c_float(), ctypes.c_int32()
-    _lib.call("wbc_solve_steady_state", settings.omega, settings.max_sweeps, settings.tolerance, settings.check_every,
+    _lib.call("wbc_solve_steady_state", settings.omega, settings.max_sweeps, float32_threshold(settings.tolerance),
+              settings.check_every,
               dx2_over_D, kinetics.uptake_max, kinetics.michaelis_k, reference, bc_x, bc_y, bc_z,
               _lib.ptr(field.fixed_or_none), _lib.ptr(density), _lib.ptr(production), 1 if source is not None else 0,
               _lib.ptr(source), float(source_k), field.values.data_ptr(), nx, ny, nz, ws.data_ptr(), ws.numel(),

@@ -39,7 +39,7 @@ from warpbiocell_amd import _lib
 from warpbiocell_amd.cells.lifecycle import LifecycleParams
 from warpbiocell_amd.cells.mechanics import ContactParams
 from warpbiocell_amd.cells.state import CellPopulation
-from warpbiocell_amd.fields.diffusion import SolveReport, solver_scalars
+from warpbiocell_amd.fields.diffusion import SolveReport, float32_threshold, solver_scalars
 from warpbiocell_amd.fields.oxygen import OxygenField
 from warpbiocell_amd.simulation.simulator import StepReport, TimeStepping
 
@@ -124,7 +124,8 @@ def replica_desc(r: Replica, cell_offset: int, node_offset: int) -> ReplicaDesc:
     d.weight[:] = [float(w) for w in ox.species.uptake_weights]
     s = ox.settings
     d.omega, d.dx2_over_D, d.uptake_max, d.michaelis_k = s.omega, dx2_over_D, kin.uptake_max, kin.michaelis_k
-    d.reference, d.tolerance, d.max_sweeps, d.check_every = reference, s.tolerance, s.max_sweeps, s.check_every
+    d.reference, d.max_sweeps, d.check_every = reference, s.max_sweeps, s.check_every
+    d.tolerance = float32_threshold(s.tolerance)
     d.dt_cells, d.death_rate = r.stepping.dt_cells, life.death_rate
     d.anoxic_death_rate, d.division_rate = life.anoxic_death_rate, life.division_rate
     d.inhibition_threshold = int(life.inhibition_threshold)
