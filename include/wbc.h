@@ -42,6 +42,14 @@ int wbc_scan_inclusive_i32(const int32_t* in, int32_t* out, int64_t n, void* wor
 int wbc_scan_exclusive_i32(const int32_t* in, int32_t* out, int64_t n, void* workspace, size_t workspace_bytes,
                            wbc_stream_t stream);
 
+/* No counterpart in the reference: wbc_scan_inclusive_i32 for callers that cannot allocate device
+ * memory (code under HIP graph capture, serving loops, harnesses that forbid hipMalloc). The tile
+ * sums live in a fixed device array of the library (256 KiB per device, kept until the process
+ * ends), so n is limited to 268,435,456; larger n returns hipErrorInvalidValue before any device
+ * work. The scan runs on the current device. Calls that overlap on one device share that array
+ * and race; calls in one stream are ordered. */
+int wbc_scan_inclusive_i32_noalloc(const int32_t* in, int32_t* out, int64_t n, wbc_stream_t stream);
+
 /* ---- lifecycle -----------------------------------------------------------------------------
  * Replaces the lifecycle_decide launch of decide_lifecycle, cells/lifecycle.py:96-132
  * (kernel kernels/cell_kernels.py:77-154); argument order and meaning as in the kernel. */

@@ -68,3 +68,16 @@ def test_no_gpu_means_a_loud_error():
         _lib.require_gpu("cuda:0")
     with pytest.raises(RuntimeError):
         _lib.require_gpu("cpu")
+
+
+def test_noalloc_scan_rejects_bad_sizes_before_any_device_work():
+    """The no-allocation scan checks n against its fixed tile-sum array (2^28 values) and its
+    pointers before touching the device, so these calls return on a machine without a GPU."""
+    from warpbiocell_amd import _lib
+
+    fn = _lib.load().wbc_scan_inclusive_i32_noalloc
+    invalid = 1  # hipErrorInvalidValue
+    assert fn(None, None, 2 ** 28 + 1, None) == invalid
+    assert fn(None, None, -1, None) == invalid
+    assert fn(None, None, 1000, None) == invalid  # null pointers
+    assert fn(None, None, 0, None) == 0  # nothing to do

@@ -26,6 +26,7 @@ SIGNATURES: dict[str, tuple] = {
     "wbc_scan_workspace_bytes": (c_size_t, [I64]),
     "wbc_scan_inclusive_i32": (I32, [P, P, I64, P, c_size_t, P]),
     "wbc_scan_exclusive_i32": (I32, [P, P, I64, P, c_size_t, P]),
+    "wbc_scan_inclusive_i32_noalloc": (I32, [P, P, I64, P]),
     "wbc_lifecycle_decide": (I32, [F32, F32, F32, F32, I32, F32, F32, F32, F32, I32, I32, F32, F32,
                                    P, P, P, P, P, P, P, P, I64, P]),
     "wbc_place_daughters": (I32, [I32, F32, P, P, P, P, P, P, P, P, P, P, P, P, P, I64, P]),

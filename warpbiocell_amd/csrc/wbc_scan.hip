@@ -7,6 +7,8 @@
 // LDS exchange of the four wave totals.
 #include "wbc_scan.hpp"
 
+#include <mutex>
+
 namespace wbc {
 
 __device__ __forceinline__ int block_exclusive_scan(int v, int* wave_sums, int& total)
@@ -137,9 +139,55 @@ hipError_t scan_i32_presummed(const int* in, int* out, int64_t n, int* partial, 
     return hipGetLastError();
 }
 
+// ---- the scan without a caller workspace --------------------------------------------------------
+//
+// Its tile sums live in a fixed device array, `scratch`: up to kNoAllocScanTiles tiles, so inputs of
+// up to kNoAllocScanMax values (256 KiB + 256 B per device, held until the process ends). Host code
+// never uses the array's name as an address: the first call on a device looks up that device's copy
+// with hipGetSymbolAddress and keeps the pointer. Every call writes each tile sum it later reads, so
+// nothing needs clearing between calls; calls that overlap on one device share the array and race.
+constexpr int64_t kNoAllocScanTiles = 65536;
+constexpr int64_t kNoAllocScanMax = kNoAllocScanTiles * kScanTile;  // 268,435,456 values
+constexpr int64_t kNoAllocScanInts = kNoAllocScanTiles + kScanBlock / 4;  // tiles + 1, 256-byte multiple
+static __device__ __align__(256) int32_t scratch[kNoAllocScanInts];
+
+namespace {
+constexpr int kMaxDevices = 64;
+std::mutex g_scratch_mutex;
+int32_t* g_scratch_ptr[kMaxDevices];  // per device: the device address of `scratch`, once looked up
+
+hipError_t scratch_of_current_device(int32_t** out)
+{
+    int dev = 0;
+    WBC_TRY_ERR(hipGetDevice(&dev));
+    if (dev < 0 || dev >= kMaxDevices) return hipErrorInvalidDevice;
+    std::lock_guard<std::mutex> lock(g_scratch_mutex);
+    if (!g_scratch_ptr[dev]) {
+        void* p = nullptr;
+        WBC_TRY_ERR(hipGetSymbolAddress(&p, HIP_SYMBOL(scratch)));
+        if (!p) return hipErrorInvalidSymbol;
+        g_scratch_ptr[dev] = (int32_t*)p;
+    }
+    *out = g_scratch_ptr[dev];
+    return hipSuccess;
+}
+}  // namespace
+
 }  // namespace wbc
 
 WBC_API size_t wbc_scan_workspace_bytes(int64_t n) { return wbc::scan_workspace_bytes(n); }
+
+// wbc_scan_inclusive_i32 with the tile sums in the library's device array instead of a caller
+// workspace: no allocation on any call. n above 268,435,456 is rejected before any device work.
+WBC_API int wbc_scan_inclusive_i32_noalloc(const int32_t* in, int32_t* out, int64_t n, hipStream_t stream)
+{
+    if (n < 0 || n > wbc::kNoAllocScanMax) return (int)hipErrorInvalidValue;
+    if (n == 0) return 0;
+    if (!in || !out) return (int)hipErrorInvalidValue;
+    int32_t* partial = nullptr;
+    WBC_TRY(wbc::scratch_of_current_device(&partial));
+    return (int)wbc::scan_i32(in, out, n, partial, false, stream);
+}
 
 WBC_API int wbc_scan_inclusive_i32(const int32_t* in, int32_t* out, int64_t n, void* workspace,
                                    size_t workspace_bytes, hipStream_t stream)
