@@ -313,8 +313,12 @@ int wbc_radial_profile(const float* r, const int32_t* cell_state, const float* v
  * population and oxygen field. Here up to WBC_BATCH_MAX_REPLICAS replicas with their own cell
  * counts, grids and parameters advance together: one cell_step of every replica is
  *   deposit_combined -> steady-state solve -> sample  (fields/species.py update, oxygen weights)
- *   -> decide_lifecycle (no divisions: division_rate must be 0)
+ *   -> lifecycle_step: decide_lifecycle, then commit_divisions per replica (cells/lifecycle.py:96-179:
+ *      inclusive scan of divide_flag over the replica's count, daughters at slot count + offset - 1,
+ *      nothing placed and the replica's capacity-error flag raised when they would not fit)
  *   -> `substeps` x (hash-grid build, contact gather, x += v dt)   (relax_contacts, no region).
+ * Cell counts live on the device (wbc_batch_cell_counts reads them); the daughters' slots must lie
+ * inside each replica's capacity.
  * The solves of all replicas run under one of two schedule families (wbc_batch_set_schedule):
  *  - persistent: ONE launch; every phase sweeps every replica that has not yet converged,
  *    separated by a grid barrier (bounded wait; a timeout sets the error flag read by
@@ -351,6 +355,7 @@ typedef struct wbc_replica {
     float apoptosis_rate, necrosis_rate;
     float cell_size, query_radius, contact_rate, dt_mechanics;              /* mechanics.py:179-189 */
     int32_t substeps;
+    float placement_factor;                 /* lifecycle.py:40-43, used by place_daughters */
 } wbc_replica;
 
 /* Packed per-cell arrays (cells/state.py field names), each of sum(capacity) entries. */
@@ -367,6 +372,7 @@ typedef struct wbc_batch_cells {
     int32_t* division_offset;
     int32_t* fate_flags;
     uint32_t* rng_state;
+    int32_t* cell_type;       /* copied to daughters */
 } wbc_batch_cells;
 
 typedef struct wbc_batch* wbc_batch_t;
@@ -399,6 +405,12 @@ int wbc_batch_solve(wbc_batch_t batch, wbc_stream_t stream);
  * when a grid barrier timed out (results of that launch are invalid). */
 int wbc_batch_report(wbc_batch_t batch, wbc_stream_t stream, int32_t* sweeps, float* residual, int32_t* converged,
                      int64_t* total_sweeps, int32_t* error);
+/* Synchronizes `stream`, then copies per replica (any pointer may be NULL): the current cell count,
+ * the daughters the last step placed, and the capacity-error flag (1 once a step's daughters did
+ * not fit: none were placed for that replica, whose step then went on as the reference's does not,
+ * since the reference raises CapacityError there; the run should stop). */
+int wbc_batch_cell_counts(wbc_batch_t batch, wbc_stream_t stream, int32_t* counts, int32_t* daughters,
+                          int32_t* capacity_error);
 int wbc_batch_destroy(wbc_batch_t batch);
 
 #ifdef __cplusplus

@@ -1,20 +1,23 @@
-"""Batched replica sweep vs running each replica alone with the plain step.
+"""Batched replica sweep vs running each replica alone with the plain step, and vs the oracle.
 
 Each replica is built twice from the same seed: one copy advances with
 ``simulation.simulator.cell_step`` (hash grid of 128 buckets per axis), the other inside one
 ``BatchedSweep`` (64 buckets per axis; the contact result does not depend on the bucket count).
-After every step the solve reports must match and, at the end, every per-cell array and every
-field bit for bit. The replicas cover:
+After every step the reports (solve and daughters) must match and, at the end, every per-cell
+array, the cell counts and every field bit for bit. The replicas cover:
 
 - cubic and non-cubic grids with partial tiles on every axis;
 - an empty population;
 - capacity gaps between replicas;
 - a solve that stops at max_sweeps unconverged;
-- oxygen-dependent death.
+- oxygen-dependent death;
+- growth: many divisions per step with deaths, and the lifecycle of the reference's sweep
+  (configs/tumor_spheroid.yaml:31-38) with contact inhibition.
 """
 
 from __future__ import annotations
 
+import ctypes
 import math
 
 import numpy as np
@@ -23,7 +26,7 @@ import torch
 
 pytestmark = pytest.mark.gpu
 
-CELL_ARRAYS = ("position", "radius", "cell_state", "age", "oxygen_local", "glucose_local", "velocity",
+CELL_ARRAYS = ("position", "radius", "cell_state", "cell_type", "age", "oxygen_local", "glucose_local", "velocity",
                "neighbor_count", "divide_flag", "division_offset", "fate_flags", "rng_state")
 
 SPECS = [
@@ -32,7 +35,9 @@ SPECS = [
     ("hypoxic33", 3000, 3000, (33, 33, 33), 20.0, 20.0, 1, "oxygen", 2000, 1e-6),
     ("skew", 1500, 1600, (29, 37, 45), 16.0, 60.0, 2, "plain", 2000, 1e-6),
     ("capped", 200, 256, (9, 9, 9), 40.0, 100.0, 3, "plain", 35, 1e-12),
-    ("empty", 0, 16, (11, 13, 15), 30.0, 150.0, 4, "plain", 2000, 1e-6),
+    ("empty", 0, 16, (11, 13, 15), 30.0, 150.0, 4, "growth", 2000, 1e-6),
+    ("growing", 1200, 2600, (25, 27, 23), 18.0, 40.0, 5, "growth", 2000, 1e-6),
+    ("sweep_lifecycle", 2500, 3600, (31, 31, 31), 20.0, 20.0, 6, "sweep", 2000, 1e-6),
 ]
 
 
@@ -58,6 +63,10 @@ def build(spec, device):
     settings = SolverSettings(omega=1.8, tolerance=tol, max_sweeps=max_sweeps, check_every=10)
     if kind == "oxygen":
         lifecycle = LifecycleParams.oxygen_dependent(division_rate=0.0, inhibition_threshold=10 ** 6)
+    elif kind == "growth":  # about 14 % of cells divide per step, 0.2 % die
+        lifecycle = LifecycleParams(division_rate=1.5, death_rate=0.02, inhibition_threshold=10 ** 6)
+    elif kind == "sweep":
+        lifecycle = LifecycleParams(**W.SWEEP_LIFECYCLE)
     else:
         lifecycle = LifecycleParams(division_rate=0.0, death_rate=0.01, inhibition_threshold=10 ** 6)
     contact = ContactParams(stiffness=W.CONTACT_RATE, damping=1.0, max_radius=float(radii.max()) if n else 5.0)
@@ -81,6 +90,7 @@ def run_serial(replicas, steps):
 
 
 def assert_same_state(serial, batch):
+    assert batch.counts() == [r.population.count for r in serial]
     for i, r in enumerate(serial):
         n = r.population.count
         for name in CELL_ARRAYS:
@@ -117,7 +127,8 @@ def test_batched_steps_are_bit_exact_per_replica(device, schedule):
         batch.step(1, use_graph=True)
         got = batch.step_reports()
         for r, a, b in zip(serial, ref_reports[k], got):
-            assert (a.field_sweeps, a.field_converged) == (b.field_sweeps, b.field_converged), (r.label, k, a, b)
+            assert (a.daughters, a.field_sweeps, a.field_converged) == \
+                   (b.daughters, b.field_sweeps, b.field_converged), (r.label, k, a, b)
             assert np.float32(a.field_residual) == np.float32(b.field_residual), (r.label, k, a, b)
     capped = [i for i, s in enumerate(SPECS) if s[0] == "capped"][0]
     assert ref_reports[0][capped].field_sweeps == 35 and not ref_reports[0][capped].field_converged
@@ -127,7 +138,11 @@ def test_batched_steps_are_bit_exact_per_replica(device, schedule):
     hyp = [i for i, s in enumerate(SPECS) if s[0] == "hypoxic33"][0]
     states = batch.population_view(hyp, "cell_state")[:3000].cpu().numpy()
     assert (states == 3).any() and (states != 3).any()
+    # the growing replicas placed daughters, so the batched slot scan and placement ran
+    grow = [i for i, s in enumerate(SPECS) if s[0] == "growing"][0]
+    assert sum(rep[grow].daughters for rep in ref_reports) > 200
     batch.write_back()
+    assert [r.population.count for r in batched] == [r.population.count for r in serial]
     batch.close()
 
 
@@ -176,10 +191,110 @@ def test_every_schedule_gives_the_same_solve(device, blocks_per_cu):
 
 
 def test_unsupported_replicas_raise(device):
-    from dataclasses import replace
-
+    from warpbiocell_amd.fields.oxygen import OxygenField
+    from warpbiocell_amd.fields.scalar_field import Boundary
     from warpbiocell_amd.simulation.batched import BatchedSweep
 
     r = build(SPECS[0], device)
+    ox = r.oxygen
+    r.oxygen = OxygenField(ox.geometry, ox.params, ox.settings, device=device,
+                           boundary=(Boundary.DIRICHLET, Boundary.NEUMANN, Boundary.DIRICHLET))
     with pytest.raises(NotImplementedError):
-        BatchedSweep([replace(r, lifecycle=replace(r.lifecycle, division_rate=0.1))])
+        BatchedSweep([r])
+
+
+def _oracle_replica(oracle, spec):
+    """The oracle's population and oxygen field for `spec`, built from the same arrays as build()."""
+    from warpbiocell_amd import workloads as W
+
+    name, n, cap, shape, dx, ob, seed, kind, max_sweeps, tol = spec
+    rng = np.random.default_rng(seed)
+    radii = (5.0 * np.exp(0.1 * rng.standard_normal(n))).astype(np.float32)
+    R = W.ball_radius_for_fraction(max(n, 1), 125.0 * math.exp(4.5 * 0.01), 0.6)
+    pos = W.ball_uniform(n, R, rng) if n else np.zeros((0, 3), np.float32)
+    origin = tuple(-0.5 * (s - 1) * dx for s in shape)
+    pop = oracle.OraclePopulation.from_numpy(pos, radii, capacity=cap, seed=seed)
+    a = oracle.SolveArgs(dx=dx, diffusion=W.D_UM2_PER_H, uptake_max=W.UPTAKE_MMHG_UM3_PER_H,
+                         michaelis_k=W.MICHAELIS_K, boundary_value=ob)
+    ox = oracle.OracleOxygen(shape=shape, origin=origin, args=a, omega=1.8, max_sweeps=max_sweeps, tolerance=tol,
+                             check_every=10)
+    return pop, ox
+
+
+def test_batched_steps_equal_the_oracle_step(oracle, device):
+    """Every replica after three batched steps (divisions, deaths and all) against
+    oracle.oracle_cell_step run three times on the same inputs: cell arrays, counts, fields and
+    reports bit for bit."""
+    from warpbiocell_amd.simulation.batched import BatchedSweep
+
+    specs = [s for s in SPECS if s[0] in ("cube21", "hypoxic33", "empty", "growing", "sweep_lifecycle")]
+    replicas = [build(s, device) for s in specs]
+    batch = BatchedSweep(replicas, hash_dim=64)
+    steps = 3
+    refs = [_oracle_replica(oracle, s) for s in specs]
+    for k in range(steps):
+        print(f"oracle comparison step {k}", flush=True)
+        batch.step(1, use_graph=True)
+        got = batch.step_reports()
+        for r, (pop, ox), b in zip(replicas, refs, got):
+            daughters, field = oracle.oracle_cell_step(pop, r.lifecycle, r.contact, r.stepping, oxygen=ox)
+            assert (daughters, field[0], field[2]) == (b.daughters, b.field_sweeps, b.field_converged), (r.label, k)
+            assert np.float32(field[1]) == np.float32(b.field_residual), (r.label, k)
+    counts = batch.counts()
+    for i, (r, (pop, ox)) in enumerate(zip(replicas, refs)):
+        n = pop.count
+        assert counts[i] == n, r.label
+        for name in ("position", "radius", "cell_state", "age", "oxygen_local", "velocity", "neighbor_count",
+                     "divide_flag", "division_offset", "rng_state"):
+            got = batch.population_view(i, name)[:n].cpu().numpy()
+            exp = getattr(pop, name)[:n]
+            assert np.array_equal(got.view(np.uint8), np.ascontiguousarray(exp).view(np.uint8)), (r.label, name)
+        assert np.array_equal(batch.field_view(i).cpu().numpy(), ox.values), r.label
+    batch.close()
+
+
+def test_capacity_overflow_places_nothing_for_that_replica(device):
+    """A replica whose daughters would not fit: none are placed for it (its count stays), the
+    others grow as serial steps do, and reading the counts raises CapacityError."""
+    from warpbiocell_amd.cells.lifecycle import CapacityError
+    from warpbiocell_amd.simulation.batched import BatchedSweep
+
+    tight = ("tight", 1200, 1210, (25, 27, 23), 18.0, 40.0, 5, "growth", 2000, 1e-6)
+    roomy = [s for s in SPECS if s[0] == "growing"][0]
+    serial = build(roomy, device)
+    ref = run_serial([serial], 1)[0][0]
+    batch = BatchedSweep([build(tight, device), build(roomy, device)], hash_dim=64)
+    batch.step(1, use_graph=True)
+    with pytest.raises(CapacityError, match="tight"):
+        batch.counts()
+    with pytest.raises(CapacityError):
+        batch.step_reports()
+    from warpbiocell_amd import _lib
+
+    counts, daughters, err = (ctypes.c_int32 * 2)(), (ctypes.c_int32 * 2)(), (ctypes.c_int32 * 2)()
+    _lib.call("wbc_batch_cell_counts", batch._h(), batch.stream, ctypes.byref(counts), ctypes.byref(daughters),
+              ctypes.byref(err))
+    assert list(err) == [1, 0]
+    assert list(counts) == [1200, serial.population.count] and list(daughters) == [0, ref.daughters]
+    assert ref.daughters > 0
+    assert batch.population_view(0, "radius")[1200:].abs().sum().item() == 0.0  # no daughter placed
+    batch.close()
+
+
+def test_three_replicas_at_128_buckets_scan_their_tile_sums_in_a_launch(device):
+    """3 x 128^3 buckets = 1,536 scan tiles, beyond the 1,024 up to which each tile adds up the
+    earlier tiles' sums itself: the bucket scan takes its tile-sum launch in every substep, and the
+    replicas still equal the serial steps bit for bit."""
+    from warpbiocell_amd.simulation.batched import BatchedSweep
+
+    specs = [s for s in SPECS if s[0] in ("growing", "sweep_lifecycle", "skew")]
+    serial = [build(s, device) for s in specs]
+    batched = [build(s, device) for s in specs]
+    ref_reports = run_serial(serial, 2)
+    batch = BatchedSweep(batched, hash_dim=128)
+    for k in range(2):
+        batch.step(1, use_graph=True)
+        assert [(a.daughters, a.field_sweeps) for a in ref_reports[k]] == \
+               [(b.daughters, b.field_sweeps) for b in batch.step_reports()]
+    assert_same_state(serial, batch)
+    batch.close()

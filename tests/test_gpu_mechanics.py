@@ -176,3 +176,22 @@ def test_dense_packing_drains_the_candidate_lists_in_order(oracle, device):
     ref, v5, c5 = pos.copy(), np.zeros_like(pos), np.zeros(2500, np.int32)
     oracle.relax_contacts(ref, radii, np.float32(q), 128, q, params.rate, 0.01, 5, v5, c5)
     np.testing.assert_array_equal(pop.positions_numpy(), ref)
+
+
+def test_relaxation_at_192_buckets_scans_its_tile_sums_in_a_launch(oracle, device):
+    """dim = 192: 192^3 buckets are 1,728 scan tiles, beyond the 1,024 up to which each tile adds up
+    the earlier tiles' sums itself, so every substep's bucket scan (the first from the tile-sum
+    pass, later ones from the sums the gather added up) takes its tile-sum launch. Three substeps
+    equal the oracle's bits."""
+    r, n = 8.0, 3000
+    pos = spherical_cluster(n, r, spacing_factor=0.85, jitter=0.2, seed=23)
+    radii = np.random.default_rng(2).uniform(6.0, 9.0, n).astype(np.float32)
+    params = ContactParams(stiffness=10.0, damping=1.0, max_radius=9.0, margin=0.5)
+    q = np.float32(params.query_radius)
+    pop = CellPopulation.from_numpy(pos, radii, device=device)
+    relax_contacts(pop, NeighborGrid(float(q), dim=192, device=device), params, dt=0.01, substeps=3)
+    ref, vel, cnt = pos.copy(), np.zeros_like(pos), np.zeros(n, np.int32)
+    oracle.relax_contacts(ref, radii, q, 192, q, params.rate, 0.01, 3, vel, cnt)
+    np.testing.assert_array_equal(pop.positions_numpy(), ref)
+    np.testing.assert_array_equal(pop.neighbor_counts_numpy(), cnt)
+    np.testing.assert_array_equal(pop.velocity.cpu().numpy()[:n], vel)

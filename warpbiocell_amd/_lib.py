@@ -85,6 +85,7 @@ SIGNATURES: dict[str, tuple] = {
     "wbc_batch_step": (I32, [P, I32, I32, P]),
     "wbc_batch_solve": (I32, [P, P]),
     "wbc_batch_report": (I32, [P, P, P, P, P, P, P]),
+    "wbc_batch_cell_counts": (I32, [P, P, P, P, P]),
     "wbc_batch_destroy": (I32, [P]),
 }
 

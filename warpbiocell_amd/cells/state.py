@@ -111,6 +111,10 @@ class CellPopulation:
         self._deposit_order[:n].copy_(order)
         self._deposit_order_count = n
 
+    def forget_deposit_order(self) -> None:
+        """Drop the recorded order (its cells were moved or replaced by another path)."""
+        self._deposit_order_count = -1
+
     def deposit_order(self) -> torch.Tensor | None:
         """The recorded order while it covers exactly the live slots (no cells added since), else None."""
         if self.count > 0 and self._deposit_order_count == self.count:

@@ -56,11 +56,12 @@ struct ReplicaDev {
     float* buf[3];   // buf[0] = the replica's field, buf[1], buf[2] workspace
     float tolerance;
     int32_t max_sweeps, check_every;
-    int32_t cell_offset, count;
+    int32_t cell_offset, count, capacity;
     int64_t node_offset, nodes;
     float ox, oy, oz, inv_dx, inv_vol, w0, w1, w2, w3;
     long long* acc;  // (4, nx, ny, nz) int64 accumulator
     LifecycleArgs life;
+    float placement_factor;
     float inv_cell, query_radius, contact_rate, dt_mech;
 };
 
@@ -463,20 +464,21 @@ __global__ void __launch_bounds__(256) batch_zero_i64_kernel(long long* __restri
 __device__ __forceinline__ bool slot_active(const ReplicaDev& d, int64_t g) { return g - d.cell_offset < d.count; }
 
 // Once a contact build has run (*order_ready), thread t deposits the cell at bucket-ordered
-// position t of that build, id[t], a permutation of the active slots (the replicas add no cells):
-// the same int64 sums, with a wave's atomics on nearby nodes. Before it, thread t takes slot t.
+// position t of that build, id[t], a permutation of the active slots: the last build of a step
+// follows its divisions, so it covers the cells the next step deposits. The same int64 sums, with
+// a wave's atomics on nearby nodes. Before any build, thread t takes slot t.
 __global__ void __launch_bounds__(256) batch_deposit_kernel(const ReplicaDev* __restrict__ rep,
                                                             const int32_t* __restrict__ rep_of_slot, int64_t slots,
                                                             const int32_t* __restrict__ id,
                                                             const int32_t* __restrict__ order_ready,
-                                                            int32_t active_total,
+                                                            const int32_t* __restrict__ active_total,
                                                             const float* __restrict__ position,
                                                             const int32_t* __restrict__ cell_state)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     int64_t g = t;
     if (*order_ready) {
-        if (t >= active_total) return;
+        if (t >= *active_total) return;
         g = id[t];
     }
     if (g >= slots) return;
@@ -513,19 +515,87 @@ __global__ void __launch_bounds__(256) batch_sample_kernel(const ReplicaDev* __r
     out[g] = sample_cell(d.buf[0], d.p.nx, d.p.ny, d.p.nz, position, g, d.ox, d.oy, d.oz, d.inv_dx);
 }
 
-// decide_lifecycle, then the division slot offsets: with no dividing cell the inclusive scan of
-// divide_flag (lifecycle.py:146) is all zeros.
+// decide_lifecycle (lifecycle.py:96-132) for every active slot, and the division flags of every
+// slot for the division scan: an active slot's divide_flag, 0 for a slot past its replica's count.
 __global__ void __launch_bounds__(256) batch_lifecycle_kernel(const ReplicaDev* __restrict__ rep,
                                                               const int32_t* __restrict__ rep_of_slot, int64_t slots,
-                                                              wbc_batch_cells c)
+                                                              wbc_batch_cells c, int32_t* __restrict__ flags)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (g >= slots) return;
     const ReplicaDev& d = rep[rep_of_slot[g]];
-    if (!slot_active(d, g)) return;
+    if (!slot_active(d, g)) {
+        flags[g] = 0;
+        return;
+    }
     lifecycle_decide_cell(d.life, g, c.cell_state, c.age, c.neighbor_count, c.oxygen_local, c.glucose_local,
                           c.fate_flags, c.rng_state, c.divide_flag);
-    c.division_offset[g] = 0;
+    flags[g] = c.divide_flag[g];
+}
+
+// commit_divisions (lifecycle.py:135-179) of every replica from one inclusive scan `scan` of the
+// flags over all slots. Past its count a replica's flags are 0, so the scan value at the slot before
+// its first is the flags of the replicas before it, and subtracting it gives the replica's own
+// inclusive scan (integers, exact). This kernel (one lane per replica) takes each replica's daughter
+// count, offset[count - 1], and its capacity test: a replica whose daughters would not fit places
+// none and raises its error flag (the reference raises CapacityError there).
+__global__ void __launch_bounds__(64) batch_division_totals_kernel(const ReplicaDev* __restrict__ rep, int32_t R,
+                                                                   const int32_t* __restrict__ scan,
+                                                                   int32_t* __restrict__ daughters,
+                                                                   int32_t* __restrict__ capacity_error)
+{
+    const int r = (int)threadIdx.x;
+    if (r >= R) return;
+    const ReplicaDev& d = rep[r];
+    int32_t n = 0;
+    if (d.count > 0) {
+        const int32_t base = d.cell_offset > 0 ? scan[d.cell_offset - 1] : 0;
+        n = scan[d.cell_offset + d.count - 1] - base;
+    }
+    if ((int64_t)d.count + n > d.capacity) {
+        capacity_error[r] = 1;
+        n = 0;
+    }
+    daughters[r] = n;
+}
+
+// Per active slot: division_offset = the replica's inclusive scan (lifecycle.py:146), and for a
+// dividing cell of a replica that places daughters, place_daughters into slot count + offset - 1
+// of that replica (kernels/cell_kernels.py:157-200). Reads the counts before this step's
+// daughters; batch_commit_counts_kernel adds them afterwards.
+__global__ void __launch_bounds__(256) batch_place_kernel(const ReplicaDev* __restrict__ rep,
+                                                          const int32_t* __restrict__ rep_of_slot, int64_t slots,
+                                                          const int32_t* __restrict__ scan,
+                                                          const int32_t* __restrict__ daughters, wbc_batch_cells c)
+{
+    const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
+    if (g >= slots) return;
+    const int r = rep_of_slot[g];
+    const ReplicaDev& d = rep[r];
+    if (!slot_active(d, g)) return;
+    const int32_t base = d.cell_offset > 0 ? scan[d.cell_offset - 1] : 0;
+    const int32_t offset = scan[g] - base;
+    c.division_offset[g] = offset;
+    if (daughters[r] == 0 || c.divide_flag[g] == 0) return;
+    const int64_t j = (int64_t)d.cell_offset + d.count + offset - 1;
+    place_daughter(g, j, d.placement_factor, c.position, c.radius, c.cell_state, c.cell_type, c.age, c.oxygen_local,
+                   c.glucose_local, c.rng_state, c.velocity, c.neighbor_count, c.fate_flags);
+}
+
+// count += daughters per replica, and the number of active slots of all replicas, which the step's
+// later kernels (hash-grid build, gather, next deposit) take as their point count.
+__global__ void __launch_bounds__(64) batch_commit_counts_kernel(ReplicaDev* __restrict__ rep, int32_t R,
+                                                                 const int32_t* __restrict__ daughters,
+                                                                 int32_t* __restrict__ active_total)
+{
+    const int r = (int)threadIdx.x;
+    int32_t n = 0;
+    if (r < R) {
+        rep[r].count += daughters[r];
+        n = rep[r].count;
+    }
+    for (int off = 32; off > 0; off >>= 1) n += __shfl_xor(n, off, 64);
+    if (r == 0) *active_total = n;
 }
 
 __global__ void __launch_bounds__(256) batch_hg_key_count(const ReplicaDev* __restrict__ rep,
@@ -547,11 +617,11 @@ __global__ void __launch_bounds__(256) batch_hg_scatter(const ReplicaDev* __rest
                                                         const int32_t* __restrict__ rep_of_slot, int64_t slots,
                                                         const int32_t* __restrict__ key, int32_t* __restrict__ start,
                                                         int32_t* __restrict__ cnt, int32_t* __restrict__ tmp,
-                                                        int32_t buckets, int32_t active_total,
+                                                        int32_t buckets, const int32_t* __restrict__ active_total,
                                                         int32_t* __restrict__ tile_sums, int32_t tiles)
 {
     const int64_t g = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (g == 0) start[buckets] = active_total;
+    if (g == 0) start[buckets] = *active_total;
     // clear the scan's tile sums (tiles + 1 ints) for the next build's key producer
     for (int64_t k = g; k <= tiles; k += (int64_t)gridDim.x * blockDim.x) tile_sums[k] = 0;
     if (g >= slots) return;
@@ -561,7 +631,8 @@ __global__ void __launch_bounds__(256) batch_hg_scatter(const ReplicaDev* __rest
     tmp[start[b] + left - 1] = (int32_t)g;
 }
 
-__global__ void __launch_bounds__(256) batch_hg_order(const int32_t* __restrict__ key, int32_t n,
+__global__ void __launch_bounds__(256) batch_hg_order(const int32_t* __restrict__ key,
+                                                      const int32_t* __restrict__ active_total,
                                                       const int32_t* __restrict__ start,
                                                       const int32_t* __restrict__ tmp,
                                                       const float* __restrict__ position,
@@ -570,7 +641,7 @@ __global__ void __launch_bounds__(256) batch_hg_order(const int32_t* __restrict_
 {
     const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
     if (t == 0) *order_ready = 1;  // id holds a bucket order of the active slots from here on
-    if (t >= n) return;
+    if (t >= *active_total) return;
     hg_order_point(key, start, tmp, position, radius, t, id, xyzr);
 }
 
@@ -582,7 +653,8 @@ __global__ void __launch_bounds__(256) batch_contact_kernel(const ReplicaDev* __
                                                             const int32_t* __restrict__ rep_of_slot,
                                                             const int32_t* __restrict__ start,
                                                             const int32_t* __restrict__ id,
-                                                            const float4* __restrict__ xyzr, int32_t n, int32_t dim,
+                                                            const float4* __restrict__ xyzr,
+                                                            const int32_t* __restrict__ active_total, int32_t dim,
                                                             float* __restrict__ velocity,
                                                             int32_t* __restrict__ neighbor_count,
                                                             float* __restrict__ position,
@@ -591,7 +663,7 @@ __global__ void __launch_bounds__(256) batch_contact_kernel(const ReplicaDev* __
                                                             int32_t* __restrict__ next_tile_sums)
 {
     const int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
-    if (t >= n) return;
+    if (t >= *active_total) return;
     const int32_t i = id[t];
     const int r = rep_of_slot[i];
     const ReplicaDev& d = rep[r];
@@ -838,7 +910,8 @@ static int tiles_for(int tile, int nx, int ny, int nz)
 // ---- workspace layout -----------------------------------------------------------------------
 struct BatchLayout {
     size_t rep, state, bar, node_off, total_sweeps, partial, acc, density, buf1, buf2, rep_of_slot, cnt, start,
-        scan, key, tmp, id, order_ready, xyzr, wrep, rblk_off, wdone, wpartial, packed, total;
+        scan, key, tmp, id, order_ready, xyzr, wrep, rblk_off, wdone, wpartial, packed, dflag, dscan, dscan_ws,
+        daughters, cap_err, active, total;
     int64_t slots, nodes, buckets;
     int32_t partial_stride, active_total, wide_res_blocks;
     bool wide_ok;  // every replica's grid fits the colour-packed passes
@@ -873,7 +946,7 @@ static bool layout_for(const wbc_replica* r, int32_t R, int32_t dim, BatchLayout
         const int64_t n = (int64_t)d.nx * d.ny * d.nz;
         if (n >= ((int64_t)1 << 31)) return false;
         if (d.max_sweeps < 1 || d.check_every < 1 || !(d.omega > 0.0f && d.omega < 2.0f)) return false;
-        if (d.division_rate != 0.0f) return false;  // the batched step places no daughters
+        if (!(d.placement_factor > 0.0f)) return false;
         if (!(d.cell_size > 0.0f) || d.substeps != r[0].substeps || d.substeps < 0) return false;
         slots += d.capacity;
         nodes += n;
@@ -926,6 +999,12 @@ static bool layout_for(const wbc_replica* r, int32_t R, int32_t dim, BatchLayout
     L.wdone = off;        off += up(sizeof(int32_t) * R);
     L.wpartial = off;     off += up(sizeof(float) * res_blocks);
     L.packed = off;       off += up((size_t)packed_bytes);
+    L.dflag = off;        off += up(sizeof(int32_t) * slots);
+    L.dscan = off;        off += up(sizeof(int32_t) * slots);
+    L.dscan_ws = off;     off += up(scan_workspace_bytes(slots));
+    L.daughters = off;    off += up(sizeof(int32_t) * R);
+    L.cap_err = off;      off += up(sizeof(int32_t) * R);
+    L.active = off;       off += up(sizeof(int32_t));
     L.total = off;
     L.wide_res_blocks = (int32_t)res_blocks;
     L.wide_ok = wide_ok;
@@ -1152,8 +1231,8 @@ static hipError_t enqueue_pre(wbc_batch* b, hipStream_t s)
     WBC_TRY_ERR(hipGetLastError());
     if (L.slots > 0) {
         batch_deposit_kernel<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, (const int32_t*)(ws + L.id),
-                                                         (const int32_t*)(ws + L.order_ready), L.active_total,
-                                                         c.position, c.cell_state);
+                                                         (const int32_t*)(ws + L.order_ready),
+                                                         (const int32_t*)(ws + L.active), c.position, c.cell_state);
         WBC_TRY_ERR(hipGetLastError());
     }
     batch_finalize_kernel<<<grid_for(L.nodes, 256), 256, 0, s>>>(rep, b->R, (const int64_t*)(ws + L.node_off),
@@ -1162,23 +1241,34 @@ static hipError_t enqueue_pre(wbc_batch* b, hipStream_t s)
     return hipSuccess;
 }
 
-// The step after the solve: sample, lifecycle decisions (no daughters), contact substeps.
+// The step after the solve: sample, lifecycle decisions and divisions, contact substeps.
 static hipError_t enqueue_post(wbc_batch* b, hipStream_t s)
 {
     const BatchLayout& L = b->L;
     char* ws = b->ws;
-    const ReplicaDev* rep = (const ReplicaDev*)(ws + L.rep);
+    ReplicaDev* rep = (ReplicaDev*)(ws + L.rep);
     const int32_t* rep_of_slot = (const int32_t*)(ws + L.rep_of_slot);
     const wbc_batch_cells& c = b->cells;
     const unsigned slot_blocks = grid_for(L.slots, 256);
     if (L.slots == 0) return hipSuccess;
     batch_sample_kernel<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, c.position, c.oxygen_local);
     WBC_TRY_ERR(hipGetLastError());
-    // Lifecycle decisions (no daughters).
-    batch_lifecycle_kernel<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, c);
+    // Lifecycle decisions, then commit_divisions of every replica.
+    int32_t* flags = (int32_t*)(ws + L.dflag);
+    int32_t* dscan = (int32_t*)(ws + L.dscan);
+    int32_t* daughters = (int32_t*)(ws + L.daughters);
+    int32_t* active = (int32_t*)(ws + L.active);
+    batch_lifecycle_kernel<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, c, flags);
     WBC_TRY_ERR(hipGetLastError());
-    // Contact relaxation: substeps x (build, gather + integrate).
-    if (L.active_total == 0) return hipSuccess;
+    WBC_TRY_ERR(scan_i32(flags, dscan, L.slots, (int32_t*)(ws + L.dscan_ws), false, s));
+    batch_division_totals_kernel<<<1, 64, 0, s>>>(rep, b->R, dscan, daughters, (int32_t*)(ws + L.cap_err));
+    WBC_TRY_ERR(hipGetLastError());
+    batch_place_kernel<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, dscan, daughters, c);
+    WBC_TRY_ERR(hipGetLastError());
+    batch_commit_counts_kernel<<<1, 64, 0, s>>>(rep, b->R, daughters, active);
+    WBC_TRY_ERR(hipGetLastError());
+    // Contact relaxation: substeps x (build, gather + integrate), over the active slots after the
+    // divisions (their number is on the device; the launches cover every slot).
     int32_t* key = (int32_t*)(ws + L.key);
     int32_t* cnt = (int32_t*)(ws + L.cnt);
     int32_t* start = (int32_t*)(ws + L.start);
@@ -1197,20 +1287,19 @@ static hipError_t enqueue_post(wbc_batch* b, hipStream_t s)
             WBC_TRY_ERR(scan_i32_presummed(cnt, start, L.buckets, tile_sums, true, s));
         const int32_t tiles = (int32_t)((L.buckets + kScanTile - 1) / kScanTile);
         batch_hg_scatter<<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, L.slots, key, start, cnt, tmp,
-                                                     (int32_t)L.buckets, L.active_total, tile_sums, tiles);
+                                                     (int32_t)L.buckets, active, tile_sums, tiles);
         WBC_TRY_ERR(hipGetLastError());
-        batch_hg_order<<<grid_for(L.active_total, 256), 256, 0, s>>>(key, L.active_total, start, tmp, c.position,
-                                                                    c.radius, id, xyzr,
-                                                                    (int32_t*)(ws + L.order_ready));
+        batch_hg_order<<<slot_blocks, 256, 0, s>>>(key, active, start, tmp, c.position, c.radius, id, xyzr,
+                                                   (int32_t*)(ws + L.order_ready));
         WBC_TRY_ERR(hipGetLastError());
         if (sub + 1 < b->substeps)
-            batch_contact_kernel<true><<<grid_for(L.active_total, 256), 256, 0, s>>>(
-                rep, rep_of_slot, start, id, xyzr, L.active_total, b->dim, c.velocity, c.neighbor_count, c.position,
-                key, cnt, tile_sums);
+            batch_contact_kernel<true><<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, start, id, xyzr, active, b->dim,
+                                                                   c.velocity, c.neighbor_count, c.position, key, cnt,
+                                                                   tile_sums);
         else
-            batch_contact_kernel<false><<<grid_for(L.active_total, 256), 256, 0, s>>>(
-                rep, rep_of_slot, start, id, xyzr, L.active_total, b->dim, c.velocity, c.neighbor_count, c.position,
-                key, cnt, tile_sums);
+            batch_contact_kernel<false><<<slot_blocks, 256, 0, s>>>(rep, rep_of_slot, start, id, xyzr, active,
+                                                                    b->dim, c.velocity, c.neighbor_count, c.position,
+                                                                    key, cnt, tile_sums);
         WBC_TRY_ERR(hipGetLastError());
     }
     return hipSuccess;
@@ -1280,9 +1369,9 @@ WBC_API int wbc_batch_create(const wbc_replica* replicas, int32_t count, int32_t
     BatchLayout L;
     if (!layout_for(replicas, count, hash_dim, L) || workspace_bytes < L.total) return (int)hipErrorInvalidValue;
     const wbc_batch_cells& c = *cells;
-    if (L.slots > 0 && (!c.position || !c.radius || !c.cell_state || !c.age || !c.oxygen_local || !c.glucose_local ||
-                        !c.velocity || !c.neighbor_count || !c.divide_flag || !c.division_offset || !c.fate_flags ||
-                        !c.rng_state))
+    if (L.slots > 0 && (!c.position || !c.radius || !c.cell_state || !c.cell_type || !c.age || !c.oxygen_local ||
+                        !c.glucose_local || !c.velocity || !c.neighbor_count || !c.divide_flag ||
+                        !c.division_offset || !c.fate_flags || !c.rng_state))
         return (int)hipErrorInvalidValue;
     char* ws = (char*)workspace;
     std::vector<ReplicaDev> dev(count);
@@ -1316,6 +1405,8 @@ WBC_API int wbc_batch_create(const wbc_replica* replicas, int32_t count, int32_t
         d.check_every = h.check_every;
         d.cell_offset = h.cell_offset;
         d.count = h.count;
+        d.capacity = h.capacity;
+        d.placement_factor = h.placement_factor;
         d.node_offset = h.node_offset;
         d.nodes = (int64_t)h.nx * h.ny * h.nz;
         d.ox = h.origin[0];
@@ -1352,6 +1443,10 @@ WBC_API int wbc_batch_create(const wbc_replica* replicas, int32_t count, int32_t
     WBC_TRY(hipMemsetAsync(ws + L.order_ready, 0, sizeof(int32_t), stream));
     WBC_TRY(hipMemsetAsync(ws + L.cnt, 0, sizeof(int32_t) * L.buckets, stream));
     WBC_TRY(hipMemsetAsync(ws + L.density, 0, sizeof(float) * L.nodes, stream));
+    WBC_TRY(hipMemsetAsync(ws + L.daughters, 0, sizeof(int32_t) * count, stream));
+    WBC_TRY(hipMemsetAsync(ws + L.cap_err, 0, sizeof(int32_t) * count, stream));
+    const int32_t active_total = L.active_total;
+    WBC_TRY(hipMemcpyAsync(ws + L.active, &active_total, sizeof(int32_t), hipMemcpyHostToDevice, stream));
     WBC_TRY(hipStreamSynchronize(stream));  // the host vectors above go out of scope
 
     wbc_batch* b = new (std::nothrow) wbc_batch();
@@ -1523,6 +1618,24 @@ WBC_API int wbc_batch_report(wbc_batch_t b, hipStream_t stream, int32_t* sweeps,
         if (total_sweeps) total_sweeps[r] = tot[r];
     }
     if (error) *error = bar.error[0];
+    return 0;
+}
+
+WBC_API int wbc_batch_cell_counts(wbc_batch_t b, hipStream_t stream, int32_t* counts, int32_t* daughters,
+                                  int32_t* capacity_error)
+{
+    if (!b) return (int)hipErrorInvalidValue;
+    WBC_TRY(hipStreamSynchronize(stream));
+    std::vector<ReplicaDev> dev(b->R);
+    std::vector<int32_t> dn(b->R), err(b->R);
+    WBC_TRY(hipMemcpy(dev.data(), b->ws + b->L.rep, sizeof(ReplicaDev) * b->R, hipMemcpyDeviceToHost));
+    WBC_TRY(hipMemcpy(dn.data(), b->ws + b->L.daughters, sizeof(int32_t) * b->R, hipMemcpyDeviceToHost));
+    WBC_TRY(hipMemcpy(err.data(), b->ws + b->L.cap_err, sizeof(int32_t) * b->R, hipMemcpyDeviceToHost));
+    for (int r = 0; r < b->R; ++r) {
+        if (counts) counts[r] = dev[r].count;
+        if (daughters) daughters[r] = dn[r];
+        if (capacity_error) capacity_error[r] = err[r];
+    }
     return 0;
 }
 

@@ -462,4 +462,40 @@ __device__ __forceinline__ void lifecycle_decide_cell(const LifecycleArgs& a, in
     rng_state[i] = rng;
 }
 
+
+// place_daughters for one dividing parent i (kernels/cell_kernels.py:157-200): the daughter takes
+// slot j; parent and daughter sit at x -/+ 0.5 placement_factor r dir, dir a uniform direction
+// drawn from the parent's stream; the daughter copies radius, state, type, oxygen, glucose and
+// fate flags, starts at age 0 with zero velocity and no neighbours, and keeps slot j's own RNG
+// stream (seeded for every slot when the population was created).
+__device__ __forceinline__ void place_daughter(int64_t i, int64_t j, float placement_factor, float* __restrict__ position,
+                                               float* __restrict__ radius, int32_t* __restrict__ cell_state,
+                                               int32_t* __restrict__ cell_type, float* __restrict__ age,
+                                               float* __restrict__ oxygen_local, float* __restrict__ glucose_local,
+                                               uint32_t* __restrict__ rng_state, float* __restrict__ velocity,
+                                               int32_t* __restrict__ neighbor_count, int32_t* __restrict__ fate_flags)
+{
+    uint32_t rng = rng_state[i];
+    float d[3];
+    sample_unit_sphere_surface(rng, d[0], d[1], d[2]);
+    rng_state[i] = rng;
+    const float s = 0.5f * placement_factor * radius[i];
+#pragma unroll
+    for (int a = 0; a < 3; ++a) {
+        const float h = d[a] * s;
+        const float x = position[3 * i + a];
+        position[3 * i + a] = x - h;
+        position[3 * j + a] = x + h;
+        velocity[3 * j + a] = 0.0f;
+    }
+    radius[j] = radius[i];
+    cell_state[j] = cell_state[i];
+    if (cell_type) cell_type[j] = cell_type[i];
+    age[j] = 0.0f;
+    oxygen_local[j] = oxygen_local[i];
+    glucose_local[j] = glucose_local[i];
+    neighbor_count[j] = 0;
+    fate_flags[j] = fate_flags[i];
+}
+
 }  // namespace wbc

@@ -37,29 +37,9 @@ __global__ void __launch_bounds__(256) place_daughters_kernel(
 {
     int64_t i = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (i >= n || divide_flag[i] == 0) return;
-    int64_t j = (int64_t)count + division_offset[i] - 1;
-    uint32_t rng = rng_state[i];
-    float d[3];
-    sample_unit_sphere_surface(rng, d[0], d[1], d[2]);
-    rng_state[i] = rng;
-    float s = 0.5f * placement_factor * radius[i];
-#pragma unroll
-    for (int a = 0; a < 3; ++a) {
-        float h = d[a] * s;
-        float x = position[3 * i + a];
-        position[3 * i + a] = x - h;
-        position[3 * j + a] = x + h;
-        velocity[3 * j + a] = 0.0f;
-    }
-    radius[j] = radius[i];
-    cell_state[j] = cell_state[i];
-    cell_type[j] = cell_type[i];
-    age[j] = 0.0f;
-    oxygen_local[j] = oxygen_local[i];
-    glucose_local[j] = glucose_local[i];
-    neighbor_count[j] = 0;
-    fate_flags[j] = fate_flags[i];
-    // rng_state[j] was seeded for every slot when the population was created; left untouched.
+    const int64_t j = (int64_t)count + division_offset[i] - 1;
+    place_daughter(i, j, placement_factor, position, radius, cell_state, cell_type, age, oxygen_local, glucose_local,
+                   rng_state, velocity, neighbor_count, fate_flags);
 }
 
 // The total of an inclusive scan, offset[n - 1], into mapped host memory.

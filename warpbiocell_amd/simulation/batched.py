@@ -7,7 +7,8 @@ with its own cell count and grid, and packs them into one set of device arrays. 
 all of them per call of :meth:`BatchedSweep.step`. A step is, per replica:
 
 1. the oxygen update: deposit, steady-state solve, sample;
-2. the lifecycle decisions;
+2. the lifecycle step: decisions, then the replica's divisions (commit_divisions, cells/lifecycle.py:
+   135-179: its own slot scan, daughters appended after its live cells, inside its own capacity);
 3. the contact relaxation substeps.
 
 The solves of all replicas run under one of two schedule families (``set_schedule``, chosen by
@@ -15,11 +16,16 @@ The solves of all replicas run under one of two schedule families (``set_schedul
 step captured once in a hipGraph and replayed; or wide launches, one per colour pass over every
 replica's colour-packed planes, driven by the host chunk by chunk between two captured graphs
 (the step's parts before and after the solve). Each replica's state after a step is
-bit-identical to ``cell_step`` on that replica alone, including its solve report.
+bit-identical to ``cell_step`` on that replica alone, including its solve report and daughter
+count. Cell counts change on the device; :meth:`BatchedSweep.counts` reads them and
+:meth:`BatchedSweep.write_back` carries them into the replicas' populations.
+
+When a replica's daughters would exceed its capacity, none are placed for it and its step goes on
+(the reference's ``commit_divisions`` raises ``CapacityError`` there instead, before the contact
+substeps); the next read of the counts or reports raises ``CapacityError`` naming the replica.
 
 Limits:
 
-- no divisions (``division_rate`` must be 0);
 - all faces Dirichlet, no pinned nodes, no production term;
 - the same number of contact substeps in every replica;
 - no tissue region and no gene network.
@@ -36,7 +42,7 @@ from dataclasses import dataclass
 import torch
 
 from warpbiocell_amd import _lib
-from warpbiocell_amd.cells.lifecycle import LifecycleParams
+from warpbiocell_amd.cells.lifecycle import CapacityError, LifecycleParams
 from warpbiocell_amd.cells.mechanics import ContactParams
 from warpbiocell_amd.cells.state import CellPopulation
 from warpbiocell_amd.fields.diffusion import SolveReport, float32_threshold, solver_scalars
@@ -46,7 +52,7 @@ from warpbiocell_amd.simulation.simulator import StepReport, TimeStepping
 MAX_REPLICAS = 64
 # Per-cell arrays of CellPopulation carried by the batch, in the order of wbc_batch_cells.
 CELL_FIELDS = ("position", "radius", "cell_state", "age", "oxygen_local", "glucose_local", "velocity",
-               "neighbor_count", "divide_flag", "division_offset", "fate_flags", "rng_state")
+               "neighbor_count", "divide_flag", "division_offset", "fate_flags", "rng_state", "cell_type")
 
 
 class ReplicaDesc(ctypes.Structure):
@@ -69,6 +75,7 @@ class ReplicaDesc(ctypes.Structure):
         ("apoptosis_rate", c_float), ("necrosis_rate", c_float),
         ("cell_size", c_float), ("query_radius", c_float), ("contact_rate", c_float), ("dt_mechanics", c_float),
         ("substeps", c_int32),
+        ("placement_factor", c_float),
     ]
 
 
@@ -94,8 +101,6 @@ class Replica:
 
 
 def _check_replica(r: Replica) -> None:
-    if r.lifecycle.division_rate != 0.0:
-        raise NotImplementedError("the batched step places no daughters: division_rate must be 0")
     if r.lifecycle.uses_network:
         raise NotImplementedError("the per-cell gene network is not part of the HIP hot path")
     f = r.oxygen.field
@@ -137,6 +142,7 @@ def replica_desc(r: Replica, cell_offset: int, node_offset: int) -> ReplicaDesc:
     d.cell_size = r.cell_size if r.cell_size is not None else r.contact.query_radius
     d.query_radius, d.contact_rate = r.contact.query_radius, r.contact.rate
     d.dt_mechanics, d.substeps = r.stepping.dt_mechanics, r.stepping.mechanics_substeps
+    d.placement_factor = life.placement_factor
     return d
 
 
@@ -153,6 +159,9 @@ class BatchedSweep:
             raise ValueError("a sweep needs at least one replica")
         if len(replicas) > MAX_REPLICAS:
             raise ValueError(f"at most {MAX_REPLICAS} replicas per batch")
+        if (_lib.query("wbc_batch_struct_size", 0) != ctypes.sizeof(ReplicaDesc)
+                or _lib.query("wbc_batch_struct_size", 1) != ctypes.sizeof(BatchCells)):
+            raise RuntimeError("the library's batch structs do not match this binding: rebuild the library")
         self.device = _lib.require_gpu(device if device is not None else replicas[0].population.device)
         for r in replicas:
             _check_replica(r)
@@ -244,7 +253,23 @@ class BatchedSweep:
         return [SolveReport(sweeps[i], float(residual[i]), bool(conv[i])) for i in range(n)]
 
     def step_reports(self) -> list[StepReport]:
-        return [StepReport(0, r.sweeps, r.residual, r.converged) for r in self.report()]
+        """The last step of every replica as ``cell_step`` reports it: daughters and solve."""
+        _, daughters = self.counts(with_daughters=True)
+        return [StepReport(d, r.sweeps, r.residual, r.converged) for d, r in zip(daughters, self.report())]
+
+    def counts(self, with_daughters: bool = False):
+        """Every replica's current cell count (synchronises the stream), and with ``with_daughters``
+        the daughters its last step placed; raises ``CapacityError`` once a replica's daughters did
+        not fit its capacity."""
+        n = len(self.replicas)
+        counts, daughters, err = (c_int32 * n)(), (c_int32 * n)(), (c_int32 * n)()
+        _lib.call("wbc_batch_cell_counts", self._h(), self.stream, ctypes.byref(counts), ctypes.byref(daughters),
+                  ctypes.byref(err))
+        bad = [self.replicas[i].label or str(i) for i in range(n) if err[i]]
+        if bad:
+            raise CapacityError(f"divisions would exceed the capacity of replica(s) {', '.join(bad)}; "
+                                f"no daughters were placed for them")
+        return (list(counts), list(daughters)) if with_daughters else list(counts)
 
     def total_sweeps(self) -> list[int]:
         n = len(self.replicas)
@@ -267,8 +292,12 @@ class BatchedSweep:
         return self.field[off: off + g.node_count].view(*g.shape)
 
     def write_back(self) -> None:
-        """Copy the batched state into every replica's population and oxygen field."""
+        """Copy the batched state into every replica's population (cell arrays and count) and
+        oxygen field."""
+        counts = self.counts()
         for i, r in enumerate(self.replicas):
             for name in CELL_FIELDS:
                 getattr(r.population, name).copy_(self.population_view(i, name))
+            r.population.count = counts[i]
+            r.population.forget_deposit_order()  # an order recorded before these steps is stale
             r.oxygen.values.copy_(self.field_view(i))

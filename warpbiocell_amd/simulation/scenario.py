@@ -1,9 +1,9 @@
 """Device objects of one synthetic workload: the arguments ``cell_step`` receives for it.
 
-Physics and parameters are those of ``warpbiocell_amd.workloads``. The lifecycle has no divisions
-or deaths unless the workload asks for divisions, and no contact inhibition. Contacts use rate
-``CONTACT_RATE`` with the query radius set by the largest cell. The steps are ``DT_CELLS`` with
-``DT_MECHANICS`` substeps.
+Physics and parameters are those of ``warpbiocell_amd.workloads``. The lifecycle is the workload's
+own when it names one; otherwise it has no deaths, no contact inhibition and divisions only when
+the workload asks for them. Contacts use rate ``CONTACT_RATE`` with the query radius set by the
+largest cell. The steps are ``DT_CELLS`` with ``DT_MECHANICS`` substeps.
 """
 
 from __future__ import annotations
@@ -55,7 +55,10 @@ def build(w: W.Workload, device, hash_dim: int = 128) -> Scenario:
     params = OxygenParams(W.D_UM2_PER_H, W.UPTAKE_MMHG_UM3_PER_H, W.MICHAELIS_K, w.boundary_mmHg)
     settings = SolverSettings(omega=w.omega, tolerance=w.tolerance, max_sweeps=w.max_sweeps,
                               check_every=w.check_every)
-    lifecycle = LifecycleParams(division_rate=w.division_rate, death_rate=0.0, inhibition_threshold=10 ** 6)
+    if w.lifecycle is not None:
+        lifecycle = LifecycleParams(**w.lifecycle)
+    else:
+        lifecycle = LifecycleParams(division_rate=w.division_rate, death_rate=0.0, inhibition_threshold=10 ** 6)
     contact = ContactParams(stiffness=W.CONTACT_RATE, damping=1.0, max_radius=float(w.radii.max()))
     stepping = TimeStepping(dt_cells=W.DT_CELLS, dt_mechanics=W.DT_MECHANICS)
     pop = CellPopulation.from_numpy(w.positions, w.radii, capacity=w.capacity, device=device, seed=w.seed)

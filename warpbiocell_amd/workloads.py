@@ -18,7 +18,10 @@ substeps per step). Cells are spheres with uniformly random centres in a ball.
   sweep_15            15 replicas = boundary {20, 40, 60, 100, 150} mmHg x seeds {0, 1, 2}; cell
                       counts log-uniform in [2,000, 200,000] and grids drawn from {41, 81, 121,
                       161}^3 over a 1600 um box (one draw per replica from a fixed generator);
-                      cells as spheroid_100k_161; solver as spheroid_1k_41; 100 steps.
+                      cells as spheroid_100k_161; solver as spheroid_1k_41; 100 steps; the
+                      lifecycle of the reference's sweep base (configs/tumor_spheroid.yaml:31-38:
+                      division 0.0289 /h, anoxic death 0.5 /h below 2 mmHg, hypoxia below 8 mmHg,
+                      contact inhibition at 8 neighbours), capacity 1.4 n + 64 for the growth.
 """
 
 from __future__ import annotations
@@ -53,6 +56,7 @@ class Workload:
     capacity: int | None = None
     seed: int = 0
     grid_nodes: int | None = None  # nodes per axis when set (dx = box / (nodes - 1)), else from dx
+    lifecycle: dict | None = None  # LifecycleParams keyword arguments, else divisions only (no deaths)
 
     @property
     def n(self) -> int:
@@ -121,6 +125,10 @@ def adjoint_100k_129(seed: int = 0, n: int = 100_000):
 
 
 SWEEP_BOUNDARIES = (20.0, 40.0, 60.0, 100.0, 150.0)
+# The lifecycle block of the sweep's base experiment (configs/tumor_spheroid.yaml:31-38), in
+# LifecycleParams keywords.
+SWEEP_LIFECYCLE = dict(division_rate=0.0289, death_rate=0.0, anoxic_death_rate=0.5, inhibition_threshold=8,
+                       hypoxia_threshold=8.0, death_threshold=2.0, placement_factor=1.0)
 SWEEP_SEEDS = (0, 1, 2)
 SWEEP_GRIDS = (41, 81, 121, 161)
 
@@ -138,5 +146,8 @@ def sweep_15(draw_seed: int = 0, box_um: float = 1600.0) -> list[Workload]:
             w.name = f"boundary_mmHg={ob:g}_seed={seed}"
             w.boundary_mmHg = ob
             w.box_um, w.dx_um, w.grid_nodes = box_um, box_um / (g - 1), g
+            w.lifecycle = dict(SWEEP_LIFECYCLE)
+            w.division_rate = SWEEP_LIFECYCLE["division_rate"]
+            w.capacity = int(1.4 * n) + 64
             out.append(w)
     return out
