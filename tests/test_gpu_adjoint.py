@@ -47,6 +47,39 @@ def test_gradients_match_the_fp64_adjoint(oracle, device):
     assert errs["S"] < 1e-4 and errs["dq"] < 2e-3 and errs["dD"] < 2e-3 and errs["dOb"] < 2e-3
 
 
+def test_two_forwards_then_both_backwards(device):
+    """Two sampled_oxygen calls on one problem (different D and O_b) before either backward: each
+    gradient is that of its own forward, equal to a forward+backward done alone within the adjoint
+    solve's tolerance (the adjoint SOR starts from the problem's previous adjoint solution, so the
+    order of the backwards moves the last bits); the two forwards' gradients differ by far more."""
+    pos, rates, targets, geom = _moderate(n=800)
+    prob = OxygenAdjointProblem(geom, torch.from_numpy(pos), 1.0, W.UPTAKE_MMHG_UM3_PER_H, device=device)
+    t = torch.from_numpy(targets).to(device)
+    cases = [(W.D_UM2_PER_H, 40.0), (0.5 * W.D_UM2_PER_H, 30.0)]
+
+    def alone(D, Ob):
+        q = torch.from_numpy(rates).to(device).requires_grad_(True)
+        Dt = torch.tensor(D, dtype=torch.float64, requires_grad=True)
+        Bt = torch.tensor(Ob, dtype=torch.float64, requires_grad=True)
+        ((sampled_oxygen(q, Dt, Bt, prob) - t) ** 2).sum().backward()
+        return q.grad.cpu().numpy(), float(Dt.grad), float(Bt.grad)
+
+    expected = [alone(D, Ob) for D, Ob in cases]
+    leaves, losses = [], []
+    for D, Ob in cases:
+        q = torch.from_numpy(rates).to(device).requires_grad_(True)
+        Dt = torch.tensor(D, dtype=torch.float64, requires_grad=True)
+        Bt = torch.tensor(Ob, dtype=torch.float64, requires_grad=True)
+        leaves.append((q, Dt, Bt))
+        losses.append(((sampled_oxygen(q, Dt, Bt, prob) - t) ** 2).sum())
+    for loss in losses:
+        loss.backward()
+    for (q, Dt, Bt), (eq, eD, eB) in zip(leaves, expected):
+        assert _rel(q.grad.cpu().numpy(), eq) < 1e-4
+        assert abs(float(Dt.grad) - eD) <= 1e-4 * abs(eD) and abs(float(Bt.grad) - eB) <= 1e-4 * abs(eB)
+    assert _rel(expected[0][0], expected[1][0]) > 0.05
+
+
 def test_autograd_function_routes_the_adjoint(device):
     pos, rates, targets, geom = _moderate(n=800)
     prob = OxygenAdjointProblem(geom, torch.from_numpy(pos), 1.0, W.UPTAKE_MMHG_UM3_PER_H, device=device)

@@ -50,6 +50,17 @@ class AdjointSettings:
             raise ValueError("max_sweeps, check_every and tolerance must be positive")
 
 
+@dataclass(frozen=True)
+class ForwardState:
+    """What a backward pass needs from one forward solve: the converged field, the deposited rate
+    density Q / q_ref and D. Each forward allocates its own, so a later forward on the same problem
+    leaves an earlier one's state intact."""
+
+    field: ScalarField
+    rate_density: torch.Tensor
+    diffusion: float
+
+
 class OxygenAdjointProblem:
     """Buffers for repeated forward / backward solves on one grid and one set of cell positions."""
 
@@ -69,14 +80,23 @@ class OxygenAdjointProblem:
         self.adjoint_settings = adjoint
         shape = geometry.shape
         self._acc = torch.zeros(shape, dtype=torch.int64, device=self.device)
-        self.rate_density = torch.zeros(shape, dtype=torch.float32, device=self.device)  # Q / q_ref
         self._g = torch.zeros(shape, dtype=torch.float32, device=self.device)
         self.mu = torch.zeros(shape, dtype=torch.float32, device=self.device)
         nbytes = _lib.query("wbc_adjoint_workspace_bytes", *shape)
         self._ws = torch.empty(nbytes, dtype=torch.uint8, device=self.device)
-        self.field: ScalarField | None = None
+        self.state: ForwardState | None = None  # of the last forward
         self.last_forward = None
         self.last_backward = None
+
+    @property
+    def field(self) -> ScalarField | None:
+        """The last forward's converged field."""
+        return None if self.state is None else self.state.field
+
+    @property
+    def rate_density(self) -> torch.Tensor | None:
+        """The last forward's deposited rate density Q / q_ref."""
+        return None if self.state is None else self.state.rate_density
 
     @property
     def n(self) -> int:
@@ -96,21 +116,25 @@ class OxygenAdjointProblem:
         _lib.call("wbc_fill_zero_i64", self._acc.data_ptr(), self._acc.numel(), s)
         _lib.call("wbc_deposit_rates_i64", self.positions.data_ptr(), rates.data_ptr(), self.n, 1.0 / self.q_ref,
                   g.origin[0], g.origin[1], g.origin[2], 1.0 / g.dx, nx, ny, nz, self._acc.data_ptr(), s)
+        rate_density = torch.empty(g.shape, dtype=torch.float32, device=self.device)
         _lib.call("wbc_fixed_to_float", self._acc.data_ptr(), g.node_count, 1.0 / g.voxel_volume,
-                  self.rate_density.data_ptr(), s)
-        self.field = ScalarField.create(g, float(boundary_value), device=self.device)
+                  rate_density.data_ptr(), s)
+        field = ScalarField.create(g, float(boundary_value), device=self.device)
         kin = UptakeKinetics(float(diffusion), self.q_ref, self.michaelis_k)
-        report = solve_steady_state(self.field, self.rate_density, kin, self.settings)
+        report = solve_steady_state(field, rate_density, kin, self.settings)
         out = torch.empty(self.n, dtype=torch.float32, device=self.device)
-        _lib.call("wbc_sample_trilinear", self.field.values.data_ptr(), nx, ny, nz, self.positions.data_ptr(), self.n,
+        _lib.call("wbc_sample_trilinear", field.values.data_ptr(), nx, ny, nz, self.positions.data_ptr(), self.n,
                   g.origin[0], g.origin[1], g.origin[2], 1.0 / g.dx, out.data_ptr(), s)
         self.last_forward = report
-        self._diffusion = float(diffusion)
+        self.state = ForwardState(field, rate_density, float(diffusion))
         return out
 
-    def backward(self, grad_sample: torch.Tensor) -> tuple[torch.Tensor, float, float]:
-        """(dL/dq, dL/dD, dL/dO_b) for dL/dS = grad_sample, at the last forward solution."""
-        if self.field is None:
+    def backward(self, grad_sample: torch.Tensor, state: ForwardState | None = None
+                 ) -> tuple[torch.Tensor, float, float]:
+        """(dL/dq, dL/dD, dL/dO_b) for dL/dS = grad_sample, at the solution of the forward whose
+        ``state`` is given (default: the last forward)."""
+        state = self.state if state is None else state
+        if state is None:
             raise RuntimeError("backward needs a forward solve first")
         g = self.geometry
         nx, ny, nz = g.shape
@@ -119,18 +143,18 @@ class OxygenAdjointProblem:
         self._g.zero_()
         _lib.call("wbc_sample_adjoint", go.data_ptr(), self.positions.data_ptr(), self.n, g.origin[0], g.origin[1],
                   g.origin[2], 1.0 / g.dx, nx, ny, nz, self._g.data_ptr(), s)
-        h = g.dx ** 2 / self._diffusion
+        h = g.dx ** 2 / state.diffusion
         a = self.adjoint_settings
         omega = a.omega if a.omega is not None else 2.0 / (1.0 + math.sin(math.pi / (max(g.shape) - 1)))
         grad_rate = torch.empty(self.n, dtype=torch.float32, device=self.device)
         sums = (ctypes.c_double * 2)()
         sweeps, residual = ctypes.c_int32(), ctypes.c_float()
         _lib.call("wbc_adjoint_solve", omega, a.max_sweeps, float32_threshold(a.tolerance), a.check_every, h, self.q_ref,
-                  self.michaelis_k, None, self.field.values.data_ptr(), self.rate_density.data_ptr(), None,
+                  self.michaelis_k, None, state.field.values.data_ptr(), state.rate_density.data_ptr(), None,
                   self._g.data_ptr(), self.mu.data_ptr(), self.positions.data_ptr(), self.n, g.origin[0], g.origin[1],
                   g.origin[2], g.dx, nx, ny, nz, grad_rate.data_ptr(), ctypes.byref(sums), ctypes.byref(sweeps),
                   ctypes.byref(residual), self._ws.data_ptr(), self._ws.numel(), s)
-        grad_D = -(h / self._diffusion) * sums[0]
+        grad_D = -(h / state.diffusion) * sums[0]
         grad_Ob = sums[1]
         self.last_backward = (sweeps.value, residual.value)
         return grad_rate, grad_D, grad_Ob
@@ -143,12 +167,13 @@ class SteadyStateOxygen(torch.autograd.Function):
     def forward(ctx, rates, diffusion, boundary_value, problem: OxygenAdjointProblem):
         out = problem.forward(rates, float(diffusion), float(boundary_value))
         ctx.problem = problem
+        ctx.state = problem.state  # this forward's own field and rate density
         ctx.shapes = (diffusion, boundary_value)
         return out
 
     @staticmethod
     def backward(ctx, grad_out):
-        grad_rate, grad_D, grad_Ob = ctx.problem.backward(grad_out)
+        grad_rate, grad_D, grad_Ob = ctx.problem.backward(grad_out, ctx.state)
         d, b = ctx.shapes
         gD = torch.tensor(grad_D, dtype=d.dtype, device=d.device) if torch.is_tensor(d) else None
         gB = torch.tensor(grad_Ob, dtype=b.dtype, device=b.device) if torch.is_tensor(b) else None
