@@ -13,8 +13,13 @@ field once, the sweep writes it once and reads the density once), against MI355X
 peak. The fused kernel itself needs only 12 B per node (one field read); that figure is
 reported beside it.
 
-CPU baseline: the oracle (C restatement of the reference, one thread) runs ONE step of the same
-workload from the exact state the GPU's first timed step started from.
+CPU leg (the only part of the benchmark that uses the oracle, the C restatement of the reference,
+and only as the checker): the oracle runs the first CPU_STEPS steps of the timed region from the
+exact state they started at, on every host core the process may use (and the first two on one
+thread); the GPU re-runs the same steps from a device copy of that state, so both sides are timed
+on the same inputs and sweeps. The same leg reports accuracy against float64: the final headline
+field against a float64 solve of the same discrete problem started from it, and the gradient of
+the 129^3 adjoint workload against the float64 adjoint and central differences of float64 solves.
 """
 
 from __future__ import annotations
@@ -31,6 +36,7 @@ ROOT = Path(__file__).resolve().parent
 sys.path.insert(0, str(ROOT))
 
 HBM_PEAK_GBPS = 8000.0
+CPU_STEPS = 10  # timed steps the CPU leg repeats on the oracle and the GPU
 METRIC = "cell-steps/s @100k cells,161\u00b3 SOR; batched-sweep & adjoint speedup vs plain"
 
 
@@ -152,12 +158,13 @@ def adjoint_measurement(device) -> dict:
     S = prob.forward(q, W.D_UM2_PER_H, w.boundary_mmHg)
     torch.cuda.synchronize()
     t1 = time.perf_counter()
-    prob.backward(2.0 * (S - T))
+    gq, gD, gOb = prob.backward(2.0 * (S - T))
     torch.cuda.synchronize()
     t2 = time.perf_counter()
+    gradient = {"rates": gq.cpu().numpy(), "D": gD, "O_b": gOb}
     return {"workload": w.name, "forward_ms": round((t1 - t0) * 1e3, 3), "forward_sweeps": prob.last_forward.sweeps,
             "adjoint_ms": round((t2 - t1) * 1e3, 3), "adjoint_sweeps": prob.last_backward[0],
-            "adjoint_over_forward": round((t2 - t1) / (t1 - t0), 3)}
+            "adjoint_over_forward": round((t2 - t1) / (t1 - t0), 3)}, gradient
 
 
 def confinement_measurement(device, repeats: int = 25) -> dict:
@@ -281,8 +288,43 @@ def host_threads() -> int:
     return max(1, min(n, int(env))) if env.isdigit() and int(env) > 0 else n
 
 
-def oracle_step_seconds(snap, w, geom, params, settings, lifecycle, contact, stepping, threads: int):
-    """One oracle step from the snapshot on `threads` host threads: (seconds, cells, sweeps)."""
+def device_copy(pop, ox):
+    """A device copy of the population's arrays and count and of the field (restored by restore_copy)."""
+    arrays = {name: getattr(pop, name).clone() for name in
+              ("position", "radius", "cell_state", "cell_type", "age", "oxygen_local", "glucose_local", "velocity",
+               "neighbor_count", "divide_flag", "division_offset", "fate_flags", "rng_state")}
+    return arrays, pop.count, ox.values.clone()
+
+
+def restore_copy(copy, pop, ox):
+    arrays, count, field = copy
+    for name, t in arrays.items():
+        getattr(pop, name).copy_(t)
+    pop.count = count
+    pop.forget_deposit_order()
+    ox.values.copy_(field)
+
+
+def gpu_same_steps(copy, pop, ox, grid, lifecycle, contact, stepping, steps: int) -> dict:
+    """The first `steps` timed steps again on the GPU, from the device copy of their start state."""
+    import torch
+
+    from warpbiocell_amd.simulation.simulator import cell_step
+
+    restore_copy(copy, pop, ox)
+    torch.cuda.synchronize()
+    cells, sweeps = 0, []
+    t0 = time.perf_counter()
+    for _ in range(steps):
+        cells += pop.count
+        sweeps.append(cell_step(pop, grid, lifecycle, contact, stepping, oxygen=ox).field_sweeps)
+    torch.cuda.synchronize()
+    dt = time.perf_counter() - t0
+    return {"value": round(cells / dt, 1), "seconds": round(dt, 4), "sweeps": sweeps}
+
+
+def oracle_steps_seconds(snap, w, geom, params, settings, lifecycle, contact, stepping, threads: int, steps: int):
+    """`steps` oracle steps from the snapshot on `threads` host threads: (seconds, cell-steps, sweeps per step)."""
     from oracle import oracle as O
 
     pop = O.OraclePopulation.from_numpy(snap["position"][: snap["count"]], snap["radius"][: snap["count"]],
@@ -297,23 +339,77 @@ def oracle_step_seconds(snap, w, geom, params, settings, lifecycle, contact, ste
                          max_sweeps=settings.max_sweeps, tolerance=settings.tolerance,
                          check_every=settings.check_every, values=snap["field"].copy())
     O.set_threads(threads)
-    n = pop.count
+    cells, sweeps = 0, []
     t0 = time.perf_counter()
-    _, field = O.oracle_cell_step(pop, lifecycle, contact, stepping, oxygen=rox)
-    return time.perf_counter() - t0, n, field[0]
+    for _ in range(steps):
+        cells += pop.count
+        _, field = O.oracle_cell_step(pop, lifecycle, contact, stepping, oxygen=rox)
+        sweeps.append(field[0])
+    return time.perf_counter() - t0, cells, sweeps
 
 
-def cpu_baseline(snap, w, geom, params, settings, lifecycle, contact, stepping) -> dict:
-    """The oracle (C restatement of the reference) on the host: one step from the snapshot the GPU's
-    first timed step started at, on every host core the process may use, and on one thread."""
+def cpu_baseline(snap, w, geom, params, settings, lifecycle, contact, stepping, gpu_same: dict) -> dict:
+    """The oracle (C restatement of the reference) on the host: the first CPU_STEPS timed steps from
+    the snapshot they started at, on every host core the process may use, and the first two on one
+    thread; beside it the GPU's time for the same steps (gpu_same)."""
     cores = host_threads()
-    dt_all, n, sweeps = oracle_step_seconds(snap, w, geom, params, settings, lifecycle, contact, stepping, cores)
-    dt_one, _, _ = oracle_step_seconds(snap, w, geom, params, settings, lifecycle, contact, stepping, 1)
-    return {"value": round(n / dt_all, 1), "unit": "cell-steps/s", "cores": cores, "kind": "port",
-            "sample": f"1 step of {w.name} ({n} cells, {sweeps} SOR sweeps) from the state the GPU's first timed "
-                      f"step started at; oracle C restatement, -O2, OpenMP over {cores} host threads: {dt_all:.2f} s",
+    dt_all, cells, sweeps = oracle_steps_seconds(snap, w, geom, params, settings, lifecycle, contact, stepping,
+                                                 cores, CPU_STEPS)
+    dt_one, cells_one, sweeps_one = oracle_steps_seconds(snap, w, geom, params, settings, lifecycle, contact,
+                                                         stepping, 1, 2)
+    return {"value": round(cells / dt_all, 1), "unit": "cell-steps/s", "cores": cores, "kind": "port",
+            "sample": f"the first {CPU_STEPS} timed steps of {w.name} ({cells} cell-steps, {sum(sweeps)} SOR sweeps) "
+                      f"from the state they started at; oracle C restatement, -O2, OpenMP over {cores} host "
+                      f"threads: {dt_all:.2f} s. The GPU ran the same steps (same sweeps) in "
+                      f"{gpu_same['seconds']:.4f} s",
             "sweeps": sweeps, "seconds": round(dt_all, 3),
-            "single_thread": {"value": round(n / dt_one, 1), "cores": 1, "seconds": round(dt_one, 3)}}
+            "gpu_same_steps": gpu_same, "same_sweeps": sweeps == gpu_same["sweeps"],
+            "single_thread": {"value": round(cells_one / dt_one, 1), "cores": 1, "seconds": round(dt_one, 3),
+                              "steps": 2, "sweeps": sweeps_one}}
+
+
+def field_accuracy(field32, density32, params, geom) -> dict:
+    """The final headline field against float64: a float64 solve of the same discrete problem (the
+    last step's density, the same boundary) started from the GPU's field and run to a residual of
+    1e-13; also the float64 residual of the GPU's field."""
+    import numpy as np
+
+    from oracle import oracle as O
+
+    a = O.SolveArgs(dx=geom.dx, diffusion=params.diffusion_coefficient, uptake_max=params.uptake_max,
+                    michaelis_k=params.michaelis_k, boundary_value=params.boundary_value)
+    f64 = field32.astype(np.float64)
+    d64 = density32.astype(np.float64)
+    res32 = O.residual(field32.astype(np.float64), d64, a)
+    sweeps, res, conv = O.solve(f64, d64, a, O.optimal_omega(geom.shape), 20000, 1e-13, 10)
+    err = np.abs(field32.astype(np.float64) - f64)
+    return {"fp64_sweeps": sweeps, "fp64_residual": res, "fp64_converged": conv,
+            "max_abs_err_over_boundary": float(err.max() / params.boundary_value),
+            "rel_l2_err": float(np.linalg.norm(err) / np.linalg.norm(f64)),
+            "fp64_residual_of_fp32_field": res32}
+
+
+def adjoint_accuracy(gradient: dict) -> dict:
+    """The 129^3 adjoint workload's GPU gradient against the float64 adjoint (every rate, D, O_b) and
+    central differences of float64 solves on 64 rates (oracle.adjoint_check)."""
+    import numpy as np
+
+    from oracle import oracle as O
+    from warpbiocell_amd import workloads as W
+    from warpbiocell_amd.fields.scalar_field import GridGeometry
+
+    w, rates, targets = W.adjoint_100k_129()
+    geom = GridGeometry.centered_cube(w.box_um, w.dx_um)
+    O.set_threads(host_threads())
+    r = O.adjoint_check(w.positions, rates, targets, W.D_UM2_PER_H, w.boundary_mmHg, W.MICHAELIS_K, geom.origin,
+                        geom.dx, geom.shape, gradient["rates"], gradient["D"], gradient["O_b"], fd_cells=64)
+    per_rate = np.asarray(r["gradient_vs_fd_rel_err"])
+    r["gradient_vs_fd_rel_err_median"] = float(np.median(per_rate))
+    r["gradient_vs_fd_rel_err_max"] = float(per_rate.max())
+    r["fd_vs_fp64_rel_err_max"] = float(np.max(r["fd_vs_fp64_rel_err"]))
+    r["fd_vs_fp64_rel_err"] = [float(f"{e:.3e}") for e in r["fd_vs_fp64_rel_err"]]
+    r["gradient_vs_fd_rel_err"] = [float(f"{e:.3e}") for e in per_rate]
+    return r
 
 
 def main() -> int:
@@ -371,6 +467,7 @@ def main() -> int:
         log(f"  sor shape forced: {args.sor_shape}")
     torch.cuda.synchronize()
     snap = None if args.no_cpu_baseline else snapshot(pop, ox)
+    start_copy = None if args.no_cpu_baseline else device_copy(pop, ox)
 
     reports, cell_steps = [], 0
     quiesce_gc()
@@ -385,10 +482,12 @@ def main() -> int:
     log(f"  timed: {elapsed:.3f} s, sweeps/step mean {sum(sweeps) / len(sweeps):.1f} "
         f"(min {min(sweeps)}, max {max(sweeps)}), unconverged steps {sum(not r.field_converged for r in reports)}")
 
+    final_field = ox.numpy() if snap is not None else None
+    final_density = ox.density.cpu().numpy() if snap is not None else None
     roof = None if args.no_roofline else sor_roofline(ox, geom, device, w.name)
     if roof:
         log(f"  sor sweep ({roof['shape']}): {roof['sweep_us']} us, {roof['achieved']} GB/s algorithmic")
-    adj = None if args.no_adjoint else adjoint_measurement(device)
+    adj, gradient = adjoint_measurement(device) if not args.no_adjoint else (None, None)
     if adj:
         log(f"  adjoint: {adj}")
     conf = None if args.no_confinement else confinement_measurement(device)
@@ -397,11 +496,18 @@ def main() -> int:
     bat = None if args.no_batched else batched_measurement(device, args.batched_steps)
     if bat:
         log(f"  batched: {bat}")
-    cpu = None
+    cpu, accuracy = None, None
     if snap is not None:
-        cpu = cpu_baseline(snap, w, geom, params, settings, lifecycle, contact, stepping)
-        log(f"  cpu baseline: {cpu['value']} cell-steps/s ({cpu['seconds']} s, {cpu['sweeps']} sweeps)")
-
+        same = gpu_same_steps(start_copy, pop, ox, grid, lifecycle, contact, stepping, CPU_STEPS)
+        cpu = cpu_baseline(snap, w, geom, params, settings, lifecycle, contact, stepping, same)
+        log(f"  cpu baseline: {cpu['value']} cell-steps/s over {CPU_STEPS} steps ({cpu['seconds']} s, "
+            f"sweeps {sum(cpu['sweeps'])}); GPU same steps {same['value']} cell-steps/s ({same['seconds']} s)")
+        accuracy = {"field": field_accuracy(final_field, final_density, params, geom)}
+        log(f"  accuracy, headline field vs float64: {accuracy['field']}")
+        if gradient is not None:
+            accuracy["adjoint"] = adjoint_accuracy(gradient)
+            log("  accuracy, adjoint gradient: " + str({k: v for k, v in accuracy["adjoint"].items()
+                                                       if not isinstance(v, list)}))
     result = {
         "metric": METRIC,
         "value": round(cell_steps / elapsed, 1),
@@ -427,6 +533,7 @@ def main() -> int:
         "batched": bat,
         "confinement": conf,
         "cpu_baseline": cpu,
+        "accuracy": accuracy,
     }
     print(json.dumps(result), flush=True)
     return 0

@@ -10,6 +10,7 @@ reference's step (simulation/simulator.py:58-81) on numpy arrays.
 from __future__ import annotations
 
 import ctypes
+import math
 import subprocess
 from ctypes import POINTER, c_float, c_int32, c_int64, c_uint32, c_void_p
 from dataclasses import dataclass, field
@@ -532,8 +533,41 @@ def forward_f64(positions, rates, diffusion, boundary_value, michaelis_k, origin
     return field, Q, S, report
 
 
-def adjoint_f64(positions, grad_S, field, Q, diffusion, michaelis_k, origin, dx, shape, sweeps=4000, omega=1.8):
-    """(dL/dq_c, dL/dD, dL/dO_b) at the float64 solution ``field`` for dL/dS = grad_S."""
+def optimal_omega(shape) -> float:
+    """2 / (1 + sin(pi / (n - 1))) for the longest axis: the SOR relaxation of the Dirichlet
+    Laplacian with the smallest spectral radius. Any omega in (0, 2) has the same fixed point, so the
+    float64 reference solves use this one to converge in fewer sweeps."""
+    return 2.0 / (1.0 + math.sin(math.pi / (max(shape) - 1)))
+
+
+def fd_rate_gradients(positions, rates, targets, diffusion, boundary_value, michaelis_k, origin, dx, shape, cells,
+                      base_field, step=1e-3, sweeps=200, omega=None) -> np.ndarray:
+    """Central differences of L = sum_c (S_c - target_c)^2 with respect to the rates of ``cells``,
+    step ``step`` relative: each perturbed float64 solve starts from ``base_field`` (the converged
+    float64 solution) and runs the same number of sweeps, so what remains of the start's residual
+    is the same in the + and - solves and cancels in their difference (to first order). The loss
+    difference is summed term by term, sum (S+ - S-)(S+ + S- - 2 t): at 100k cells a full L is
+    ~4e7 while one rate moves it by ~1e-7, below the rounding of L itself."""
+    q0 = np.asarray(rates, dtype=np.float64)
+    t = np.asarray(targets, dtype=np.float64)
+    w = optimal_omega(shape) if omega is None else omega
+    out = np.zeros(len(cells))
+    for n, c in enumerate(cells):
+        eps = step * q0[c]
+        S = []
+        for sign in (1.0, -1.0):
+            q = q0.copy()
+            q[c] += sign * eps
+            S.append(forward_f64(positions, q, diffusion, boundary_value, michaelis_k, origin, dx, shape, sweeps=sweeps,
+                                 tolerance=0.0, omega=w, start=base_field)[2])
+        out[n] = float(np.sum((S[0] - S[1]) * (S[0] + S[1] - 2.0 * t))) / (2.0 * eps)
+    return out
+
+
+def adjoint_f64(positions, grad_S, field, Q, diffusion, michaelis_k, origin, dx, shape, sweeps=4000, omega=None):
+    """(dL/dq_c, dL/dD, dL/dO_b) at the float64 solution ``field`` for dL/dS = grad_S; ``sweeps``
+    red-black SOR sweeps on the adjoint system at ``omega`` (default optimal_omega)."""
+    omega = optimal_omega(shape) if omega is None else omega
     sc, st, sw = _sample_weights(positions, origin, dx, shape)
     g = np.zeros(int(np.prod(shape)))
     np.add.at(g, st, sw * np.asarray(grad_S, dtype=np.float64)[sc])
@@ -560,6 +594,51 @@ def adjoint_f64(positions, grad_S, field, Q, diffusion, michaelis_k, origin, dx,
     nd[:, :, -2] += 1
     grad_Ob = float(g[face].sum() + np.sum(mu[inner] * nd[inner]))
     return grad_q, grad_D, grad_Ob, mu
+
+
+def adjoint_check(positions, rates, targets, diffusion, boundary_value, michaelis_k, origin, dx, shape, grad_q,
+                  grad_D, grad_Ob, fd_cells=64, fd_sweeps=150, seed=0) -> dict:
+    """Checks of a gradient of L = sum_c (S_c - t_c)^2 (per-cell rates, D, O_b) computed elsewhere:
+
+    - against the float64 adjoint at the converged float64 solution: the whole rate vector
+      (relative 2-norm error, largest error over the largest gradient), each rate whose gradient is
+      at least 1e-3 of the largest (relative errors: median, 99th percentile, maximum), D and O_b;
+    - against central differences (step 1e-3 relative, fd_rate_gradients) on ``fd_cells`` rates
+      drawn at random among those checked above: each parameter's error relative to its own float64
+      gradient, for the float64 adjoint (which the differences pin) and for the gradient under test.
+      Cells in a depleted core (oxygen near 0) have gradients down to ~1e-7 of the largest; a
+      relative error of those says nothing, so they are counted but not drawn.
+    """
+    pos = np.asarray(positions, dtype=np.float32)
+    q = np.asarray(rates, dtype=np.float64)
+    t = np.asarray(targets, dtype=np.float64)
+    w = optimal_omega(shape)
+    field, Q, S, rep = forward_f64(pos, q, diffusion, boundary_value, michaelis_k, origin, dx, shape,
+                                   tolerance=1e-13, omega=w)
+    sweeps = max(1500, 12 * max(shape))  # 129^3 at the optimal omega: converged to ~1e-15 by 2,000
+    gq64, gD64, gOb64, _ = adjoint_f64(pos, 2.0 * (S - t), field, Q, diffusion, michaelis_k, origin, dx, shape,
+                                       sweeps=sweeps, omega=w)
+    g = np.asarray(grad_q, dtype=np.float64)
+    scale = float(np.abs(gq64).max())
+    big = np.abs(gq64) >= 1e-3 * scale
+    rel = np.abs(g[big] - gq64[big]) / np.abs(gq64[big])
+    cells = np.random.default_rng(seed).choice(np.nonzero(big)[0], fd_cells, replace=False)
+    fd = fd_rate_gradients(pos, q, t, diffusion, boundary_value, michaelis_k, origin, dx, shape, cells, field,
+                           sweeps=fd_sweeps, omega=w)
+    return {
+        "fp64_forward_sweeps": int(rep[0]),
+        "rates_rel_l2": float(np.linalg.norm(g - gq64) / np.linalg.norm(gq64)),
+        "rates_max_err_over_max_grad": float(np.abs(g - gq64).max() / scale),
+        "rates_rel_err_median": float(np.median(rel)), "rates_rel_err_p99": float(np.percentile(rel, 99)),
+        "rates_rel_err_max": float(rel.max()), "rates_checked": int(big.sum()),
+        "rates_below_1e-3_of_max": int((~big).sum()),
+        "D_rel_err": float(abs(grad_D - gD64) / abs(gD64)), "O_b_rel_err": float(abs(grad_Ob - gOb64) / abs(gOb64)),
+        "fd_cells": cells.tolist(),
+        "fd_vs_fp64_rel_err": (np.abs(fd - gq64[cells]) / np.abs(gq64[cells])).tolist(),
+        "gradient_vs_fd_rel_err": (np.abs(g[cells] - fd) / np.abs(fd)).tolist(),
+        "fd_max_err_over_max_grad": float(np.abs(fd - gq64[cells]).max() / scale),
+        "gradient_vs_fd_max_err_over_max_grad": float(np.abs(g[cells] - fd).max() / scale),
+    }
 
 
 # ---- population metrics (metrics/population.py) ---------------------------------------------------

@@ -673,8 +673,11 @@ void orc_adjoint_sor_f64(int32_t sweeps, double omega, double dx2_over_D, double
                          int32_t ny, int32_t nz)
 {
     const int64_t plane = (int64_t)ny * nz;
+    /* A colour pass reads only the other colour, so its nodes update in any order: the loop over
+     * planes runs on all threads with the same result as one thread. */
     for (int32_t s = 0; s < sweeps; ++s)
         for (int32_t color = 0; color < 2; ++color)
+#pragma omp parallel for schedule(static)
             for (int32_t i = 1; i < nx - 1; ++i)
                 for (int32_t j = 1; j < ny - 1; ++j)
                     for (int32_t k = 1; k < nz - 1; ++k) {

@@ -135,3 +135,33 @@ def test_full_size_gradients_against_central_differences(device):
           f"{t_bwd * 1e3:.1f} ms ({prob.last_backward[0]} sweeps), ratio {t_bwd / t_fwd:.2f}; FD relative errors "
           + ", ".join(f"{k} {e:.2e}" for k, e in errs.items()))
     assert all(e < 2e-2 for e in errs.values()), errs
+
+
+def test_full_size_gradients_against_the_fp64_adjoint_and_central_differences(oracle, device):
+    """The 100k-cell, 129^3 problem: the GPU gradient (all rates, D, O_b) against the float64
+    adjoint at the float64 solution, and on 64 random rates against central differences of float64
+    solves (step 1e-3 relative), which also pin the float64 adjoint at this size."""
+    w, rates, targets = W.adjoint_100k_129()
+    geom = GridGeometry.centered_cube(w.box_um, w.dx_um)
+    D, Ob, K = W.D_UM2_PER_H, w.boundary_mmHg, W.MICHAELIS_K
+    prob = OxygenAdjointProblem(geom, torch.from_numpy(w.positions), K, W.UPTAKE_MMHG_UM3_PER_H,
+                                SolverSettings(omega=w.omega, tolerance=w.tolerance, max_sweeps=w.max_sweeps,
+                                               check_every=w.check_every), device=device)
+    S = prob.forward(torch.from_numpy(rates).to(device), D, Ob)
+    gq, gD, gOb = prob.backward(2.0 * (S - torch.from_numpy(targets).to(device)))
+    oracle.set_threads(oracle.get_threads())
+    r = oracle.adjoint_check(w.positions, rates, targets, D, Ob, K, geom.origin, geom.dx, geom.shape,
+                             gq.cpu().numpy(), gD, gOb, fd_cells=64)
+    fd_fp64 = np.asarray(r["fd_vs_fp64_rel_err"])
+    gpu_fd = np.asarray(r["gradient_vs_fd_rel_err"])
+    print("129^3 / 100k gradient check:", {k: (f"{v:.3e}" if isinstance(v, float) else v) for k, v in r.items()
+                                            if not isinstance(v, list)},
+          f"FD vs FP64 adjoint per rate: max {fd_fp64.max():.2e}; GPU vs FD per rate: median "
+          f"{np.median(gpu_fd):.2e}, max {gpu_fd.max():.2e}")
+    # Measured on MI355X: FD vs FP64 adjoint 6.4e-5 at most; GPU vs FP64 adjoint 7.4e-5 (2-norm),
+    # 3.1e-4 of the largest gradient at most, D 7.3e-5, O_b 9.4e-5; GPU vs FD median 9.7e-5.
+    assert fd_fp64.max() < 1e-3  # the differences pin the float64 adjoint rate by rate
+    assert r["rates_rel_l2"] < 1e-3 and r["rates_max_err_over_max_grad"] < 2e-3
+    assert r["rates_rel_err_p99"] < 1e-2
+    assert r["D_rel_err"] < 1e-3 and r["O_b_rel_err"] < 1e-3
+    assert np.median(gpu_fd) < 1e-3 and gpu_fd.max() < 1e-2

@@ -43,3 +43,30 @@ def test_adjoint_gradients_match_central_differences(oracle):
         eps = 1e-3 * (D if name == "D" else Ob)
         fd = (fn(eps) - fn(-eps)) / (2 * eps)
         assert abs(fd - grad) <= 2e-3 * abs(fd), (name, fd, grad)
+
+
+def test_warm_started_central_differences_match_the_adjoint(oracle):
+    """fd_rate_gradients (fixed-sweep solves from the converged field at the optimal omega, loss
+    difference summed term by term) against the FP64 adjoint and against central differences of
+    fully converged solves, on the small spheroid."""
+    pos, rates, targets, geom = _problem()
+    D, Ob, K = W.D_UM2_PER_H, 40.0, 1.0
+    args = dict(michaelis_k=K, origin=geom.origin, dx=geom.dx, shape=geom.shape)
+    field, Q, S, rep = oracle.forward_f64(pos, rates, D, Ob, tolerance=1e-13, omega=oracle.optimal_omega(geom.shape),
+                                          **args)
+    assert rep[2]
+    gq, _, _, _ = oracle.adjoint_f64(pos, 2.0 * (S - targets), field, Q, D, K, geom.origin, geom.dx, geom.shape)
+    cells = np.random.default_rng(2).choice(len(rates), 16, replace=False)
+    fd = oracle.fd_rate_gradients(pos, rates, targets, D, Ob, K, geom.origin, geom.dx, geom.shape, cells, field,
+                                  sweeps=300)
+    scale = np.abs(gq).max()
+    assert np.abs(fd - gq[cells]).max() <= 1e-6 * scale
+    c = cells[0]
+    eps = 1e-3 * rates[c]
+    ls = []
+    for sign in (1, -1):
+        q = rates.copy()
+        q[c] += sign * eps
+        ls.append(float(np.sum((oracle.forward_f64(pos, q, D, Ob, tolerance=1e-13, start=field, **args)[2] - targets)
+                               ** 2)))
+    assert abs((ls[0] - ls[1]) / (2 * eps) - fd[0]) <= 1e-4 * scale
