@@ -60,8 +60,9 @@ COUNTER_TABLE = ROOT / "profiles" / "sor_counters_spheroid_100k_161.md"
 
 
 def measured_traffic(shape: str, workload: str):
-    """HBM bytes per launch of tile shape `shape` from the committed FETCH_SIZE / WRITE_SIZE table
-    (scaled by the copy calibration), or None when the table does not cover this workload/shape."""
+    """HBM bytes of one red+black sweep of tile shape `shape` (both launches of a two-pass shape)
+    from the committed FETCH_SIZE / WRITE_SIZE table (scaled by the copy calibration), or None when
+    the table does not cover this workload/shape."""
     if workload != "spheroid_100k_161" or not COUNTER_TABLE.exists():
         return None
     for line in COUNTER_TABLE.read_text().splitlines():
@@ -112,13 +113,18 @@ def sor_roofline(ox, geom, device, workload: str, launches: int = 200) -> dict:
     achieved = bytes_per_sweep / per_sweep_s / 1e9
     family = {0: "tile", 1: "march", 2: "pipe", 3: "copy", 4: "packed", 5: "quad"}[kind.value]
     shape = shape_name(kind.value, tj.value, ti.value)
+    launches = 2 if family in ("packed", "quad") else 1
+    sweep_traffic = measured_traffic(shape, workload)
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
-            "frac": round(achieved / HBM_PEAK_GBPS, 4), "traffic": measured_traffic(shape, workload),
+            "frac": round(achieved / HBM_PEAK_GBPS, 4),
+            "traffic": None if sweep_traffic is None else sweep_traffic // launches,
             "kernel": KERNEL_OF_KIND[family].format(tj=tj.value, ti=ti.value), "shape": shape,
-            "launches_per_sweep": 2 if family in ("packed", "quad") else 1,
+            "launches_per_sweep": launches,
             "sweep_us": round(per_sweep_s * 1e6, 3),
-            "launch_us": round(per_sweep_s * 1e6 / (2 if family in ("packed", "quad") else 1), 3),
+            "launch_us": round(per_sweep_s * 1e6 / launches, 3),
+            "algorithmic_bytes_per_launch": bytes_per_sweep // launches,
             "algorithmic_bytes_per_sweep": bytes_per_sweep,
+            "traffic_per_sweep": sweep_traffic,
             "achieved_at_12B": round(12 * geom.node_count / per_sweep_s / 1e9, 1),
             "traffic_source": str(COUNTER_TABLE.relative_to(ROOT))}
 
@@ -427,6 +433,8 @@ def main() -> int:
     ap.add_argument("--sor-shape", default=None,
                     help="run the SOR sweeps with this shape (autotune.config_shapes() name) instead of the "
                          "autotuner's choice; the autotuner still times every shape")
+    ap.add_argument("--batched-only", action="store_true",
+                    help="only run the batched-sweep leg and print its result as one JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
                     help="only run a plain 256 MiB device copy 5 times (counter calibration)")
     args = ap.parse_args()
@@ -449,6 +457,9 @@ def main() -> int:
             dst.copy_(src)
         torch.cuda.synchronize()
         log("copy calibration: 5 x 256 MiB device-to-device copies")
+        return 0
+    if args.batched_only:
+        print(json.dumps({"batched": batched_measurement(device, args.batched_steps)}), flush=True)
         return 0
     w, geom, params, settings, lifecycle, contact, stepping, pop, ox, grid = setup(args.workload, device)
     log(f"bench: {w.name}: {pop.count} cells, grid {geom.shape}, warmup {args.warmup}, steps {args.steps}")

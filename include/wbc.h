@@ -96,8 +96,21 @@ int wbc_deposit_trilinear_i64(const float* position, const int32_t* cell_state, 
 int wbc_deposit_trilinear_i64_ordered(const float* position, const int32_t* cell_state, const int32_t* order,
                                       int64_t n, float ox, float oy, float oz, float inv_dx, int32_t nx, int32_t ny,
                                       int32_t nz, int64_t* accumulator, wbc_stream_t stream);
+/* The deposit step of CellDensities in one call: zero the accumulator's layers named in the
+ * device int32 *layers (a bit per state; bits only ever get added), then deposit as
+ * wbc_deposit_trilinear_i64 (in `order` when not NULL, as wbc_deposit_trilinear_i64_ordered) and
+ * add the states deposited to *layers. The layers outside *layers are then zero without having
+ * been written: the accumulator must be zero when *layers is first set to 0, and nothing else
+ * may write it while *layers tracks it. No reference counterpart (the reference zeroes and
+ * converts all four layers); the accumulator equals the reference's to the bit. */
+int wbc_deposit_accumulate_i64(const float* position, const int32_t* cell_state, const int32_t* order, int64_t n,
+                               float ox, float oy, float oz, float inv_dx, int32_t nx, int32_t ny, int32_t nz,
+                               int64_t* accumulator, int32_t* layers, wbc_stream_t stream);
+/* `layers` (device int32, may be NULL = all four) names the layers that may be nonzero; the
+ * others are taken as zero without being read. */
 int wbc_density_finalize(const int64_t* accumulator, int64_t nodes, float inv_voxel_volume, float w0, float w1,
-                         float w2, float w3, float* combined, float* per_state, wbc_stream_t stream);
+                         float w2, float w3, float* combined, float* per_state, const int32_t* layers,
+                         wbc_stream_t stream);
 /* Replaces CellDensities.combine, fields/densities.py:45-50 (combine_densities). */
 int wbc_combine_densities(const float* per_state, int64_t nodes, float w0, float w1, float w2, float w3, float* out,
                           wbc_stream_t stream);
@@ -170,6 +183,13 @@ int wbc_steady_state_residual(float dx2_over_D, float uptake_max, float michaeli
  * a second field buffer in the workspace; the result always ends in `field`. The host reads
  * chunk c's residual while chunk c+1 runs; a device-side latch turns sweeps queued after
  * convergence into no-ops. Returns after the stream has produced the report. */
+/* How far the solve loop queues ahead of the check the host waits on: a whole chunk while the
+ * residual is far from the tolerance, `near_ahead` sweeps (>= 1) once the next check's predicted
+ * residual is below near_factor x tolerance; near_ahead >= check_every always queues whole
+ * chunks. Process-wide; defaults 3 and 2.0. Results do not depend on it, only how many no-op
+ * launches follow convergence and how much queued work covers the host's reaction. No reference
+ * counterpart (the reference loop synchronises at every check). */
+int wbc_solve_set_lookahead(int32_t near_ahead, float near_factor);
 size_t wbc_solve_workspace_bytes(int32_t max_sweeps, int32_t check_every, int32_t nx, int32_t ny, int32_t nz);
 int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolerance, int32_t check_every, float dx2_over_D,
                            float uptake_max, float michaelis_k, float reference, int32_t bc_x, int32_t bc_y,

@@ -41,6 +41,35 @@ def test_deposit_accumulator_is_bit_exact_and_order_independent(oracle, device):
     np.testing.assert_array_equal(dens.density.cpu().numpy(), oracle.fixed_point_to_density(expected, geom.dx))
 
 
+def test_deposit_layer_tracking_follows_states_that_come_and_go(oracle, device):
+    """The deposit zeroes and reads only the accumulator layers some deposit wrote (a bit per
+    state on the device). Cells move between states across deposits: first all in one state, then
+    spread over three, then all in the fourth; every accumulator, per-state density and combined
+    density equals the oracle's, so a layer the cells left is zeroed and a layer never written
+    stays zero. Empty and zero-cell deposits keep the layers' bits."""
+    pos, _, pop = _cluster_with_states(3000, 31, device, outside=11)
+    geom = GridGeometry.centered_cube(600.0, 20.0)
+    dens = CellDensities(geom, device=device)
+    rng = np.random.default_rng(5)
+    weights = (1.0, 0.25, 0.5, 2.0)
+    for step, states in enumerate((np.zeros(3000, np.int32), rng.choice([0, 1, 2], 3000).astype(np.int32),
+                                   np.full(3000, 3, np.int32), np.full(3000, 1, np.int32))):
+        pop.cell_state[:3000].copy_(torch.from_numpy(states))
+        expected = oracle.deposit_accumulator(pos, states, geom.origin, geom.dx, geom.shape)
+        combined = torch.empty(geom.shape, dtype=torch.float32, device=device)
+        dens.deposit_combined(pop, weights, combined)
+        np.testing.assert_array_equal(dens.accumulator.cpu().numpy(), expected, err_msg=f"deposit {step}")
+        per_state = oracle.fixed_point_to_density(expected, geom.dx)
+        np.testing.assert_array_equal(combined.cpu().numpy(), oracle.combine(weights, per_state),
+                                      err_msg=f"combined {step}")
+        np.testing.assert_array_equal(dens.density.cpu().numpy(), per_state, err_msg=f"density {step}")
+    assert int(dens._layers.item()) == 0b1111
+    pop.count = 0
+    dens.deposit(pop)
+    assert not dens.accumulator.any()
+    assert int(dens._layers.item()) == 0b1111
+
+
 def test_deposit_in_any_cell_order_gives_the_same_accumulator(oracle, device):
     """wbc_deposit_trilinear_i64_ordered visits cells in a given permutation of the live slots:
     reversed, random and the hash grid's bucket order (recorded by a contact relaxation) all give

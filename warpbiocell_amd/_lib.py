@@ -34,7 +34,8 @@ SIGNATURES: dict[str, tuple] = {
     "wbc_fill_zero_i64": (I32, [P, I64, P]),
     "wbc_deposit_trilinear_i64": (I32, [P, P, I64, F32, F32, F32, F32, I32, I32, I32, P, P]),
     "wbc_deposit_trilinear_i64_ordered": (I32, [P, P, P, I64, F32, F32, F32, F32, I32, I32, I32, P, P]),
-    "wbc_density_finalize": (I32, [P, I64, F32, F32, F32, F32, F32, P, P, P]),
+    "wbc_deposit_accumulate_i64": (I32, [P, P, P, I64, F32, F32, F32, F32, I32, I32, I32, P, P, P]),
+    "wbc_density_finalize": (I32, [P, I64, F32, F32, F32, F32, F32, P, P, P, P]),
     "wbc_combine_densities": (I32, [P, I64, F32, F32, F32, F32, P, P]),
     "wbc_sample_trilinear": (I32, [P, I32, I32, I32, P, I64, F32, F32, F32, F32, P, P]),
     "wbc_sor_sweep": (I32, [I32, F32, F32, F32, F32, I32, I32, I32, P, P, P, I32, P, F32, P, I32, I32, I32, P]),
@@ -44,6 +45,7 @@ SIGNATURES: dict[str, tuple] = {
     "wbc_sor_fused_config_count": (I32, []),
     "wbc_sor_fused_config_shape": (I32, [I32, P, P, P]),
     "wbc_sor_fused_set_config": (I32, [I32]),
+    "wbc_solve_set_lookahead": (I32, [I32, F32]),
     "wbc_sor_fused_get_config": (I32, []),
     "wbc_sor_sweep_fused_cfg": (I32, [I32, F32, F32, F32, F32, P, P, P, I32, P, F32, P, P, I32, I32, I32, P]),
     "wbc_sor_sweeps_workspace_bytes": (c_size_t, [I32, I32, I32]),

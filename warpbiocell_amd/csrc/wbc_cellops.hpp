@@ -40,14 +40,16 @@ __device__ __forceinline__ void deposit_cell(const float* __restrict__ position,
 
 // fixed_point_to_density + combine_densities for node t of a grid of `nodes` nodes
 // (kernels/field_kernels.py:119-142).
+// Layers whose bit is clear in `layers` are zero and not read (their density is the +0 a zero
+// converts to).
 __device__ __forceinline__ float finalize_node(const long long* __restrict__ acc, int64_t nodes, int64_t t,
                                                float inv_vol, float w0, float w1, float w2, float w3,
-                                               float* __restrict__ per_state)
+                                               float* __restrict__ per_state, int32_t layers = 0xF)
 {
-    float d0 = (float)acc[t] / kFixedOne * inv_vol;
-    float d1 = (float)acc[nodes + t] / kFixedOne * inv_vol;
-    float d2 = (float)acc[2 * nodes + t] / kFixedOne * inv_vol;
-    float d3 = (float)acc[3 * nodes + t] / kFixedOne * inv_vol;
+    float d0 = (layers & 1) ? (float)acc[t] / kFixedOne * inv_vol : 0.0f;
+    float d1 = (layers & 2) ? (float)acc[nodes + t] / kFixedOne * inv_vol : 0.0f;
+    float d2 = (layers & 4) ? (float)acc[2 * nodes + t] / kFixedOne * inv_vol : 0.0f;
+    float d3 = (layers & 8) ? (float)acc[3 * nodes + t] / kFixedOne * inv_vol : 0.0f;
     if (per_state) {
         per_state[t] = d0;
         per_state[nodes + t] = d1;

@@ -18,17 +18,32 @@ namespace wbc {
 // Thread t deposits cell order[t] (cell t without an order). A spatially coherent order, such as
 // the hash grid's bucket order, puts a wave's atomics on nearby nodes: 100k cells on a 161^3 grid,
 // 53.1 us in slot order, 29.0-30.0 us in bucket or node order.
+//
+// `layers` (optional) is the bit set of the accumulator's state layers that may hold nonzero
+// values: each wave ORs in the states of its cells (an atomic only when it adds a bit), so that
+// zeroing and finalizing can skip the layers no cell has reached; a population in one state
+// touches a quarter of the accumulator.
 __global__ void __launch_bounds__(256) deposit_trilinear_kernel(const float* __restrict__ position,
                                                                  const int32_t* __restrict__ cell_state,
                                                                  const int32_t* __restrict__ order, int64_t n,
                                                                  float ox, float oy, float oz, float inv_dx,
                                                                  int32_t nx, int32_t ny, int32_t nz,
-                                                                 unsigned long long* __restrict__ acc)
+                                                                 unsigned long long* __restrict__ acc,
+                                                                 int32_t* __restrict__ layers)
 {
     const int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int64_t c = order ? (int64_t)order[t] : t;
-    deposit_cell(position, c, cell_state[c], ox, oy, oz, inv_dx, nx, ny, nz, acc);
+    const int32_t s = cell_state[c];
+    if (layers) {
+        int32_t m = 0;
+#pragma unroll
+        for (int l = 0; l < 4; ++l)
+            if (__ballot(s == l)) m |= 1 << l;
+        const int leader = __builtin_ctzll(__ballot(1));
+        if ((int)__lane_id() == leader && (m & ~layers[0]) != 0) atomicOr(layers, m);
+    }
+    deposit_cell(position, c, s, ox, oy, oz, inv_dx, nx, ny, nz, acc);
 }
 
 __global__ void __launch_bounds__(256) zero_u64_kernel(unsigned long long* __restrict__ p, int64_t count)
@@ -38,15 +53,33 @@ __global__ void __launch_bounds__(256) zero_u64_kernel(unsigned long long* __res
     for (; t < count; t += stride) p[t] = 0ull;
 }
 
+// Zero the layers of the (4, nodes) accumulator whose bit is set in *layers (every layer without
+// a set); the others are zero already.
+__global__ void __launch_bounds__(256) zero_layers_kernel(unsigned long long* __restrict__ acc, int64_t nodes,
+                                                          const int32_t* __restrict__ layers)
+{
+    const int32_t m = layers ? layers[0] : 0xF;
+    const int64_t stride = (int64_t)gridDim.x * blockDim.x;
+    for (int l = 0; l < 4; ++l) {
+        if (!(m & (1 << l))) continue;
+        unsigned long long* a = acc + (int64_t)l * nodes;
+        for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nodes; t += stride) a[t] = 0ull;
+    }
+}
+
 // density_s = float(acc_s) / 2^24 * inv_voxel_volume; combined = w0 d0 + w1 d1 + w2 d2 + w3 d3.
+// A layer outside *layers is zero, so it is not read: its density is +0, the value its zeros
+// convert to, and the sums are the same to the bit.
 __global__ void __launch_bounds__(256) density_finalize_kernel(const long long* __restrict__ acc, int64_t nodes,
                                                                 float inv_vol, float w0, float w1, float w2,
                                                                 float w3, float* __restrict__ combined,
-                                                                float* __restrict__ per_state)
+                                                                float* __restrict__ per_state,
+                                                                const int32_t* __restrict__ layers)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
     if (t >= nodes) return;
-    const float v = finalize_node(acc, nodes, t, inv_vol, w0, w1, w2, w3, per_state);
+    const int32_t m = layers ? layers[0] : 0xF;
+    const float v = finalize_node(acc, nodes, t, inv_vol, w0, w1, w2, w3, per_state, m);
     if (combined) combined[t] = v;
 }
 
@@ -175,7 +208,28 @@ WBC_API int wbc_deposit_trilinear_i64(const float* position, const int32_t* cell
     if (!position || !cell_state || !accumulator) return (int)hipErrorInvalidValue;
     deposit_trilinear_kernel<<<grid_for(n, 256), 256, 0, stream>>>(position, cell_state, nullptr, n, ox, oy, oz,
                                                                     inv_dx, nx, ny, nz,
-                                                                    (unsigned long long*)accumulator);
+                                                                    (unsigned long long*)accumulator, nullptr);
+    WBC_CHECK_LAUNCH();
+    return 0;
+}
+
+// Zero the accumulator's layers in *layers, then deposit (in `order` when given) and add the
+// states deposited to *layers.
+WBC_API int wbc_deposit_accumulate_i64(const float* position, const int32_t* cell_state, const int32_t* order,
+                                       int64_t n, float ox, float oy, float oz, float inv_dx, int32_t nx, int32_t ny,
+                                       int32_t nz, int64_t* accumulator, int32_t* layers, hipStream_t stream)
+{
+    if (n < 0 || n > INT32_MAX || !grid_ok(nx, ny, nz)) return (int)hipErrorInvalidValue;
+    if (!accumulator || !layers || (n > 0 && (!position || !cell_state))) return (int)hipErrorInvalidValue;
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    int64_t blocks = grid_for(nodes, 256);
+    if (blocks > 8192) blocks = 8192;
+    zero_layers_kernel<<<(unsigned)blocks, 256, 0, stream>>>((unsigned long long*)accumulator, nodes, layers);
+    WBC_CHECK_LAUNCH();
+    if (n == 0) return 0;
+    deposit_trilinear_kernel<<<grid_for(n, 256), 256, 0, stream>>>(position, cell_state, order, n, ox, oy, oz, inv_dx,
+                                                                    nx, ny, nz, (unsigned long long*)accumulator,
+                                                                    layers);
     WBC_CHECK_LAUNCH();
     return 0;
 }
@@ -191,20 +245,21 @@ WBC_API int wbc_deposit_trilinear_i64_ordered(const float* position, const int32
     if (n == 0) return 0;
     if (!position || !cell_state || !order || !accumulator) return (int)hipErrorInvalidValue;
     deposit_trilinear_kernel<<<grid_for(n, 256), 256, 0, stream>>>(position, cell_state, order, n, ox, oy, oz, inv_dx,
-                                                                    nx, ny, nz, (unsigned long long*)accumulator);
+                                                                    nx, ny, nz, (unsigned long long*)accumulator,
+                                                                    nullptr);
     WBC_CHECK_LAUNCH();
     return 0;
 }
 
 WBC_API int wbc_density_finalize(const int64_t* accumulator, int64_t nodes, float inv_voxel_volume, float w0,
                                  float w1, float w2, float w3, float* combined, float* per_state,
-                                 hipStream_t stream)
+                                 const int32_t* layers, hipStream_t stream)
 {
     if (nodes <= 0 || nodes >= ((int64_t)1 << 31)) return (int)hipErrorInvalidValue;
     if (!accumulator || (!combined && !per_state)) return (int)hipErrorInvalidValue;
     density_finalize_kernel<<<grid_for(nodes, 256), 256, 0, stream>>>((const long long*)accumulator, nodes,
                                                                        inv_voxel_volume, w0, w1, w2, w3, combined,
-                                                                       per_state);
+                                                                       per_state, layers);
     WBC_CHECK_LAUNCH();
     return 0;
 }
@@ -287,17 +342,19 @@ WBC_API int wbc_steady_state_residual(float dx2_over_D, float uptake_max, float 
 
 // ----------------------------------------------------------------------------------------------
 // The steady-state solve loop (fields/diffusion.py:79-123) with the host checks pipelined:
-// chunk c+1 is queued before the host reads chunk c's residual, and a device-side latch makes
-// the queued sweeps no-ops once a chunk converged, so results equal the synchronous loop's.
+// sweeps of the next chunk are queued before the host reads chunk c's residual, and a device-side
+// latch makes the queued sweeps no-ops once a chunk converged, so results equal the synchronous
+// loop's.
 // ----------------------------------------------------------------------------------------------
 namespace {
 constexpr int kEventPool = 8;
-// Chunks queued beyond the one the host is waiting on (must stay below kEventPool). One keeps the
-// GPU busy while the host checks (a chunk runs for ~150 us, the check takes tens of us) and leaves
-// one chunk of no-op launches after convergence. Headline step (100k cells, 161^3): 0 -> 2.352 ms,
-// 1 -> 2.283 ms, 2 -> 2.330 ms.
-constexpr int kLag = 1;
-static_assert(kLag >= 1, "the colour-packed loop records chunk c's check while queueing chunk c + 1");
+// Sweeps queued beyond the chunk the host waits on: a whole chunk far from the tolerance, and
+// g_near_ahead once the next check's predicted residual (this one times the last ratio, at most 1)
+// is below g_near_factor x tolerance. Process-wide settings (wbc_solve_set_lookahead);
+// g_near_ahead >= 1 because the colour-packed loop closes chunk c's check with chunk c + 1's first
+// pass.
+int32_t g_near_ahead = 3;
+float g_near_factor = 2.0f;
 struct EventPool {
     int device = -1;
     hipEvent_t ev[kEventPool];
@@ -390,6 +447,14 @@ WBC_API size_t wbc_solve_workspace_bytes(int32_t max_sweeps, int32_t check_every
     return solve_layout(chunks, p).total;
 }
 
+WBC_API int wbc_solve_set_lookahead(int32_t near_ahead, float near_factor)
+{
+    if (near_ahead < 1 || !(near_factor >= 0.0f)) return (int)hipErrorInvalidValue;
+    g_near_ahead = near_ahead;
+    g_near_factor = near_factor;
+    return 0;
+}
+
 WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolerance, int32_t check_every,
                                    float dx2_over_D, float uptake_max, float michaelis_k, float reference,
                                    int32_t bc_x, int32_t bc_y, int32_t bc_z, const int32_t* fixed,
@@ -444,7 +509,8 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
     auto chunk_len = [&](int32_t c) {
         return max_sweeps - c * check_every < check_every ? max_sweeps - c * check_every : check_every;
     };
-    int32_t queued = 0, recorded = 0, checked = 0, sweeps = 0, cur = 0;  // buf[cur]: the field after the queued sweeps
+    // buf[cur]: the field after the queued sweeps; qs: sweeps queued so far.
+    int32_t qs = 0, recorded = 0, checked = 0, sweeps = 0, cur = 0;
     // Chunk c's check: reduce its partials into slots[c] (and the mapped host copy), then its event.
     auto queue_check = [&](int32_t c) -> hipError_t {
         WBC_TRY_ERR(launch_residual_finish(partial, partials, slots, slots_mapped, c, tolerance, state, stream));
@@ -459,38 +525,49 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
                                          partial + pass_blocks, stream));
         return queue_check(q - 1);
     };
-    float residual = __builtin_huge_valf();
+    // Sweep qs, with the check of its chunk when it ends one (colour-packed: the black terms now,
+    // the red terms and the check with the next chunk's first red pass).
+    auto queue_sweep = [&]() -> hipError_t {
+        const int32_t c = qs / check_every, s = qs - c * check_every;
+        const bool last = s == chunk_len(c) - 1;
+        if (packed) {
+            float* r = red[c & 1];
+            if (s == 0 && c > 0)
+                WBC_TRY_ERR(red_check(c));
+            else
+                WBC_TRY_ERR(launch_packed_colour(p, 0, packed_rows, 0, r, r, pf.f[1], pf.d[0], state, nullptr,
+                                                 stream));
+            WBC_TRY_ERR(launch_packed_colour(p, 1, packed_rows, last ? 2 : 0, pf.f[1], pf.f[1], r, pf.d[1], state,
+                                             last ? partial : nullptr, stream));
+        } else if (fused) {
+            WBC_TRY_ERR(launch_sor_fused(p, buf[cur], buf[cur ^ 1], state, stream));
+            cur ^= 1;
+        } else {
+            WBC_TRY_ERR(launch_sor_color(p, 0, field, state, stream));
+            WBC_TRY_ERR(launch_sor_color(p, 1, field, state, stream));
+        }
+        if (last && !packed) {
+            WBC_TRY_ERR(launch_residual_partial(p, buf[cur], partial, state, stream));
+            WBC_TRY_ERR(queue_check(c));
+        }
+        ++qs;
+        if (packed && qs == max_sweeps) WBC_TRY_ERR(red_check(chunks));
+        return hipSuccess;
+    };
+    float residual = __builtin_huge_valf(), previous = __builtin_huge_valf();
     bool converged = false;
     int32_t result_buf = 0;
+    // While the host waits on chunk c's check, chunk c+1 and `ahead` sweeps of chunk c+2 are queued.
+    // A whole chunk ahead keeps the GPU busy through the host's wake-up and launches; near the
+    // tolerance, where the residual of these solves hovers within float32 noise of it and any check
+    // may converge, near_ahead sweeps are enough for that and bound the no-op launches the latch
+    // leaves after convergence. Which sweeps run, and so the results, do not depend on it.
+    const int32_t near_ahead = g_near_ahead;
+    const float near_factor = g_near_factor;
+    int32_t ahead = check_every;
     while (true) {
-        while (queued < chunks && queued - checked <= kLag) {
-            const int32_t len = chunk_len(queued);
-            for (int32_t s = 0; s < len; ++s) {
-                if (packed) {
-                    float* r = red[queued & 1];
-                    if (s == 0 && queued > 0)
-                        WBC_TRY(red_check(queued));
-                    else
-                        WBC_TRY(launch_packed_colour(p, 0, packed_rows, 0, r, r, pf.f[1], pf.d[0], state, nullptr,
-                                                     stream));
-                    const bool last = s == len - 1;
-                    WBC_TRY(launch_packed_colour(p, 1, packed_rows, last ? 2 : 0, pf.f[1], pf.f[1], r, pf.d[1], state,
-                                                 last ? partial : nullptr, stream));
-                } else if (fused) {
-                    WBC_TRY(launch_sor_fused(p, buf[cur], buf[cur ^ 1], state, stream));
-                    cur ^= 1;
-                } else {
-                    WBC_TRY(launch_sor_color(p, 0, field, state, stream));
-                    WBC_TRY(launch_sor_color(p, 1, field, state, stream));
-                }
-            }
-            if (!packed) {
-                WBC_TRY(launch_residual_partial(p, buf[cur], partial, state, stream));
-                WBC_TRY(queue_check(queued));
-            }
-            ++queued;
-            if (packed && queued == chunks) WBC_TRY(red_check(chunks));
-        }
+        const int64_t target = (int64_t)(checked + 1) * check_every + ahead;
+        while (qs < max_sweeps && qs < target) WBC_TRY(queue_sweep());
         if (checked == recorded) break;
         WBC_TRY(hipEventSynchronize(ev[checked % kEventPool]));
         residual = ((volatile float*)slots_host)[checked];
@@ -501,6 +578,9 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
             converged = true;
             break;
         }
+        const bool near = checked >= 2 && residual * fminf(residual / previous, 1.0f) < near_factor * tolerance;
+        ahead = near && near_ahead < check_every ? near_ahead : check_every;
+        previous = residual;
     }
     // The field after the last executed chunk is in buf[sweeps % 2] (fused path) or in the packed
     // arrays, red[(checked - 1) & 1] for the red nodes; chunks queued after convergence did nothing.

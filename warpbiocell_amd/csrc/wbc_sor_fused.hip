@@ -155,21 +155,35 @@ __global__ void __launch_bounds__(256) residual_partial_kernel(SorParams p, cons
     }
 }
 
-// Reduce the partials into residual[chunk] and latch convergence (one block).
-__global__ void __launch_bounds__(256) residual_finish_kernel(const float* __restrict__ partial, int count,
-                                                              float* __restrict__ residual,
-                                                              float* __restrict__ host_out, int32_t chunk,
-                                                              float tolerance, int32_t* __restrict__ state)
+// Reduce the partials into residual[chunk] and latch convergence (one block of kFinishThreads).
+// The check sits on the solve's critical path between two passes, so each thread keeps four
+// independent loads in flight (a few thousand partials take one round of loads) instead of one
+// dependent load per iteration. fmaxf drops NaNs in any order, so the result does not depend on
+// the order of the reduction.
+constexpr int kFinishThreads = 1024;
+__global__ void __launch_bounds__(kFinishThreads) residual_finish_kernel(const float* __restrict__ partial, int count,
+                                                                         float* __restrict__ residual,
+                                                                         float* __restrict__ host_out, int32_t chunk,
+                                                                         float tolerance, int32_t* __restrict__ state)
 {
-    __shared__ float wave_max_lds[4];
-    if (state[0] != 0) return;
-    float m = 0.0f;
-    for (int b = threadIdx.x; b < count; b += 256) m = fmaxf(m, partial[b]);
-    m = wave_max(m);
+    constexpr int kWaves = kFinishThreads / kWave;
+    __shared__ float wave_max_lds[kWaves];
+    float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
+    int b = threadIdx.x;
+    for (; b + 3 * kFinishThreads < count; b += 4 * kFinishThreads) {
+        m0 = fmaxf(m0, partial[b]);
+        m1 = fmaxf(m1, partial[b + kFinishThreads]);
+        m2 = fmaxf(m2, partial[b + 2 * kFinishThreads]);
+        m3 = fmaxf(m3, partial[b + 3 * kFinishThreads]);
+    }
+    for (; b < count; b += kFinishThreads) m0 = fmaxf(m0, partial[b]);
+    if (state[0] != 0) return;  // a check after convergence leaves the latched results alone
+    float m = wave_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)));
     if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;
     __syncthreads();
     if (threadIdx.x == 0) {
-        m = fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
+        m = wave_max_lds[0];
+        for (int w = 1; w < kWaves; ++w) m = fmaxf(m, wave_max_lds[w]);
         residual[chunk] = m;
         if (host_out) host_out[chunk] = m;
         if (m < tolerance) {
@@ -324,7 +338,7 @@ hipError_t launch_residual_atomic(const SorParams& p, const float* field, float*
 hipError_t launch_residual_finish(const float* partial, int count, float* residual, float* host_out, int32_t chunk,
                                   float tol, int32_t* state, hipStream_t s)
 {
-    residual_finish_kernel<<<1, 256, 0, s>>>(partial, count, residual, host_out, chunk, tol, state);
+    residual_finish_kernel<<<1, kFinishThreads, 0, s>>>(partial, count, residual, host_out, chunk, tol, state);
     return hipGetLastError();
 }
 

@@ -4,7 +4,9 @@ The deposit accumulates int64 fixed-point trilinear weights (2^24 per cell), so 
 does not depend on the order in which the atomics land. ``density`` (per state,
 ``(4, nx, ny, nz)`` float32) is produced on demand from the accumulator: a species that owns
 its densities asks for its weighted sum directly (``deposit_combined``) and never writes the
-four per-state layers unless something reads them.
+four per-state layers unless something reads them. A bit set on the device records the layers
+(states) deposits have written; the others stay zero and are neither zeroed nor read, so a
+population in one state touches a quarter of the accumulator.
 """
 
 from __future__ import annotations
@@ -25,6 +27,9 @@ class CellDensities:
         self.device = _lib.require_gpu(device)
         shape = (NUM_STATES, *geometry.shape)
         self._accumulator = torch.zeros(shape, dtype=torch.int64, device=self.device)
+        # Bit set of the accumulator layers (states) a deposit has written; the others stay zero
+        # and are neither zeroed nor read.
+        self._layers = torch.zeros(1, dtype=torch.int32, device=self.device)
         self._density = torch.zeros(shape, dtype=torch.float32, device=self.device)
         self._density_stale = False
 
@@ -42,17 +47,11 @@ class CellDensities:
         stream = _lib.stream_of(self.device)
         nx, ny, nz = self.geometry.shape
         origin, inv_dx, _ = self._scalars()
-        _lib.call("wbc_fill_zero_i64", self._accumulator.data_ptr(), self._accumulator.numel(), stream)
-        if population.count > 0:
-            order = population.deposit_order()
-            if order is not None:  # a permutation of the live slots: the same accumulator, faster
-                _lib.call("wbc_deposit_trilinear_i64_ordered", population.position.data_ptr(),
-                          population.cell_state.data_ptr(), order.data_ptr(), population.count, origin[0], origin[1],
-                          origin[2], inv_dx, nx, ny, nz, self._accumulator.data_ptr(), stream)
-            else:
-                _lib.call("wbc_deposit_trilinear_i64", population.position.data_ptr(),
-                          population.cell_state.data_ptr(), population.count, origin[0], origin[1], origin[2], inv_dx,
-                          nx, ny, nz, self._accumulator.data_ptr(), stream)
+        # A permutation of the live slots (deposit_order) gives the same accumulator, faster.
+        order = population.deposit_order() if population.count > 0 else None
+        _lib.call("wbc_deposit_accumulate_i64", population.position.data_ptr(), population.cell_state.data_ptr(),
+                  _lib.ptr(order), population.count, origin[0], origin[1], origin[2], inv_dx, nx, ny, nz,
+                  self._accumulator.data_ptr(), self._layers.data_ptr(), stream)
 
     def deposit(self, population) -> None:
         """Zero the accumulator, deposit every active cell in its state's layer, convert."""
@@ -71,7 +70,8 @@ class CellDensities:
             raise ValueError(f"need {NUM_STATES} weights, got {len(w)}")
         _, _, inv_vol = self._scalars()
         _lib.call("wbc_density_finalize", self._accumulator.data_ptr(), self.geometry.node_count, inv_vol, *w,
-                  _lib.ptr(out), self._density.data_ptr() if per_state else None, _lib.stream_of(self.device))
+                  _lib.ptr(out), self._density.data_ptr() if per_state else None, self._layers.data_ptr(),
+                  _lib.stream_of(self.device))
         self._density_stale = not per_state
 
     @property
@@ -88,7 +88,7 @@ class CellDensities:
         if self._density_stale:
             _, _, inv_vol = self._scalars()
             _lib.call("wbc_density_finalize", self._accumulator.data_ptr(), self.geometry.node_count, inv_vol, *w,
-                      out.data_ptr(), None, _lib.stream_of(self.device))
+                      out.data_ptr(), None, self._layers.data_ptr(), _lib.stream_of(self.device))
             return
         _lib.call("wbc_combine_densities", self._density.data_ptr(), self.geometry.node_count, *w, out.data_ptr(),
                   _lib.stream_of(self.device))
