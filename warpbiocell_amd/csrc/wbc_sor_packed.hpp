@@ -168,9 +168,12 @@ __device__ __forceinline__ void packed_lane(int tile, int blocks_per_plane, int&
 // nodes (the row passes above issue 6 RJ + 2 four-byte loads for RJ nodes). The k-neighbours of
 // the four nodes are the other colour's row j at kk0 - 1 + par .. kk0 + 4 + par: the middle four are
 // the lane's own load, the outer two (kk0 - 1 and kk0 + 4) the neighbouring lanes' (a wave shift
-// by one lane); lanes 0 and 63 load theirs. Where the neighbouring lane sits on another row (q = 0
-// or the last q), the value it supplies is one no interior node uses: node kk0 = 0 of parity 0 is
-// k = 0 and node kk0 + 3 at the row end is k >= nz - 1, both on the Dirichlet faces.
+// by one lane); lanes 0 and 63 load theirs. A row has floor(nz / 2) / 4 groups, rounded up: the
+// packed indices that hold an interior node of either parity (k <= nz - 2), so a 161^3 pass has
+// 1976 workgroups, under the 2048 the chip holds at once, where whole padded rows would take 2074.
+// The row's last group loads its right value (kk0 + 4, the face node of an odd nz, inside the
+// row's pitch); the left value lane q = 0 gets from the previous row is one no interior node uses
+// (node kk0 = 0 of parity 0 is k = 0, on the Dirichlet face).
 //
 // Addresses are 32-bit byte offsets from the arrays' bases (the host requires the packed array
 // below 4 GiB).
@@ -243,10 +246,15 @@ __device__ __forceinline__ float quad_update(const SorParams& p, const QuadGrid&
     const wbc_f4 im = at(ob, w - g.plane_bytes), ip = at(ob, w + g.plane_bytes);
     const wbc_f4 o = at(own, w), d = at(dens, w);
     const int lane = (int)__lane_id();
-    float edge = 0.0f;
-    if (lane == 0 || lane == kWave - 1) edge = *(const float*)(ob + w + (lane == 0 ? -4 : 16));
+    // Lane 0 has no left neighbour in the wave; lane 63 has no right one, and the row's last
+    // group has no lane to its right in the row: they load the value.
+    const bool load_right = lane == kWave - 1 || q == g.row.d - 1;
+    float edge_left = 0.0f, edge_right = 0.0f;
+    if (lane == 0) edge_left = *(const float*)(ob + w - 4);
+    if (load_right) edge_right = *(const float*)(ob + w + 16);
     if (stop) return -1.0f;
-    const float ext[6] = {lane_from_left(jc.w, edge), jc.x, jc.y, jc.z, jc.w, lane_from_right(jc.x, edge)};
+    const float from_right = lane_from_right(jc.x, edge_right);
+    const float ext[6] = {lane_from_left(jc.w, edge_left), jc.x, jc.y, jc.z, jc.w, load_right ? edge_right : from_right};
     const int k0 = 2 * kk0 + par;
     wbc_f4 val;
     float m = 0.0f;
@@ -293,7 +301,7 @@ inline int64_t packed_lanes_per_plane(const SorParams& p, int rj)
 inline bool quad_grid_of(const SorParams& p, QuadGrid& g)
 {
     const int64_t pitch = packed_pitch(p.nz, kQuadRows);
-    const int64_t groups = pitch / 4, lanes = (int64_t)(p.ny - 2) * groups, total = lanes * (p.nx - 2);
+    const int64_t groups = (p.nz / 2 + 3) / 4, lanes = (int64_t)(p.ny - 2) * groups, total = lanes * (p.nx - 2);
     if (packed_floats(p) * 4 >= ((int64_t)1 << 32) || total >= INT32_MAX - 256) return false;
     g.plane = fast_div_of((uint32_t)lanes);
     g.row = fast_div_of((uint32_t)groups);
