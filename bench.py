@@ -72,9 +72,9 @@ def measured_traffic(shape: str, workload: str):
     return None
 
 
-def sor_roofline(ox, geom, device, workload: str, launches: int = 200) -> dict:
+def sor_roofline(ox, geom, device, workload: str, timed_sweeps: int = 1000) -> dict:
     """Average duration of one red+black SOR sweep with the selected shape, measured with HIP
-    events on its stream over `launches` sweeps in one multi-sweep call.
+    events on its stream over `timed_sweeps` sweeps in one multi-sweep call.
 
     Algorithmic bytes of one red+black sweep: 16 B per node (two colour passes each reading the
     field, one field write, one density read). The fused kernels read the field once, so their own
@@ -105,10 +105,10 @@ def sor_roofline(ox, geom, device, workload: str, launches: int = 200) -> dict:
     sweeps(4)
     start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
     start.record(stream)
-    sweeps(launches)
+    sweeps(timed_sweeps)
     end.record(stream)
     end.synchronize()
-    per_sweep_s = start.elapsed_time(end) / 1e3 / launches
+    per_sweep_s = start.elapsed_time(end) / 1e3 / timed_sweeps
     bytes_per_sweep = 16 * geom.node_count
     achieved = bytes_per_sweep / per_sweep_s / 1e9
     family = {0: "tile", 1: "march", 2: "pipe", 3: "copy", 4: "packed", 5: "quad"}[kind.value]
@@ -120,7 +120,7 @@ def sor_roofline(ox, geom, device, workload: str, launches: int = 200) -> dict:
             "traffic": None if sweep_traffic is None else sweep_traffic // launches,
             "kernel": KERNEL_OF_KIND[family].format(tj=tj.value, ti=ti.value), "shape": shape,
             "launches_per_sweep": launches,
-            "sweep_us": round(per_sweep_s * 1e6, 3),
+            "timed_sweeps": timed_sweeps, "sweep_us": round(per_sweep_s * 1e6, 3),
             "launch_us": round(per_sweep_s * 1e6 / launches, 3),
             "algorithmic_bytes_per_launch": bytes_per_sweep // launches,
             "algorithmic_bytes_per_sweep": bytes_per_sweep,
@@ -231,10 +231,8 @@ def batched_measurement(device, steps: int) -> dict:
     ws = W.sweep_15()
     serial = [scenario.build(w, device) for w in ws]
     batched = [scenario.build(w, device) for w in ws]
-    configs = []
-    for sc in serial:  # tune every distinct grid before the timed region (cached per shape)
-        configs.append(tune_sor(sc.oxygen.field, sc.oxygen.density, sc.oxygen.species.kinetics,
-                                sc.settings.omega)["choice"])
+    tuned = [tune_sor(sc.oxygen.field, sc.oxygen.density, sc.oxygen.species.kinetics, sc.settings.omega)
+             for sc in serial]  # every distinct grid before the timed region (cached per shape)
     tuner = BatchedSweep([sc.replica() for sc in batched])
     tuner.step(1, use_graph=False)
     tune = tune_batch(tuner)
@@ -249,12 +247,18 @@ def batched_measurement(device, steps: int) -> dict:
 
     quiesce_gc()
     t0 = time.perf_counter()
-    serial_sweeps = 0
-    for sc, cfg in zip(serial, configs):
-        _lib.call("wbc_sor_fused_set_config", cfg)
+    serial_sweeps, per_replica = 0, []
+    for sc, tune_r in zip(serial, tuned):
+        _lib.call("wbc_sor_fused_set_config", tune_r["choice"])
+        r0, sw = time.perf_counter(), 0
         for _ in range(steps):
-            serial_sweeps += cell_step(sc.population, sc.grid, sc.lifecycle, sc.contact, sc.stepping,
-                                       oxygen=sc.oxygen).field_sweeps
+            sw += cell_step(sc.population, sc.grid, sc.lifecycle, sc.contact, sc.stepping,
+                            oxygen=sc.oxygen).field_sweeps
+        torch.cuda.synchronize()
+        per_replica.append({"grid": sc.geometry.shape[0], "cells": sc.workload.n, "sweeps": sw,
+                            "sor_shape": tune_r["shape"],
+                            "serial_s": round(time.perf_counter() - r0, 4)})
+        serial_sweeps += sw
     torch.cuda.synchronize()
     t1 = time.perf_counter()
     batch.step(steps, use_graph=True)
@@ -271,13 +275,19 @@ def batched_measurement(device, steps: int) -> dict:
                                        batch.field_view(i).cpu().numpy().view(np.uint32))
     cells = sum(w.n for w in ws)
     batch.close()
+    # The colour passes' floor: 16 B per node and sweep (two passes each reading both colours'
+    # field values once, one write, the density) at the HBM peak, over every replica's sweeps.
+    node_sweeps = sum(r["sweeps"] * r["grid"] ** 3 for r in per_replica)
+    floor_s = 16 * node_sweeps / (HBM_PEAK_GBPS * 1e9)
     return {"workload": "sweep_15", "replicas": len(ws), "cells": cells,
             "nodes": sum(sc.geometry.node_count for sc in batched), "steps": steps,
             "serial_s": round(t1 - t0, 4), "batched_s": round(t2 - t1, 4),
             "speedup": round((t1 - t0) / (t2 - t1), 3), "bit_exact": bool(same),
             "sweeps": batched_sweeps, "serial_cell_steps_per_s": round(cells * steps / (t1 - t0), 1),
             "batched_cell_steps_per_s": round(cells * steps / (t2 - t1), 1),
-            "schedule": tune["schedule"], "schedule_ms_per_solve": tune["ms_per_solve"]}
+            "schedule": tune["schedule"], "schedule_ms_per_solve": tune["ms_per_solve"],
+            "node_sweeps": node_sweeps, "sweep_floor_s": round(floor_s, 4),
+            "speedup_bound_at_floor": round((t1 - t0) / floor_s, 2), "per_replica": per_replica}
 
 
 def snapshot(pop, ox):

@@ -409,9 +409,10 @@ int wbc_batch_create(const wbc_replica* replicas, int32_t count, int32_t hash_di
 /* Schedule of the solve, `tile` in [0, wbc_batch_schedule_count()). Persistent entries first: tile
  * shape 60 x TJ x TI nodes (wbc_batch_schedule_shape; the second half of those runs the same shapes
  * with each workgroup copying its next tile while computing the current one) with workgroups per
- * CU (0: as many as fit). Then the wide entries, reported with TJ = rows per thread, TI = 0 and
- * one occupancy choice (blocks_per_cu ignored); setting one fails when a replica's grid is too
- * large for the packed passes. Resets the graphs. */
+ * CU (0: as many as fit). Then the wide entries, reported with TJ = rows per thread (1: the quad
+ * passes, four packed indices of a row per lane), TI = 0 and one occupancy choice (blocks_per_cu
+ * ignored); setting one fails when a replica's grid is too large for the packed passes. Resets
+ * the graphs. */
 int wbc_batch_schedule_count(void);
 int wbc_batch_schedule_shape(int32_t tile, int32_t* tj, int32_t* ti, int32_t* max_blocks_per_cu);
 int wbc_batch_set_schedule(wbc_batch_t batch, int32_t tile, int32_t blocks_per_cu);

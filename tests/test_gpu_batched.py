@@ -112,7 +112,7 @@ def use_schedule(batch, name):
     batch.set_schedule(tile, bpc)
 
 
-@pytest.mark.parametrize("schedule", ["default", "wide/2", "wide/4"])
+@pytest.mark.parametrize("schedule", ["default", "wide/2", "wide/4", "wide/quad"])
 def test_batched_steps_are_bit_exact_per_replica(device, schedule):
     from warpbiocell_amd.simulation.batched import BatchedSweep
 
@@ -146,7 +146,7 @@ def test_batched_steps_are_bit_exact_per_replica(device, schedule):
     batch.close()
 
 
-@pytest.mark.parametrize("schedule", ["default", "wide/2"])
+@pytest.mark.parametrize("schedule", ["default", "wide/2", "wide/quad"])
 def test_graph_replays_match_eager_steps(device, schedule):
     from warpbiocell_amd.simulation.batched import BatchedSweep
 

@@ -96,8 +96,9 @@ def tune_sor(field, density: torch.Tensor, kinetics, omega: float, sweeps: int =
 def batch_schedules() -> list[tuple[int, int, str]]:
     """(schedule, workgroups per CU, name) of every schedule of the batched solve: the persistent
     launch with each tile shape at 1, 2, ... up to the occupancy limit workgroups per CU
-    ('copy' / 'pipe' 60 x TJ x TI @ n/CU), and the wide launches ('wide/RJ': one launch per colour
-    pass over every replica, RJ rows per thread)."""
+    ('copy' / 'pipe' 60 x TJ x TI @ n/CU), and the wide launches (one launch per colour pass over
+    every replica: 'wide/RJ' with RJ rows per thread, 'wide/quad' with four packed indices per
+    lane)."""
     count = _lib.query("wbc_batch_schedule_count")
     shapes = []
     for tile in range(count):
@@ -108,7 +109,7 @@ def batch_schedules() -> list[tuple[int, int, str]]:
     out = []
     for tile, tj, ti, most in shapes:
         if ti == 0:
-            out.append((tile, 0, f"wide/{tj}"))
+            out.append((tile, 0, "wide/quad" if tj == 1 else f"wide/{tj}"))
             continue
         kind = "pipe" if tile >= persistent // 2 else "copy"
         for bpc in range(1, most + 1):

@@ -695,14 +695,16 @@ __global__ void __launch_bounds__(256) batch_contact_kernel(const ReplicaDev* __
 // ---- wide solve ------------------------------------------------------------------------------
 struct WideRep {
     SorParams p;          // the replica's solve (density unused: the packed arrays below)
-    PackedField pf;       // the replica's colour-packed field and density
-    int32_t bpp, lanes;   // colour pass: workgroups and lanes per plane at the schedule's rows per thread
-    int32_t rbpp, rlanes; // residual: the same at kResidualRows rows per thread
+    PackedField pf;       // the replica's colour-packed field and density (pitch of the schedule)
+    int32_t bpp, lanes;   // row passes: workgroups and lanes per plane at the schedule's rows per thread
+    int32_t rbpp, rlanes; // row passes' residual: the same at kResidualRows rows per thread
+    QuadGrid qg;          // quad passes: lane geometry (schedule wide/quad)
+    int32_t pass_blocks;  // workgroups of one colour pass of the replica under the schedule
+    int32_t res_blocks;   // workgroups of its residual (both colours) under the schedule
     uint32_t magic;       // nz2_magic_of(p)
     int32_t q_finite;
     int32_t max_sweeps, check_every;
     float tolerance;
-    int32_t pad;
     int64_t node_offset;  // first node of the replica in the concatenated unpacked arrays
 };
 
@@ -753,6 +755,57 @@ __global__ void __launch_bounds__(256) wide_pass_kernel(const WideRep* __restric
         packed_update<RJ, 1>(w.p, own, nullptr, other, dens, i, (uint32_t)u, w.magic, w.q_finite);
     else
         packed_update<RJ, 0>(w.p, own, nullptr, other, dens, i, (uint32_t)u, w.magic, w.q_finite);
+}
+
+// One quad colour pass (wbc_sor_packed.hpp) over the replicas in g: workgroup t of replica r
+// covers its lanes (t - off) * 256 .. ; a replica's workgroups are whole, so a wave never mixes
+// replicas and the quad pass's lane shifts stay within one grid. Lanes past the replica's last
+// duplicate it and store nothing; a finished replica's done flag stops its lanes.
+template <int C>
+__global__ void __launch_bounds__(256) wide_quad_kernel(const WideRep* __restrict__ wr, const WideGrid g,
+                                                         const int32_t* __restrict__ done)
+{
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = range_of(g.off, g.n, t);
+    const int r = g.rep[k];
+    const WideRep& w = wr[r];
+    uint32_t lane = (uint32_t)(t - g.off[k]) * 256u + threadIdx.x;
+    const bool in_range = lane < (uint32_t)w.qg.total;
+    if (!in_range) lane = (uint32_t)w.qg.total - 1;
+    quad_update<C, 0>(w.p, w.qg, lane, in_range, w.pf.f[C], nullptr, w.pf.f[C ^ 1], w.pf.d[C], done + r,
+                      w.q_finite);
+}
+
+// The residual of the replicas in g that are active and check after sweep q, on quad lanes: the
+// replica's first pass_blocks workgroups take colour 0, the next colour 1, each computing the
+// residual terms of the values it reads (the quad pass with kRes 1, storing nothing).
+__global__ void __launch_bounds__(256) wide_quad_residual_kernel(const WideRep* __restrict__ wr, const WideGrid g,
+                                                                  const int32_t* __restrict__ rblk_off,
+                                                                  const int32_t* __restrict__ done, int32_t q,
+                                                                  float* __restrict__ partial)
+{
+    __shared__ float wave_max_lds[4];
+    const int t = xcd_tile(blockIdx.x, gridDim.x);
+    const int k = range_of(g.off, g.n, t);
+    const int r = g.rep[k];
+    const WideRep& w = wr[r];
+    if (done[r] || !wide_is_check(w, q)) return;
+    const int local = t - g.off[k];
+    const int colour = local >= w.pass_blocks ? 1 : 0;
+    uint32_t lane = (uint32_t)(local - colour * w.pass_blocks) * 256u + threadIdx.x;
+    const bool in_range = lane < (uint32_t)w.qg.total;
+    if (!in_range) lane = (uint32_t)w.qg.total - 1;
+    float m = colour ? quad_update<1, 1>(w.p, w.qg, lane, false, w.pf.f[1], nullptr, w.pf.f[0], w.pf.d[1], nullptr,
+                                        w.q_finite)
+                     : quad_update<0, 1>(w.p, w.qg, lane, false, w.pf.f[0], nullptr, w.pf.f[1], w.pf.d[0], nullptr,
+                                        w.q_finite);
+    if (!in_range) m = 0.0f;  // a duplicate's wave neighbours are not the last lane's
+    m = wave_max(m);
+    if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;
+    __syncthreads();
+    if (threadIdx.x == 0)
+        partial[rblk_off[r] + local] =
+            fmaxf(fmaxf(wave_max_lds[0], wave_max_lds[1]), fmaxf(wave_max_lds[2], wave_max_lds[3]));
 }
 
 // Residual partials (both colours) of the replicas in g that are active and check after sweep q:
@@ -858,7 +911,7 @@ __global__ void __launch_bounds__(256) wide_pack_kernel(const WideRep* __restric
         const int k = (int)(local % nz);
         const int64_t ij = local / nz;
         const int j = (int)(ij % ny), i = (int)(ij / ny);
-        const int64_t e = ij * packed_nz2(nz) + (k >> 1);
+        const int64_t e = ij * w.pf.pitch + (k >> 1);
         const int colour = (i + j + k) & 1;
         if (kUnpack) {
             field[t] = w.pf.f[colour][e];
@@ -875,8 +928,9 @@ struct BatchTile {
 };
 constexpr BatchTile kBatchTiles[] = {{4, 4}, {4, 8}, {8, 4}, {8, 8}};
 constexpr int kBatchTileCount = sizeof(kBatchTiles) / sizeof(kBatchTiles[0]);
-// Wide schedules (index 2 kBatchTileCount + w): rows per thread of the colour-pass launches.
-constexpr int kWideRows[] = {2, 4};
+// Wide schedules (index 2 kBatchTileCount + w): rows per thread of the colour-pass launches, or
+// kQuadRows for the quad passes.
+constexpr int kWideRows[] = {2, 4, kQuadRows};
 constexpr int kWideCount = sizeof(kWideRows) / sizeof(kWideRows[0]);
 constexpr int kScheduleCount = 2 * kBatchTileCount + kWideCount;
 
@@ -917,9 +971,14 @@ struct BatchLayout {
     bool wide_ok;  // every replica's grid fits the colour-packed passes
 };
 
-// Residual workgroups of one replica in the wide solve (both colours).
-static int64_t wide_res_blocks_of(const SorParams& p)
+// Residual workgroups of one replica in the wide solve (both colours) under the row passes
+// (rows 2 or 4) or the quad passes, or -1 when that indexing does not cover the grid.
+static int64_t wide_res_blocks_of(const SorParams& p, int rows)
 {
+    if (rows == kQuadRows) {
+        QuadGrid g;
+        return quad_grid_of(p, g) ? 2 * (((int64_t)g.total + 255) / 256) : -1;
+    }
     const int64_t lanes = packed_lanes_per_plane(p, kResidualRows);
     return lanes < 0 ? -1 : 2 * ((lanes + 255) / 256) * (p.nx - 2);
 }
@@ -963,10 +1022,11 @@ static bool layout_for(const wbc_replica* r, int32_t R, int32_t dim, BatchLayout
     bool wide_ok = true;
     for (int i = 0; i < R; ++i) {
         const SorParams g = grid_params(r[i]);
-        const int64_t rb = wide_res_blocks_of(g);
-        if (rb < 0 || packed_lanes_per_plane(g, kWideRows[0]) < 0 || packed_floats(g) >= ((int64_t)1 << 31))
+        const int64_t rb = wide_res_blocks_of(g, kWideRows[0]), rq = wide_res_blocks_of(g, kQuadRows);
+        if (rb < 0 || rq < 0 || packed_lanes_per_plane(g, kWideRows[0]) < 0 ||
+            packed_floats(g) >= ((int64_t)1 << 31))
             wide_ok = false;
-        res_blocks += rb < 0 ? 0 : rb;
+        res_blocks += rb < 0 || rq < 0 ? 0 : std::max(rb, rq);
         packed_bytes += 4 * (int64_t)((packed_floats(g) * sizeof(float) + 255) / 256 * 256);
     }
     if (res_blocks >= ((int64_t)1 << 30)) wide_ok = false;
@@ -1070,14 +1130,25 @@ static hipError_t wide_configure(wbc_batch* b, int rows)
     int64_t acc = 0, racc = 0;
     for (int r = 0; r < R; ++r) {
         WideRep& w = b->wide[r];
-        const int64_t lanes = packed_lanes_per_plane(w.p, rows);
-        if (lanes < 0) return hipErrorInvalidValue;
-        w.lanes = (int32_t)lanes;
-        w.bpp = (int32_t)((lanes + 255) / 256);
+        if (rows == kQuadRows) {
+            if (!quad_grid_of(w.p, w.qg)) return hipErrorInvalidValue;
+            w.pass_blocks = (int32_t)(((int64_t)w.qg.total + 255) / 256);
+        } else {
+            const int64_t lanes = packed_lanes_per_plane(w.p, rows);
+            if (lanes < 0) return hipErrorInvalidValue;
+            w.lanes = (int32_t)lanes;
+            w.bpp = (int32_t)((lanes + 255) / 256);
+            w.pass_blocks = w.bpp * (w.p.nx - 2);
+        }
+        const int64_t rb = wide_res_blocks_of(w.p, rows);
+        if (rb < 0) return hipErrorInvalidValue;
+        w.res_blocks = (int32_t)rb;
+        w.pf.pitch = packed_pitch(w.p.nz, rows);  // the packed arrays are laid out anew by each solve
         rblk[r] = (int32_t)racc;
-        acc += (int64_t)w.bpp * (w.p.nx - 2);
-        racc += wide_res_blocks_of(w.p);
+        acc += w.pass_blocks;
+        racc += rb;
     }
+    if (racc > b->L.wide_res_blocks) return hipErrorInvalidValue;
     if (acc >= ((int64_t)1 << 31)) return hipErrorInvalidValue;
     rblk[R] = (int32_t)racc;
     WBC_TRY_ERR(hipMemcpy(b->ws + b->L.wrep, b->wide.data(), sizeof(WideRep) * R, hipMemcpyHostToDevice));
@@ -1165,7 +1236,7 @@ static hipError_t wide_solve(wbc_batch* b, hipStream_t s)
             if (residual && !(q <= w.max_sweeps && (q % w.check_every == 0 || q == w.max_sweeps))) continue;
             g.rep[g.n] = r;
             g.off[g.n] = acc;
-            acc += residual ? (int32_t)wide_res_blocks_of(w.p) : w.bpp * (w.p.nx - 2);
+            acc += residual ? w.res_blocks : w.pass_blocks;
             ++g.n;
         }
         g.off[g.n] = acc;
@@ -1179,7 +1250,10 @@ static hipError_t wide_solve(wbc_batch* b, hipStream_t s)
             const int32_t pass_blocks = grid_of(false, q, pass);
             for (; swept < q; ++swept) {
                 if (pass_blocks == 0) continue;
-                if (b->wide_rows == 2) {
+                if (b->wide_rows == kQuadRows) {
+                    wide_quad_kernel<0><<<pass_blocks, 256, 0, s>>>(wr, pass, done);
+                    wide_quad_kernel<1><<<pass_blocks, 256, 0, s>>>(wr, pass, done);
+                } else if (b->wide_rows == 2) {
                     wide_pass_kernel<0, 2><<<pass_blocks, 256, 0, s>>>(wr, pass, done);
                     wide_pass_kernel<1, 2><<<pass_blocks, 256, 0, s>>>(wr, pass, done);
                 } else {
@@ -1190,7 +1264,10 @@ static hipError_t wide_solve(wbc_batch* b, hipStream_t s)
             }
             const int32_t res_blocks = grid_of(true, q, res);
             if (res_blocks > 0) {
-                wide_residual_kernel<<<res_blocks, 256, 0, s>>>(wr, res, rblk_off, done, q, partial);
+                if (b->wide_rows == kQuadRows)
+                    wide_quad_residual_kernel<<<res_blocks, 256, 0, s>>>(wr, res, rblk_off, done, q, partial);
+                else
+                    wide_residual_kernel<<<res_blocks, 256, 0, s>>>(wr, res, rblk_off, done, q, partial);
                 WBC_TRY_ERR(hipGetLastError());
             }
             wide_finish_kernel<<<R, 256, 0, s>>>(wr, rblk_off, done, q, partial, st, totals,
@@ -1330,7 +1407,7 @@ static hipError_t wide_setup(wbc_batch* b, const wbc_replica* replicas, const st
         w.pf.f[1] = (float*)(b->ws + off + each);
         w.pf.d[0] = (float*)(b->ws + off + 2 * each);
         w.pf.d[1] = (float*)(b->ws + off + 3 * each);
-        w.pf.pitch = packed_pitch(w.p.nz, kWideRows[0]);  // the row passes' pitch, ceil(nz / 2)
+        w.pf.pitch = packed_pitch(w.p.nz, kWideRows[0]);  // set per schedule by wide_configure
         off += 4 * each;
         const int64_t rl = packed_lanes_per_plane(w.p, kResidualRows);
         w.rlanes = (int32_t)rl;
