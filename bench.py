@@ -44,11 +44,14 @@ def log(msg: str) -> None:
     print(msg, file=sys.stderr, flush=True)
 
 
-def setup(workload_name: str, device):
+def setup(workload_name: str, device, grid_nodes: int | None = None):
     from warpbiocell_amd import workloads as W
     from warpbiocell_amd.simulation import scenario
 
-    sc = scenario.build(getattr(W, workload_name)(), device)
+    w = getattr(W, workload_name)()
+    if grid_nodes is not None:  # the same cells and box on grid_nodes^3 nodes
+        w.grid_nodes = grid_nodes
+    sc = scenario.build(w, device)
     return (sc.workload, sc.geometry, sc.params, sc.settings, sc.lifecycle, sc.contact, sc.stepping,
             sc.population, sc.oxygen, sc.grid)
 
@@ -443,6 +446,9 @@ def main() -> int:
     ap.add_argument("--sor-shape", default=None,
                     help="run the SOR sweeps with this shape (autotune.config_shapes() name) instead of the "
                          "autotuner's choice; the autotuner still times every shape")
+    ap.add_argument("--sweep-grid", type=int, default=None,
+                    help="only time the SOR sweep (autotuned shape, 1000 sweeps) of the workload's cells on "
+                         "an N^3 grid over its box, after one warmup step, and print the roofline as one JSON line")
     ap.add_argument("--batched-only", action="store_true",
                     help="only run the batched-sweep leg and print its result as one JSON line")
     ap.add_argument("--copy-calibration", action="store_true",
@@ -463,13 +469,32 @@ def main() -> int:
     if args.copy_calibration:
         src = torch.ones(64 * 2 ** 20, dtype=torch.float32, device=device)
         dst = torch.empty_like(src)
+        start, end = torch.cuda.Event(enable_timing=True), torch.cuda.Event(enable_timing=True)
+        start.record()
         for _ in range(5):
             dst.copy_(src)
+        end.record()
         torch.cuda.synchronize()
+        moved = 5 * 2 * src.numel() * 4  # each copy reads and writes 256 MiB, past the 256 MB Infinity Cache
         log("copy calibration: 5 x 256 MiB device-to-device copies")
+        print(json.dumps({"copy_bytes_moved": moved, "seconds": start.elapsed_time(end) / 1e3,
+                          "GB_per_s": round(moved / (start.elapsed_time(end) / 1e3) / 1e9, 1)}), flush=True)
         return 0
     if args.batched_only:
         print(json.dumps({"batched": batched_measurement(device, args.batched_steps)}), flush=True)
+        return 0
+    if args.sweep_grid:
+        from warpbiocell_amd.autotune import tune_sor
+
+        w, geom, params, settings, lifecycle, contact, stepping, pop, ox, grid = setup(args.workload, device,
+                                                                                        args.sweep_grid)
+        cell_step(pop, grid, lifecycle, contact, stepping, oxygen=ox)
+        tune = tune_sor(ox.field, ox.density, ox.species.kinetics, settings.omega)
+        roof = sor_roofline(ox, geom, device, w.name)
+        log(f"sweep only: {w.name} cells on {geom.shape}: {roof['shape']} {roof['sweep_us']} us per sweep, "
+            f"frac {roof['frac']}")
+        print(json.dumps({"workload": w.name, "grid": list(geom.shape), "roofline": roof, "sor_autotune": tune}),
+              flush=True)
         return 0
     w, geom, params, settings, lifecycle, contact, stepping, pop, ox, grid = setup(args.workload, device)
     log(f"bench: {w.name}: {pop.count} cells, grid {geom.shape}, warmup {args.warmup}, steps {args.steps}")
