@@ -511,6 +511,7 @@ def main() -> int:
         _lib.call("wbc_sor_fused_set_config", config_shapes().index(args.sor_shape))
         tune = dict(tune, forced=args.sor_shape)
         log(f"  sor shape forced: {args.sor_shape}")
+    headline_config = _lib.query("wbc_sor_fused_get_config")
     torch.cuda.synchronize()
     snap = None if args.no_cpu_baseline else snapshot(pop, ox)
     start_copy = None if args.no_cpu_baseline else device_copy(pop, ox)
@@ -544,6 +545,8 @@ def main() -> int:
         log(f"  batched: {bat}")
     cpu, accuracy = None, None
     if snap is not None:
+        # The batched leg selects each replica's tuned shape in turn; the headline's goes back.
+        _lib.call("wbc_sor_fused_set_config", headline_config)
         same = gpu_same_steps(start_copy, pop, ox, grid, lifecycle, contact, stepping, CPU_STEPS)
         cpu = cpu_baseline(snap, w, geom, params, settings, lifecycle, contact, stepping, same)
         log(f"  cpu baseline: {cpu['value']} cell-steps/s over {CPU_STEPS} steps ({cpu['seconds']} s, "
