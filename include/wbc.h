@@ -280,6 +280,10 @@ int wbc_adjoint_solve(float omega, int32_t max_sweeps, float tolerance, int32_t 
                       float dx, int32_t nx, int32_t ny, int32_t nz, float* grad_rate, double* sums,
                       int32_t* sweeps_out, float* residual_out, void* workspace, size_t workspace_bytes,
                       wbc_stream_t stream);
+/* The layout of wbc_adjoint_solve's sweeps: 1 (default) on colour-packed arrays with four packed
+ * indices per lane, as the forward's quad passes; 0 on the grid layout, one node per lane. The
+ * two give the same mu, gradients, sweeps and residual bit for bit. Process-wide. */
+int wbc_adjoint_set_packed(int32_t packed);
 
 /* ---- population metrics ---------------------------------------------------------------------
  * Device-side reductions behind metrics/population.py:13-106 (state_counts, population_summary,

@@ -47,11 +47,65 @@ def test_gradients_match_the_fp64_adjoint(oracle, device):
     assert errs["S"] < 1e-4 and errs["dq"] < 2e-3 and errs["dD"] < 2e-3 and errs["dOb"] < 2e-3
 
 
+def test_packed_adjoint_sweeps_equal_the_grid_layout_ones(device):
+    """wbc_adjoint_solve on the colour-packed quad lanes and on the grid layout, from the same
+    forward field, rates and g: mu, the rate gradients, both sums, the sweep count and the residual
+    are equal bit for bit, also with pinned interior nodes (mu held at 0) and with a budget that
+    stops the solve unconverged."""
+    import ctypes
+
+    from warpbiocell_amd import _lib
+    from warpbiocell_amd.fields.diffusion import float32_threshold
+
+    pos, rates, targets, geom = _moderate(n=2000)
+    prob = OxygenAdjointProblem(geom, torch.from_numpy(pos), 1.0, W.UPTAKE_MMHG_UM3_PER_H, device=device)
+    S = prob.forward(torch.from_numpy(rates), W.D_UM2_PER_H, 40.0)
+    prob.backward(2.0 * (S - torch.from_numpy(targets).to(device)))  # leaves g in prob._g
+    g = prob._g.clone()
+    state = prob.state
+    nx, ny, nz = geom.shape
+    h = geom.dx ** 2 / state.diffusion
+    omega = 2.0 / (1.0 + np.sin(np.pi / (max(geom.shape) - 1)))
+    rng = np.random.default_rng(7)
+    fixed = np.zeros(geom.shape, np.int32)
+    fixed[tuple(rng.integers(1, n - 1, 400) for n in geom.shape)] = 1
+    fixed_d = torch.from_numpy(fixed).to(device)
+    s = torch.cuda.current_stream(device).cuda_stream
+
+    def solve(packed, pinned, max_sweeps):
+        _lib.call("wbc_adjoint_set_packed", packed)
+        mu = torch.full(geom.shape, 7.0, dtype=torch.float32, device=device)  # overwritten entirely
+        grad = torch.empty(prob.n, dtype=torch.float32, device=device)
+        sums = (ctypes.c_double * 2)()
+        sweeps, residual = ctypes.c_int32(), ctypes.c_float()
+        _lib.call("wbc_adjoint_solve", omega, max_sweeps, float32_threshold(1e-6), 10, h, prob.q_ref,
+                  prob.michaelis_k, fixed_d.data_ptr() if pinned else None, state.field.values.data_ptr(),
+                  state.rate_density.data_ptr(), None, g.data_ptr(), mu.data_ptr(), prob.positions.data_ptr(),
+                  prob.n, geom.origin[0], geom.origin[1], geom.origin[2], geom.dx, nx, ny, nz, grad.data_ptr(),
+                  ctypes.byref(sums), ctypes.byref(sweeps), ctypes.byref(residual), prob._ws.data_ptr(),
+                  prob._ws.numel(), s)
+        torch.cuda.synchronize()
+        return mu.cpu().numpy(), grad.cpu().numpy(), tuple(sums), sweeps.value, residual.value
+
+    try:
+        for pinned, budget in ((False, 20000), (True, 20000), (False, 35)):
+            print(f"adjoint layouts: pinned {pinned}, max_sweeps {budget}", flush=True)
+            a, b = solve(1, pinned, budget), solve(0, pinned, budget)
+            np.testing.assert_array_equal(a[0].view(np.uint32), b[0].view(np.uint32))
+            np.testing.assert_array_equal(a[1].view(np.uint32), b[1].view(np.uint32))
+            assert a[2:] == b[2:], (a[2:], b[2:])
+            if pinned:
+                assert (a[0][fixed == 1] == 0.0).all()
+            assert a[3] > 0 and (budget != 35 or a[3] == 35)
+    finally:
+        _lib.call("wbc_adjoint_set_packed", 1)
+
+
 def test_two_forwards_then_both_backwards(device):
     """Two sampled_oxygen calls on one problem (different D and O_b) before either backward: each
     gradient is that of its own forward, equal to a forward+backward done alone within the adjoint
-    solve's tolerance (the adjoint SOR starts from the problem's previous adjoint solution, so the
-    order of the backwards moves the last bits); the two forwards' gradients differ by far more."""
+    solve's tolerance (the sampling adjoint scatters dL/dS into the grid with float atomics, whose
+    order moves the last bits); the two forwards' gradients differ by far more."""
     pos, rates, targets, geom = _moderate(n=800)
     prob = OxygenAdjointProblem(geom, torch.from_numpy(pos), 1.0, W.UPTAKE_MMHG_UM3_PER_H, device=device)
     t = torch.from_numpy(targets).to(device)

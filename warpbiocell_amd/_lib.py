@@ -70,6 +70,7 @@ SIGNATURES: dict[str, tuple] = {
     "wbc_fixed_to_float": (I32, [P, I64, F32, P, P]),
     "wbc_sample_adjoint": (I32, [P, P, I64, F32, F32, F32, F32, I32, I32, I32, P, P]),
     "wbc_adjoint_workspace_bytes": (c_size_t, [I32, I32, I32]),
+    "wbc_adjoint_set_packed": (I32, [I32]),
     "wbc_adjoint_solve": (I32, [F32, I32, F32, I32, F32, F32, F32, P, P, P, P, P, P, P, I64, F32, F32, F32, F32,
                                 I32, I32, I32, P, P, P, P, P, c_size_t, P]),
     "wbc_metrics_workspace_bytes": (c_size_t, [I32, I32]),

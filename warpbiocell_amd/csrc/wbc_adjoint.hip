@@ -17,6 +17,7 @@
 // The max(., 0) clamp of the forward update is taken as inactive at the fixed point (MM uptake
 // vanishes as O -> 0, so the discrete solution is non-negative without it).
 #include "wbc_field.hpp"
+#include "wbc_sor_packed.hpp"
 
 namespace wbc {
 
@@ -154,6 +155,144 @@ __global__ void __launch_bounds__(256) adjoint_color_kernel(AdjointParams a, int
     if (!adj_updated(a, i, j, k, t)) return;
     const float nb = mu[t - plane] + mu[t + plane] + mu[t - a.nz] + mu[t + a.nz] + mu[t - 1] + mu[t + 1];
     mu[t] = (1.0f - a.omega) * mu[t] + a.omega * fmaf(nb, inv_diag[t], rhs[t]);
+}
+
+// ---- the adjoint sweeps on colour-packed arrays ---------------------------------------------
+// mu, inv_diag and rhs in the colour-packed layout of the forward's quad passes: slot
+// e = (i ny + j) pitch + kk of colour c holds node k = 2 kk + ((c + i + j) & 1), a zero where
+// k >= nz (the row's padding). Every slot is written here, so the passes read no stale memory.
+// Pinned and face nodes get inv_diag = rhs = mu = 0: an update leaves such a mu at +0
+// ((1 - w) 0 + w fma(nb, 0, 0) = +0 for finite nb) and its residual term is 0, so they need no
+// mask and the sweeps equal the unpacked ones bit for bit.
+__global__ void __launch_bounds__(256) adjoint_setup_packed_kernel(AdjointParams a, const float* __restrict__ g,
+                                                                   int32_t pitch, float* __restrict__ inv0,
+                                                                   float* __restrict__ inv1, float* __restrict__ rhs0,
+                                                                   float* __restrict__ rhs1, float* __restrict__ mu0,
+                                                                   float* __restrict__ mu1)
+{
+    const int64_t per = (int64_t)a.nx * a.ny * pitch;
+    for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < 2 * per; s += (int64_t)gridDim.x * blockDim.x) {
+        const int c = s >= per ? 1 : 0;
+        const int64_t e = s - c * per;
+        const int64_t row = e / pitch;
+        const int kk = (int)(e - row * pitch);
+        const int i = (int)(row / a.ny), j = (int)(row - (int64_t)i * a.ny);
+        const int k = 2 * kk + ((c + i + j) & 1);
+        float id = 0.0f, r = 0.0f;
+        if (k < a.nz) {
+            const int64_t t = row * a.nz + k;
+            if (adj_updated(a, i, j, k, t)) {
+                const float kpo = a.michaelis_k + a.O[t];
+                const float diag = 6.0f + a.h * (a.q_ref * a.R[t]) * a.michaelis_k / (kpo * kpo);
+                id = 1.0f / diag;
+                r = g[t] * id;
+            }
+        }
+        (c ? inv1 : inv0)[e] = id;
+        (c ? rhs1 : rhs0)[e] = r;
+        (c ? mu1 : mu0)[e] = 0.0f;
+    }
+}
+
+// One lane of an adjoint colour pass of colour C on the quad lanes (wbc_sor_packed.hpp's lane
+// geometry and k-neighbour exchange): mu <- (1 - w) mu + w fma(nb, inv_diag, rhs) on the four
+// nodes, or with kResidual the maximum of |fma(nb, inv_diag, rhs) - mu| and no store.
+template <int C, bool kResidual>
+__device__ __forceinline__ float adjoint_quad_lane(const QuadGrid& g, int nz, float omega, uint32_t t, bool store,
+                                                   float* __restrict__ own, const float* __restrict__ other,
+                                                   const float* __restrict__ inv, const float* __restrict__ rhs)
+{
+    const uint32_t pl = fast_div(t, g.plane);
+    const uint32_t u = t - pl * g.plane.d;
+    const uint32_t row = fast_div(u, g.row);
+    const uint32_t q = u - row * g.row.d;
+    const int i = (int)pl + 1, j = (int)row + 1;
+    const int kk0 = 4 * (int)q;
+    const int par = (C + i + j) & 1;
+    const uint32_t w = (uint32_t)i * g.plane_bytes + (uint32_t)j * g.row_bytes + (uint32_t)kk0 * 4u;
+    const char* ob = (const char*)other;
+    auto at = [](const void* base, uint32_t off) { return *(const wbc_f4*)((const char*)base + off); };
+    const wbc_f4 jm = at(ob, w - g.row_bytes), jc = at(ob, w), jp = at(ob, w + g.row_bytes);
+    const wbc_f4 im = at(ob, w - g.plane_bytes), ip = at(ob, w + g.plane_bytes);
+    const wbc_f4 o = at(own, w), di = at(inv, w), r = at(rhs, w);
+    const int lane = (int)__lane_id();
+    const bool load_right = lane == kWave - 1 || q == g.row.d - 1;
+    float edge_left = 0.0f, edge_right = 0.0f;
+    if (lane == 0) edge_left = *(const float*)(ob + w - 4);
+    if (load_right) edge_right = *(const float*)(ob + w + 16);
+    const float from_right = lane_from_right(jc.x, edge_right);
+    const float ext[6] = {lane_from_left(jc.w, edge_left), jc.x, jc.y, jc.z, jc.w, load_right ? edge_right : from_right};
+    const int k0 = 2 * kk0 + par;
+    wbc_f4 val;
+    float m = 0.0f;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        val[e] = o[e];
+        const int k = k0 + 2 * e;
+        if (k < 1 || k > nz - 2) continue;
+        const float km = par ? ext[e + 1] : ext[e];
+        const float kp = par ? ext[e + 2] : ext[e + 1];
+        const float nb = im[e] + ip[e] + jm[e] + jp[e] + km + kp;
+        const float x = fmaf(nb, di[e], r[e]);
+        if (kResidual)
+            m = fmaxf(m, fabsf(x - o[e]));
+        else
+            val[e] = (1.0f - omega) * o[e] + omega * x;
+    }
+    if (!kResidual && store) __builtin_nontemporal_store(val, (wbc_f4*)((char*)own + w));
+    return m;
+}
+
+template <int C>
+__global__ void __launch_bounds__(256) adjoint_quad_kernel(QuadGrid g, int nz, float omega, float* __restrict__ own,
+                                                           const float* __restrict__ other,
+                                                           const float* __restrict__ inv,
+                                                           const float* __restrict__ rhs)
+{
+    uint32_t t = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
+    const bool in_range = t < (uint32_t)g.total;
+    if (!in_range) t = (uint32_t)g.total - 1;  // a duplicate of the last lane: loads, no store
+    adjoint_quad_lane<C, false>(g, nz, omega, t, in_range, own, other, inv, rhs);
+}
+
+// The residual maxima of both colours: workgroups [0, B) take colour 0, [B, 2B) colour 1, one
+// maximum each into partial.
+__global__ void __launch_bounds__(256) adjoint_quad_residual_kernel(QuadGrid g, int nz, int32_t B,
+                                                                    const float* __restrict__ mu0,
+                                                                    const float* __restrict__ mu1,
+                                                                    const float* __restrict__ inv0,
+                                                                    const float* __restrict__ inv1,
+                                                                    const float* __restrict__ rhs0,
+                                                                    const float* __restrict__ rhs1,
+                                                                    float* __restrict__ partial)
+{
+    __shared__ float wm[4];
+    const int tile = xcd_tile(blockIdx.x, gridDim.x);
+    const int colour = tile >= B ? 1 : 0;
+    uint32_t t = (uint32_t)(tile - colour * B) * 256u + threadIdx.x;
+    const bool in_range = t < (uint32_t)g.total;
+    if (!in_range) t = (uint32_t)g.total - 1;
+    float m = colour ? adjoint_quad_lane<1, true>(g, nz, 0.0f, t, false, const_cast<float*>(mu1), mu0, inv1, rhs1)
+                     : adjoint_quad_lane<0, true>(g, nz, 0.0f, t, false, const_cast<float*>(mu0), mu1, inv0, rhs0);
+    if (!in_range) m = 0.0f;  // a duplicate's wave neighbours are not the last lane's
+    m = wave_max(m);
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) partial[blockIdx.x] = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
+}
+
+// mu on the grid from the packed arrays (faces and pinned nodes hold their zeros).
+__global__ void __launch_bounds__(256) adjoint_unpack_kernel(int nx, int ny, int nz, int32_t pitch,
+                                                             const float* __restrict__ mu0,
+                                                             const float* __restrict__ mu1, float* __restrict__ mu)
+{
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nodes; t += (int64_t)gridDim.x * blockDim.x) {
+        const int64_t row = t / nz;
+        const int k = (int)(t - row * nz);
+        const int i = (int)(row / ny), j = (int)(row - (int64_t)i * ny);
+        mu[t] = (((i + j + k) & 1) ? mu1 : mu0)[row * pitch + (k >> 1)];
+    }
 }
 
 // Per-block maxima of |g - A mu| (scaled by inv_diag: |rhs + nb*inv_diag - mu|) for the check.
@@ -350,14 +489,56 @@ WBC_API int wbc_sample_adjoint(const float* grad_out, const float* position, int
 
 // Workspace: inv_diag and rhs (nodes floats each), residual partials, double partials, and the
 // reduced scalars.
+namespace {
+// Whether the adjoint sweeps run on the colour-packed arrays (process-wide; the default) or on
+// the grid layout, which the packed layout equals bit for bit.
+int g_adjoint_packed = 1;
+
+// Workspace of the adjoint solve: the grid-layout inverse diagonal and right-hand side, the
+// residual partials (the larger of the two layouts' counts), the double partials of the global
+// sums, the reduced scalars, then six colour-packed arrays (mu, inv_diag, rhs per colour).
+struct AdjointLayout {
+    int64_t parts, max_parts, packed_each;
+    size_t inv, rhs, partial, dpartial, scalars, packed, total;
+};
+AdjointLayout adjoint_layout(int32_t nx, int32_t ny, int32_t nz)
+{
+    auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
+    const int64_t nodes = (int64_t)nx * ny * nz;
+    const dim3 g = strided_grid(nx, ny, nz);
+    AdjointLayout L;
+    L.parts = (int64_t)g.x * g.y * g.z;
+    SorParams p{};
+    p.nx = nx;
+    p.ny = ny;
+    p.nz = nz;
+    QuadGrid qg;
+    const int64_t qparts = quad_grid_of(p, qg) ? 2 * (((int64_t)qg.total + 255) / 256) : 0;
+    L.max_parts = L.parts > qparts ? L.parts : qparts;
+    L.packed_each = up(packed_floats(p) * 4);
+    size_t off = 0;
+    L.inv = off;      off += up(nodes * 4);
+    L.rhs = off;      off += up(nodes * 4);
+    L.partial = off;  off += up(L.max_parts * 4);
+    L.dpartial = off; off += up(L.parts * 16);
+    L.scalars = off;  off += 256;
+    L.packed = off;   off += 6 * L.packed_each;
+    L.total = off;
+    return L;
+}
+}  // namespace
+
 WBC_API size_t wbc_adjoint_workspace_bytes(int32_t nx, int32_t ny, int32_t nz)
 {
     if (nx < 3 || ny < 3 || nz < 3) return 0;
-    const int64_t nodes = (int64_t)nx * ny * nz;
-    dim3 g = strided_grid(nx, ny, nz);
-    const int64_t parts = (int64_t)g.x * g.y * g.z;
-    auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
-    return (size_t)(up(nodes * 4) * 2 + up(parts * 4) + up(parts * 16) + 256);
+    return adjoint_layout(nx, ny, nz).total;
+}
+
+WBC_API int wbc_adjoint_set_packed(int32_t packed)
+{
+    if (packed != 0 && packed != 1) return (int)hipErrorInvalidValue;
+    g_adjoint_packed = packed;
+    return 0;
 }
 
 // Solve A mu = g by red-black SOR from mu = 0, checking max|nb*inv_diag + rhs - mu| every
@@ -375,26 +556,45 @@ WBC_API int wbc_adjoint_solve(float omega, int32_t max_sweeps, float tolerance, 
     if (!O || !R || !g || !mu || !workspace || !sums || !sweeps_out || !residual_out) return (int)hipErrorInvalidValue;
     if (n > 0 && (!position || !grad_rate)) return (int)hipErrorInvalidValue;
     if (workspace_bytes < wbc_adjoint_workspace_bytes(nx, ny, nz)) return (int)hipErrorInvalidValue;
-    const int64_t nodes = (int64_t)nx * ny * nz;
-    auto up = [](int64_t b) { return (b + 255) / 256 * 256; };
+    const AdjointLayout L = adjoint_layout(nx, ny, nz);
     char* ws = (char*)workspace;
-    float* inv_diag = (float*)ws;
-    float* rhs = (float*)(ws + up(nodes * 4));
-    float* partial = (float*)(ws + 2 * up(nodes * 4));
-    dim3 sg = strided_grid(nx, ny, nz);
-    const int parts = (int)(sg.x * sg.y * sg.z);
-    double* dpartial = (double*)(ws + 2 * up(nodes * 4) + up((int64_t)parts * 4));
-    char* scalars = ws + 2 * up(nodes * 4) + up((int64_t)parts * 4) + up((int64_t)parts * 16);
-    float* dmax = (float*)scalars;            // reduced maximum
-    double* dsums = (double*)(scalars + 64);  // reduced sums
+    float* inv_diag = (float*)(ws + L.inv);
+    float* rhs = (float*)(ws + L.rhs);
+    float* partial = (float*)(ws + L.partial);
+    const dim3 sg = strided_grid(nx, ny, nz);
+    const int parts = (int)L.parts;
+    double* dpartial = (double*)(ws + L.dpartial);
+    float* dmax = (float*)(ws + L.scalars);            // reduced maximum
+    double* dsums = (double*)(ws + L.scalars + 64);    // reduced sums
     AdjointParams a{omega, h, q_ref, michaelis_k, fixed, O, R, nx, ny, nz};
     const dim3 blk(64, 4, 1);
-    adjoint_setup_kernel<<<node_grid(nx, ny, nz), blk, 0, stream>>>(a, g, inv_diag, rhs, mu);
+    SorParams gp{};
+    gp.nx = nx;
+    gp.ny = ny;
+    gp.nz = nz;
+    QuadGrid qg;
+    const bool packed = g_adjoint_packed && quad_grid_of(gp, qg);
+    const int32_t pitch = packed_pitch(nz, kQuadRows);
+    float* pk[6];
+    for (int e = 0; e < 6; ++e) pk[e] = (float*)(ws + L.packed + e * L.packed_each);
+    float *mu0 = pk[0], *mu1 = pk[1], *inv0 = pk[2], *inv1 = pk[3], *rhs0 = pk[4], *rhs1 = pk[5];
+    const int32_t qblocks = packed ? (int32_t)grid_for(qg.total, 256) : 0;
+    if (packed) {
+        adjoint_setup_packed_kernel<<<2048, 256, 0, stream>>>(a, g, pitch, inv0, inv1, rhs0, rhs1, mu0, mu1);
+    } else {
+        adjoint_setup_kernel<<<node_grid(nx, ny, nz), blk, 0, stream>>>(a, g, inv_diag, rhs, mu);
+    }
     WBC_CHECK_LAUNCH();
     // Scale of the right-hand side for the relative stopping test: max|rhs| (the residual at mu = 0).
     auto reduced_max = [&](float& out) -> hipError_t {
-        adjoint_residual_kernel<<<sg, blk, 0, stream>>>(a, inv_diag, rhs, mu, partial);
-        max_reduce_kernel<<<1, 256, 0, stream>>>(partial, parts, dmax);
+        if (packed) {
+            adjoint_quad_residual_kernel<<<2 * qblocks, 256, 0, stream>>>(qg, nz, qblocks, mu0, mu1, inv0, inv1, rhs0,
+                                                                         rhs1, partial);
+            max_reduce_kernel<<<1, 256, 0, stream>>>(partial, 2 * qblocks, dmax);
+        } else {
+            adjoint_residual_kernel<<<sg, blk, 0, stream>>>(a, inv_diag, rhs, mu, partial);
+            max_reduce_kernel<<<1, 256, 0, stream>>>(partial, parts, dmax);
+        }
         WBC_TRY_ERR(hipGetLastError());
         WBC_TRY_ERR(hipMemcpyAsync(&out, dmax, sizeof(float), hipMemcpyDeviceToHost, stream));
         return hipStreamSynchronize(stream);
@@ -407,14 +607,23 @@ WBC_API int wbc_adjoint_solve(float omega, int32_t max_sweeps, float tolerance, 
     while (scale > 0.0f && sweeps < max_sweeps) {
         const int32_t chunk = max_sweeps - sweeps < check_every ? max_sweeps - sweeps : check_every;
         for (int32_t s = 0; s < chunk; ++s) {
-            adjoint_color_kernel<<<cg, blk, 0, stream>>>(a, 0, inv_diag, rhs, mu);
-            adjoint_color_kernel<<<cg, blk, 0, stream>>>(a, 1, inv_diag, rhs, mu);
+            if (packed) {
+                adjoint_quad_kernel<0><<<qblocks, 256, 0, stream>>>(qg, nz, omega, mu0, mu1, inv0, rhs0);
+                adjoint_quad_kernel<1><<<qblocks, 256, 0, stream>>>(qg, nz, omega, mu1, mu0, inv1, rhs1);
+            } else {
+                adjoint_color_kernel<<<cg, blk, 0, stream>>>(a, 0, inv_diag, rhs, mu);
+                adjoint_color_kernel<<<cg, blk, 0, stream>>>(a, 1, inv_diag, rhs, mu);
+            }
         }
         WBC_CHECK_LAUNCH();
         sweeps += chunk;
         WBC_TRY(reduced_max(res));
         res /= scale;
         if (res < tolerance) break;
+    }
+    if (packed) {
+        adjoint_unpack_kernel<<<2048, 256, 0, stream>>>(nx, ny, nz, pitch, mu0, mu1, mu);
+        WBC_CHECK_LAUNCH();
     }
     if (n > 0) {
         rate_gradient_kernel<<<grid_for(n, 256), 256, 0, stream>>>(a, mu, position, n, ox, oy, oz, 1.0f / dx,
