@@ -197,11 +197,15 @@ __global__ void __launch_bounds__(256) adjoint_setup_packed_kernel(AdjointParams
 // One lane of an adjoint colour pass of colour C on the quad lanes (wbc_sor_packed.hpp's lane
 // geometry and k-neighbour exchange): mu <- (1 - w) mu + w fma(nb, inv_diag, rhs) on the four
 // nodes, or with kResidual the maximum of |fma(nb, inv_diag, rhs) - mu| and no store.
+// A set latch `done` (the solve has converged) makes the lane return -1 without storing; it is
+// read after the loads are issued, so its latency overlaps theirs.
 template <int C, bool kResidual>
 __device__ __forceinline__ float adjoint_quad_lane(const QuadGrid& g, int nz, float omega, uint32_t t, bool store,
                                                    float* __restrict__ own, const float* __restrict__ other,
-                                                   const float* __restrict__ inv, const float* __restrict__ rhs)
+                                                   const float* __restrict__ inv, const float* __restrict__ rhs,
+                                                   const int32_t* __restrict__ done)
 {
+    const int32_t stop = done ? *done : 0;
     const uint32_t pl = fast_div(t, g.plane);
     const uint32_t u = t - pl * g.plane.d;
     const uint32_t row = fast_div(u, g.row);
@@ -220,6 +224,7 @@ __device__ __forceinline__ float adjoint_quad_lane(const QuadGrid& g, int nz, fl
     float edge_left = 0.0f, edge_right = 0.0f;
     if (lane == 0) edge_left = *(const float*)(ob + w - 4);
     if (load_right) edge_right = *(const float*)(ob + w + 16);
+    if (stop) return -1.0f;
     const float from_right = lane_from_right(jc.x, edge_right);
     const float ext[6] = {lane_from_left(jc.w, edge_left), jc.x, jc.y, jc.z, jc.w, load_right ? edge_right : from_right};
     const int k0 = 2 * kk0 + par;
@@ -247,12 +252,13 @@ template <int C>
 __global__ void __launch_bounds__(256) adjoint_quad_kernel(QuadGrid g, int nz, float omega, float* __restrict__ own,
                                                            const float* __restrict__ other,
                                                            const float* __restrict__ inv,
-                                                           const float* __restrict__ rhs)
+                                                           const float* __restrict__ rhs,
+                                                           const int32_t* __restrict__ done)
 {
     uint32_t t = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
     const bool in_range = t < (uint32_t)g.total;
     if (!in_range) t = (uint32_t)g.total - 1;  // a duplicate of the last lane: loads, no store
-    adjoint_quad_lane<C, false>(g, nz, omega, t, in_range, own, other, inv, rhs);
+    adjoint_quad_lane<C, false>(g, nz, omega, t, in_range, own, other, inv, rhs, done);
 }
 
 // The residual maxima of both colours: workgroups [0, B) take colour 0, [B, 2B) colour 1, one
@@ -264,6 +270,7 @@ __global__ void __launch_bounds__(256) adjoint_quad_residual_kernel(QuadGrid g, 
                                                                     const float* __restrict__ inv1,
                                                                     const float* __restrict__ rhs0,
                                                                     const float* __restrict__ rhs1,
+                                                                    const int32_t* __restrict__ done,
                                                                     float* __restrict__ partial)
 {
     __shared__ float wm[4];
@@ -272,13 +279,54 @@ __global__ void __launch_bounds__(256) adjoint_quad_residual_kernel(QuadGrid g, 
     uint32_t t = (uint32_t)(tile - colour * B) * 256u + threadIdx.x;
     const bool in_range = t < (uint32_t)g.total;
     if (!in_range) t = (uint32_t)g.total - 1;
-    float m = colour ? adjoint_quad_lane<1, true>(g, nz, 0.0f, t, false, const_cast<float*>(mu1), mu0, inv1, rhs1)
-                     : adjoint_quad_lane<0, true>(g, nz, 0.0f, t, false, const_cast<float*>(mu0), mu1, inv0, rhs0);
+    float m = colour ? adjoint_quad_lane<1, true>(g, nz, 0.0f, t, false, const_cast<float*>(mu1), mu0, inv1, rhs1, done)
+                     : adjoint_quad_lane<0, true>(g, nz, 0.0f, t, false, const_cast<float*>(mu0), mu1, inv0, rhs0, done);
+    if (m < 0.0f) return;     // converged (the same for every lane)
     if (!in_range) m = 0.0f;  // a duplicate's wave neighbours are not the last lane's
     m = wave_max(m);
     if ((threadIdx.x & 63) == 0) wm[threadIdx.x / 64] = m;
     __syncthreads();
     if (threadIdx.x == 0) partial[blockIdx.x] = fmaxf(fmaxf(wm[0], wm[1]), fmaxf(wm[2], wm[3]));
+}
+
+// Chunk c's check: reduce the partials (in any order: a maximum), divide by the right-hand side's
+// scale as the synchronous loop does on the host, store the relative residual into the mapped
+// host slot c and latch convergence (state[0]) below the tolerance. A check after convergence
+// leaves everything alone.
+__global__ void __launch_bounds__(1024) adjoint_finish_kernel(const float* __restrict__ partial, int count,
+                                                              const float* __restrict__ scale, float tolerance,
+                                                              int32_t chunk, float* __restrict__ host_out,
+                                                              int32_t* __restrict__ state)
+{
+    __shared__ float wm[16];
+    float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
+    int b = threadIdx.x;
+    for (; b + 3 * 1024 < count; b += 4 * 1024) {
+        m0 = fmaxf(m0, partial[b]);
+        m1 = fmaxf(m1, partial[b + 1024]);
+        m2 = fmaxf(m2, partial[b + 2048]);
+        m3 = fmaxf(m3, partial[b + 3072]);
+    }
+    for (; b < count; b += 1024) m0 = fmaxf(m0, partial[b]);
+    if (state[0] != 0) return;
+    float m = wave_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)));
+    if ((threadIdx.x & 63) == 0) wm[threadIdx.x / 64] = m;
+    __syncthreads();
+    if (threadIdx.x == 0) {
+        m = wm[0];
+        for (int w = 1; w < 16; ++w) m = fmaxf(m, wm[w]);
+        const float res = m / scale[0];
+        host_out[chunk] = res;
+        if (res < tolerance) {
+            state[0] = 1;
+            state[1] = chunk;
+        }
+    }
+}
+
+__global__ void __launch_bounds__(64) adjoint_clear_kernel(int32_t* __restrict__ p, int n)
+{
+    if ((int)threadIdx.x < n) p[threadIdx.x] = 0;
 }
 
 // mu on the grid from the packed arrays (faces and pinned nodes hold their zeros).
@@ -494,6 +542,43 @@ namespace {
 // the grid layout, which the packed layout equals bit for bit.
 int g_adjoint_packed = 1;
 
+// Per device: mapped, coherent host memory the packed loop's checks write their residuals into
+// (grown on demand) and the chunk events, created on first use and kept until the process ends.
+constexpr int kAdjEvents = 8;
+constexpr int32_t kAdjAhead = 4;
+struct AdjointHost {
+    float* host = nullptr;
+    float* dev = nullptr;
+    int32_t cap = 0;
+    bool events = false;
+    hipEvent_t ev[kAdjEvents];
+};
+AdjointHost g_adjoint_host[16];
+
+hipError_t adjoint_host(int32_t chunks, AdjointHost** out)
+{
+    int d = 0;
+    WBC_TRY_ERR(hipGetDevice(&d));
+    if (d < 0 || d >= 16) return hipErrorInvalidDevice;
+    AdjointHost& h = g_adjoint_host[d];
+    if (!h.events) {
+        for (int e = 0; e < kAdjEvents; ++e) WBC_TRY_ERR(hipEventCreateWithFlags(&h.ev[e], hipEventDisableTiming));
+        h.events = true;
+    }
+    if (h.cap < chunks) {
+        if (h.host) WBC_TRY_ERR(hipHostFree(h.host));
+        h.host = h.dev = nullptr;
+        h.cap = 0;
+        const int32_t cap = chunks < 1024 ? 1024 : chunks;
+        WBC_TRY_ERR(hipHostMalloc((void**)&h.host, (size_t)cap * sizeof(float),
+                                  hipHostMallocMapped | hipHostMallocCoherent));
+        WBC_TRY_ERR(hipHostGetDevicePointer((void**)&h.dev, h.host, 0));
+        h.cap = cap;
+    }
+    *out = &h;
+    return hipSuccess;
+}
+
 // Workspace of the adjoint solve: the grid-layout inverse diagonal and right-hand side, the
 // residual partials (the larger of the two layouts' counts), the double partials of the global
 // sums, the reduced scalars, then six colour-packed arrays (mu, inv_diag, rhs per colour).
@@ -589,7 +674,7 @@ WBC_API int wbc_adjoint_solve(float omega, int32_t max_sweeps, float tolerance, 
     auto reduced_max = [&](float& out) -> hipError_t {
         if (packed) {
             adjoint_quad_residual_kernel<<<2 * qblocks, 256, 0, stream>>>(qg, nz, qblocks, mu0, mu1, inv0, inv1, rhs0,
-                                                                         rhs1, partial);
+                                                                         rhs1, nullptr, partial);
             max_reduce_kernel<<<1, 256, 0, stream>>>(partial, 2 * qblocks, dmax);
         } else {
             adjoint_residual_kernel<<<sg, blk, 0, stream>>>(a, inv_diag, rhs, mu, partial);
@@ -600,20 +685,59 @@ WBC_API int wbc_adjoint_solve(float omega, int32_t max_sweeps, float tolerance, 
         return hipStreamSynchronize(stream);
     };
     float scale = 0.0f;
-    WBC_TRY(reduced_max(scale));
+    WBC_TRY(reduced_max(scale));  // also leaves max|rhs| in dmax for the packed loop's checks
     const dim3 cg(grid_for((nz + 1) / 2, 64), grid_for(ny, 4), nx);
     int32_t sweeps = 0;
     float res = scale;
-    while (scale > 0.0f && sweeps < max_sweeps) {
+    if (packed && scale > 0.0f) {
+        // The chunk loop of the grid layout below, with the checks pipelined as in the forward
+        // solve loop: sweeps of chunk c+1 are queued before the host reads chunk c's residual from
+        // mapped memory, and a device latch set by the converging check turns the queued sweeps
+        // and checks into no-ops, so the sweeps, mu and residual are the synchronous loop's.
+        const int32_t chunks = (max_sweeps + check_every - 1) / check_every;
+        auto chunk_len = [&](int32_t c) {
+            return max_sweeps - c * check_every < check_every ? max_sweeps - c * check_every : check_every;
+        };
+        AdjointHost* hs;
+        WBC_TRY(adjoint_host(chunks, &hs));
+        int32_t* state = (int32_t*)(ws + L.scalars + 128);  // [0] converged latch, [1] converged chunk
+        adjoint_clear_kernel<<<1, 64, 0, stream>>>(state, 2);
+        WBC_CHECK_LAUNCH();
+        // While the host waits on chunk c's check, kAdjAhead sweeps of chunk c+1 are queued behind
+        // it: enough to cover the host's wake-up and next launches, and all that runs as no-ops
+        // after convergence (a whole chunk ahead would be up to check_every no-op sweeps).
+        int32_t qs = 0, recorded = 0, checked = 0;
+        auto queue_sweep = [&]() -> hipError_t {
+            adjoint_quad_kernel<0><<<qblocks, 256, 0, stream>>>(qg, nz, omega, mu0, mu1, inv0, rhs0, state);
+            adjoint_quad_kernel<1><<<qblocks, 256, 0, stream>>>(qg, nz, omega, mu1, mu0, inv1, rhs1, state);
+            ++qs;
+            if (qs == max_sweeps || qs % check_every == 0) {  // the chunk ends: its check
+                adjoint_quad_residual_kernel<<<2 * qblocks, 256, 0, stream>>>(qg, nz, qblocks, mu0, mu1, inv0, inv1,
+                                                                             rhs0, rhs1, state, partial);
+                adjoint_finish_kernel<<<1, 1024, 0, stream>>>(partial, 2 * qblocks, dmax, tolerance, recorded,
+                                                              hs->dev, state);
+                WBC_TRY_ERR(hipGetLastError());
+                WBC_TRY_ERR(hipEventRecord(hs->ev[recorded % kAdjEvents], stream));
+                ++recorded;
+            }
+            return hipGetLastError();
+        };
+        while (true) {
+            const int64_t target = (int64_t)(checked + 1) * check_every + kAdjAhead;
+            while (qs < max_sweeps && qs < target) WBC_TRY(queue_sweep());
+            if (checked == recorded) break;
+            WBC_TRY(hipEventSynchronize(hs->ev[checked % kAdjEvents]));
+            res = ((volatile float*)hs->host)[checked];
+            sweeps += chunk_len(checked);
+            ++checked;
+            if (res < tolerance) break;
+        }
+    }
+    while (!packed && scale > 0.0f && sweeps < max_sweeps) {
         const int32_t chunk = max_sweeps - sweeps < check_every ? max_sweeps - sweeps : check_every;
         for (int32_t s = 0; s < chunk; ++s) {
-            if (packed) {
-                adjoint_quad_kernel<0><<<qblocks, 256, 0, stream>>>(qg, nz, omega, mu0, mu1, inv0, rhs0);
-                adjoint_quad_kernel<1><<<qblocks, 256, 0, stream>>>(qg, nz, omega, mu1, mu0, inv1, rhs1);
-            } else {
-                adjoint_color_kernel<<<cg, blk, 0, stream>>>(a, 0, inv_diag, rhs, mu);
-                adjoint_color_kernel<<<cg, blk, 0, stream>>>(a, 1, inv_diag, rhs, mu);
-            }
+            adjoint_color_kernel<<<cg, blk, 0, stream>>>(a, 0, inv_diag, rhs, mu);
+            adjoint_color_kernel<<<cg, blk, 0, stream>>>(a, 1, inv_diag, rhs, mu);
         }
         WBC_CHECK_LAUNCH();
         sweeps += chunk;
