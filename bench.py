@@ -7,19 +7,20 @@ decisions and slot allocation -> 4 contact substeps (hash-grid build, gather, in
 Inputs are synthetic (``warpbiocell_amd.workloads.spheroid_100k_161``) and resident on the GPU
 before the timed region. One JSON line goes to stdout; progress goes to stderr.
 
-Roofline: the fused SOR sweep kernel, timed live with HIP events on the stream it runs on,
-priced at the sweep's algorithmic bytes, 16 B per node (each of the two colour passes reads the
-field once, the sweep writes it once and reads the density once), against MI355X's 8 TB/s HBM
-peak. The fused kernel itself needs only 12 B per node (one field read); that figure is
-reported beside it.
+Roofline: the SOR shape the autotuner selects (on the headline grid, the two colour-pass
+launches of `packed/quad`), timed live with HIP events on the stream it runs on over 1000 sweeps,
+priced at 8 B per node per launch (16 B per sweep: each of the two colour passes reads the field
+once, the sweep writes it once and reads the density once) against MI355X's 8 TB/s HBM peak.
+`--sweep-grid N` times the same on an N^3 grid.
 
 CPU leg (the only part of the benchmark that uses the oracle, the C restatement of the reference,
 and only as the checker): the oracle runs the first CPU_STEPS steps of the timed region from the
 exact state they started at, on every host core the process may use (and the first two on one
 thread); the GPU re-runs the same steps from a device copy of that state, so both sides are timed
 on the same inputs and sweeps. The same leg reports accuracy against float64: the final headline
-field against a float64 solve of the same discrete problem started from it, and the gradient of
-the 129^3 adjoint workload against the float64 adjoint and central differences of float64 solves.
+field against a float64 solve of the same discrete problem started from it, the contact gather at
+the final positions against a float64 restatement over every pair, and the gradient of the 129^3
+adjoint workload against the float64 adjoint and central differences of float64 solves.
 """
 
 from __future__ import annotations
@@ -408,6 +409,43 @@ def field_accuracy(field32, density32, params, geom) -> dict:
             "fp64_residual_of_fp32_field": res32}
 
 
+def mechanics_accuracy(pop, grid, contact) -> dict:
+    """The contact gather at the final positions (one substep with dt = 0, which moves nothing)
+    against a float64 restatement of kernels/mechanics_kernels.py:26-57 over every pair within the
+    query radius (pairs found with scipy's cKDTree): velocities and neighbour counts."""
+    import numpy as np
+    import torch
+    from scipy.spatial import cKDTree
+
+    from warpbiocell_amd.cells.mechanics import contact_substep
+
+    contact_substep(pop, grid, contact, 0.0)
+    torch.cuda.synchronize()
+    n = pop.count
+    x = pop.position[:n].cpu().numpy().astype(np.float64)
+    r = pop.radius[:n].cpu().numpy().astype(np.float64)
+    v32 = pop.velocity[:n].cpu().numpy().astype(np.float64)
+    c32 = pop.neighbor_count[:n].cpu().numpy()
+    q, rate = float(contact.query_radius), float(contact.rate)
+    pairs = cKDTree(x).query_pairs(q * (1.0 + 1e-5), output_type="ndarray")
+    i, j = pairs[:, 0], pairs[:, 1]
+    dv = x[i] - x[j]
+    d = np.sqrt(np.sum(dv * dv, axis=1))
+    near = d < q
+    i, j, dv, d = i[near], j[near], dv[near], d[near]
+    count = np.bincount(i, minlength=n) + np.bincount(j, minlength=n)
+    push = (r[i] + r[j] - d > 0.0) & (d > 1e-6)
+    coef = np.where(push, rate * (r[i] + r[j] - d) / np.where(d > 0.0, d, 1.0), 0.0)
+    v64 = np.zeros((n, 3))
+    np.add.at(v64, i, coef[:, None] * dv)
+    np.add.at(v64, j, -coef[:, None] * dv)
+    scale = float(np.abs(v64).max())
+    return {"cells": n, "pairs_within_query_radius": int(near.sum()),
+            "velocity_max_abs_err_over_max_speed": float(np.abs(v32 - v64).max() / scale) if scale > 0 else 0.0,
+            "velocity_rel_l2_err": float(np.linalg.norm(v32 - v64) / np.linalg.norm(v64)) if scale > 0 else 0.0,
+            "neighbor_count_mismatches": int(np.count_nonzero(c32 != count))}
+
+
 def adjoint_accuracy(gradient: dict) -> dict:
     """The 129^3 adjoint workload's GPU gradient against the float64 adjoint (every rate, D, O_b) and
     central differences of float64 solves on 64 rates (oracle.adjoint_check)."""
@@ -531,6 +569,7 @@ def main() -> int:
 
     final_field = ox.numpy() if snap is not None else None
     final_density = ox.density.cpu().numpy() if snap is not None else None
+    mech_accuracy = mechanics_accuracy(pop, grid, contact) if snap is not None else None
     roof = None if args.no_roofline else sor_roofline(ox, geom, device, w.name)
     if roof:
         log(f"  sor sweep ({roof['shape']}): {roof['sweep_us']} us, {roof['achieved']} GB/s algorithmic")
@@ -551,8 +590,9 @@ def main() -> int:
         cpu = cpu_baseline(snap, w, geom, params, settings, lifecycle, contact, stepping, same)
         log(f"  cpu baseline: {cpu['value']} cell-steps/s over {CPU_STEPS} steps ({cpu['seconds']} s, "
             f"sweeps {sum(cpu['sweeps'])}); GPU same steps {same['value']} cell-steps/s ({same['seconds']} s)")
-        accuracy = {"field": field_accuracy(final_field, final_density, params, geom)}
+        accuracy = {"field": field_accuracy(final_field, final_density, params, geom), "mechanics": mech_accuracy}
         log(f"  accuracy, headline field vs float64: {accuracy['field']}")
+        log(f"  accuracy, contact gather at the final positions vs float64: {mech_accuracy}")
         if gradient is not None:
             accuracy["adjoint"] = adjoint_accuracy(gradient)
             log("  accuracy, adjoint gradient: " + str({k: v for k, v in accuracy["adjoint"].items()

@@ -35,10 +35,14 @@ def test_counts_match_brute_force_and_velocities_are_bit_exact(oracle, device):
     contact_substep(pop, make_neighbor_grid(params, device), params, dt=0.0)
     vel, cnt = oracle.contact_velocities(pos, radii, np.float32(params.query_radius), 128, params.query_radius,
                                          params.rate)
-    _, brute = contact_brute_force(pos, radii, params.rate, params.query_radius)
+    v64, brute = contact_brute_force(pos, radii, params.rate, params.query_radius)
     np.testing.assert_array_equal(pop.neighbor_counts_numpy(), brute)
     np.testing.assert_array_equal(pop.neighbor_counts_numpy(), cnt)
     np.testing.assert_array_equal(pop.velocity.cpu().numpy(), vel)
+    # against float64 (the accuracy bound for mechanics: 1e-5 relative)
+    err = np.linalg.norm(pop.velocity.cpu().numpy() - v64) / np.linalg.norm(v64)
+    print(f"contact velocities vs float64: relative L2 {err:.2e}")
+    assert err < 1e-5
     np.testing.assert_array_equal(pop.positions_numpy(), pos)
 
 
