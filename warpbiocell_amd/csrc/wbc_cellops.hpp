@@ -5,37 +5,56 @@
 
 namespace wbc {
 
+// The trilinear support of one cell (kernels/field_kernels.py:88-101): the lower corner node and
+// the fractional offsets, or false for a cell outside the grid's support or of no valid state.
+__device__ __forceinline__ bool deposit_support(const float* __restrict__ position, int64_t c, int32_t s, float ox,
+                                                float oy, float oz, float inv_dx, int32_t nx, int32_t ny, int32_t nz,
+                                                int32_t& i0, int32_t& j0, int32_t& k0, float& fx, float& fy,
+                                                float& fz)
+{
+    if (s < 0 || s >= 4) return false;
+    float ux = (position[3 * c + 0] - ox) * inv_dx;
+    float uy = (position[3 * c + 1] - oy) * inv_dx;
+    float uz = (position[3 * c + 2] - oz) * inv_dx;
+    i0 = (int32_t)floorf(ux);
+    j0 = (int32_t)floorf(uy);
+    k0 = (int32_t)floorf(uz);
+    if (i0 < 0 || j0 < 0 || k0 < 0) return false;
+    if (i0 + 1 >= nx || j0 + 1 >= ny || k0 + 1 >= nz) return false;
+    fx = ux - (float)i0;
+    fy = uy - (float)j0;
+    fz = uz - (float)k0;
+    return true;
+}
+
+// The fixed-point weight int(w * 2^24 + 0.5) of corner (di, dj, dk) (kernels/field_kernels.py:103-116).
+__device__ __forceinline__ long long deposit_weight(float fx, float fy, float fz, int di, int dj, int dk)
+{
+    const float wx = di == 0 ? 1.0f - fx : fx;
+    const float wy = dj == 0 ? 1.0f - fy : fy;
+    const float wz = dk == 0 ? 1.0f - fz : fz;
+    const float w = wx * wy * wz;
+    return (long long)(int32_t)(w * kFixedOne + 0.5f);
+}
+
 // deposit_trilinear for one cell (kernels/field_kernels.py:86-116): 8 int64 atomics of
 // int(w * 2^24 + 0.5) into the (4, nx, ny, nz) accumulator, layer = cell state.
 __device__ __forceinline__ void deposit_cell(const float* __restrict__ position, int64_t c, int32_t s, float ox,
                                              float oy, float oz, float inv_dx, int32_t nx, int32_t ny, int32_t nz,
                                              unsigned long long* __restrict__ acc)
 {
-    if (s < 0 || s >= 4) return;
-    float ux = (position[3 * c + 0] - ox) * inv_dx;
-    float uy = (position[3 * c + 1] - oy) * inv_dx;
-    float uz = (position[3 * c + 2] - oz) * inv_dx;
-    int32_t i0 = (int32_t)floorf(ux), j0 = (int32_t)floorf(uy), k0 = (int32_t)floorf(uz);
-    if (i0 < 0 || j0 < 0 || k0 < 0) return;
-    if (i0 + 1 >= nx || j0 + 1 >= ny || k0 + 1 >= nz) return;
-    float fx = ux - (float)i0, fy = uy - (float)j0, fz = uz - (float)k0;
+    int32_t i0, j0, k0;
+    float fx, fy, fz;
+    if (!deposit_support(position, c, s, ox, oy, oz, inv_dx, nx, ny, nz, i0, j0, k0, fx, fy, fz)) return;
     const int64_t plane = (int64_t)ny * nz;
     unsigned long long* base = acc + (int64_t)s * nx * plane + i0 * plane + (int64_t)j0 * nz + k0;
 #pragma unroll
-    for (int di = 0; di < 2; ++di) {
-        float wx = di == 0 ? 1.0f - fx : fx;
+    for (int di = 0; di < 2; ++di)
 #pragma unroll
-        for (int dj = 0; dj < 2; ++dj) {
-            float wy = dj == 0 ? 1.0f - fy : fy;
+        for (int dj = 0; dj < 2; ++dj)
 #pragma unroll
-            for (int dk = 0; dk < 2; ++dk) {
-                float wz = dk == 0 ? 1.0f - fz : fz;
-                float w = wx * wy * wz;
-                long long q = (long long)(int32_t)(w * kFixedOne + 0.5f);
-                atomicAdd(base + di * plane + dj * nz + dk, (unsigned long long)q);
-            }
-        }
-    }
+            for (int dk = 0; dk < 2; ++dk)
+                atomicAdd(base + di * plane + dj * nz + dk, (unsigned long long)deposit_weight(fx, fy, fz, di, dj, dk));
 }
 
 // fixed_point_to_density + combine_densities for node t of a grid of `nodes` nodes
