@@ -59,7 +59,7 @@ def setup(workload_name: str, device, grid_nodes: int | None = None):
 
 KERNEL_OF_KIND = {"tile": "sor_fused_kernel<{tj}, {ti}>", "march": "sor_march_kernel<{tj}, {ti}>",
                   "pipe": "sor_pipe_kernel<{tj}, {ti}>", "copy": "sor_copy_kernel<{tj}, {ti}>",
-                  "packed": "sor_packed_kernel<C, {tj}, 0>", "quad": "sor_quad_kernel<C, 0>"}
+                  "packed": "sor_packed_kernel<C, {tj}, 0>", "quad": "sor_quad_kernel<C, 0, {q}>"}
 COUNTER_TABLE = ROOT / "profiles" / "sor_counters_spheroid_100k_161.md"
 
 
@@ -122,7 +122,7 @@ def sor_roofline(ox, geom, device, workload: str, timed_sweeps: int = 1000) -> d
     return {"bound": "hbm", "achieved": round(achieved, 1), "peak": HBM_PEAK_GBPS, "unit": "GB/s",
             "frac": round(achieved / HBM_PEAK_GBPS, 4),
             "traffic": None if sweep_traffic is None else sweep_traffic // launches,
-            "kernel": KERNEL_OF_KIND[family].format(tj=tj.value, ti=ti.value), "shape": shape,
+            "kernel": KERNEL_OF_KIND[family].format(tj=tj.value, ti=ti.value, q=max(ti.value, 1)), "shape": shape,
             "launches_per_sweep": launches,
             "timed_sweeps": timed_sweeps, "sweep_us": round(per_sweep_s * 1e6, 3),
             "launch_us": round(per_sweep_s * 1e6 / launches, 3),

@@ -176,7 +176,8 @@ def test_colour_passes_and_residual_are_bit_exact(oracle, device, boundary, pinn
     assert np.float32(gpu_res) == np.float32(oracle.residual(ref, density, a, fixed=fx, production=production))
 
 
-@pytest.mark.parametrize("shape_name", ["tile60x8x8", "copy60x4x8", "packed/2", "packed/8", "packed/quad"])
+@pytest.mark.parametrize("shape_name", ["tile60x8x8", "copy60x4x8", "packed/2", "packed/8", "packed/quad",
+                                        "packed/quad2", "packed/quad4"])
 def test_solve_loop_matches_the_oracle_report_and_field(oracle, device, shape_name):
     """The whole chunked loop: same sweeps, same residual, same field bits (41^3 spheroid), with
     the solve loop's sweeps on a one-launch tile and on the colour-packed arrays."""
@@ -218,7 +219,8 @@ def _solve_loop_matches(oracle, device):
     assert err < 1e-4
 
 
-@pytest.mark.parametrize("shape_name", ["tile60x8x8", "packed/2", "packed/4", "packed/8", "packed/quad"])
+@pytest.mark.parametrize("shape_name", ["tile60x8x8", "packed/2", "packed/4", "packed/8", "packed/quad",
+                                        "packed/quad2", "packed/quad4"])
 def test_max_sweeps_and_pipelined_checks(oracle, device, shape_name):
     """Budget exhaustion mid-chunk, a last chunk shorter than the others, one sweep per check and
     convergence inside a queued-ahead chunk, on the one-launch tile and on the colour-packed loop

@@ -22,15 +22,15 @@ _cache: dict = {}
 
 def shape_name(kind: int, tj: int, ti: int) -> str:
     return {0: f"tile60x{tj}x{ti}", 1: f"march60x{tj}/{ti}", 2: f"pipe60x{tj}x{ti}", 3: f"copy60x{tj}x{ti}",
-            4: f"packed/{tj}", 5: "packed/quad"}[kind]
+            4: f"packed/{tj}", 5: "packed/quad" if ti <= 1 else f"packed/quad{ti}"}[kind]
 
 
 def config_shapes() -> list[str]:
     """Names of the shape table: 'tile60xTJxTI' (3-D tile), 'march60xTJ/CI' (marching i) or
     'pipe60xTJxTI' (3-D tile, pipelined persistent workgroups), 'copy60xTJxTI' (3-D tile,
     global->LDS copies), 'packed/RJ' (colour passes on colour-packed arrays, RJ rows per
-    thread) or 'packed/quad' (colour passes on colour-packed arrays, four packed indices of one row
-    per lane)."""
+    thread) or 'packed/quad', 'packed/quadQ' (colour passes on colour-packed arrays, four packed
+    indices of one row per lane, Q rounds of 64 lanes per wave)."""
     out = []
     for c in range(_lib.query("wbc_sor_fused_config_count")):
         kind, tj, ti = ctypes.c_int32(), ctypes.c_int32(), ctypes.c_int32()

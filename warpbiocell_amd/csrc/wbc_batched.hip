@@ -772,7 +772,7 @@ __global__ void __launch_bounds__(256) wide_quad_kernel(const WideRep* __restric
     uint32_t lane = (uint32_t)(t - g.off[k]) * 256u + threadIdx.x;
     const bool in_range = lane < (uint32_t)w.qg.total;
     if (!in_range) lane = (uint32_t)w.qg.total - 1;
-    quad_update<C, 0>(w.p, w.qg, lane, in_range, w.pf.f[C], nullptr, w.pf.f[C ^ 1], w.pf.d[C], done + r,
+    quad_update<C, 0>(w.p, w.qg, lane, in_range, w.pf.f[C], nullptr, w.pf.f[C ^ 1], w.pf.d[C], w.pf.box, done + r,
                       w.q_finite);
 }
 
@@ -795,10 +795,10 @@ __global__ void __launch_bounds__(256) wide_quad_residual_kernel(const WideRep* 
     uint32_t lane = (uint32_t)(local - colour * w.pass_blocks) * 256u + threadIdx.x;
     const bool in_range = lane < (uint32_t)w.qg.total;
     if (!in_range) lane = (uint32_t)w.qg.total - 1;
-    float m = colour ? quad_update<1, 1>(w.p, w.qg, lane, false, w.pf.f[1], nullptr, w.pf.f[0], w.pf.d[1], nullptr,
-                                        w.q_finite)
-                     : quad_update<0, 1>(w.p, w.qg, lane, false, w.pf.f[0], nullptr, w.pf.f[1], w.pf.d[0], nullptr,
-                                        w.q_finite);
+    float m = colour ? quad_update<1, 1>(w.p, w.qg, lane, false, w.pf.f[1], nullptr, w.pf.f[0], w.pf.d[1], w.pf.box,
+                                        nullptr, w.q_finite)
+                     : quad_update<0, 1>(w.p, w.qg, lane, false, w.pf.f[0], nullptr, w.pf.f[1], w.pf.d[0], w.pf.box,
+                                        nullptr, w.q_finite);
     if (!in_range) m = 0.0f;  // a duplicate's wave neighbours are not the last lane's
     m = wave_max(m);
     if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;

@@ -209,4 +209,16 @@ __device__ __forceinline__ float wave_max(float v)
     return v;
 }
 
+__device__ __forceinline__ int wave_max_i32(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = max(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
+__device__ __forceinline__ int wave_min_i32(int v)
+{
+    for (int off = 32; off > 0; off >>= 1) v = min(v, __shfl_xor(v, off, 64));
+    return v;
+}
+
 }  // namespace wbc

@@ -170,6 +170,12 @@ __global__ void __launch_bounds__(256) zero_i32_kernel(int32_t* __restrict__ p, 
     if (t < count) p[t] = 0;
 }
 
+// The solve loop's device state: [0] convergence latch, [1] converged chunk, [2, 4) unused
+// (zeroed by each solve), [kSolveBoxAt, + kDensityBoxInts) the density box of the colour-packed
+// path (written by its pack).
+constexpr int kSolveBoxAt = 8;
+constexpr int kSolveStateInts = kSolveBoxAt + kDensityBoxInts;
+
 hipError_t launch_sor_color(const SorParams& p, int32_t color, float* field, const int32_t* done, hipStream_t s)
 {
     dim3 block(64, 4, 1);
@@ -408,12 +414,12 @@ hipError_t get_events(hipEvent_t** out)
 
 namespace {
 struct SolveLayout {
-    size_t state, slots, partial, scratch, packed, packed_each, total;
+    size_t state, slots, partial, scratch, box_part, packed, packed_each, total;
 };
 
 // Residual partials of the residual kernel or of the checked colour passes, a second field
-// buffer for the ping-pong sweeps, and five colour-packed arrays for the packed sweeps: field and
-// density per colour and a second red field.
+// buffer for the ping-pong sweeps, the pack's partial density boxes, and five colour-packed arrays
+// for the packed sweeps: field and density per colour and a second red field.
 SolveLayout solve_layout(int32_t chunks, const SorParams& p)
 {
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
@@ -423,10 +429,11 @@ SolveLayout solve_layout(int32_t chunks, const SorParams& p)
     const int64_t nodes = (int64_t)p.nx * p.ny * p.nz;
     SolveLayout L;
     size_t off = 0;
-    L.state = off;   off += up(4 * sizeof(int32_t));
+    L.state = off;   off += up(kSolveStateInts * sizeof(int32_t));
     L.slots = off;   off += up((size_t)chunks * sizeof(float));
     L.partial = off; off += up((size_t)partials * sizeof(float));
     L.scratch = off; off += up((size_t)nodes * sizeof(float));
+    L.box_part = off; off += up((size_t)kBoxPartInts * sizeof(int32_t));
     L.packed_each = up((size_t)packed_floats(p) * sizeof(float));
     L.packed = off;  off += 5 * L.packed_each;
     L.total = off;
@@ -490,17 +497,19 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
     // c stores the black terms; the red pass that starts chunk c + 1 stores the red terms of the
     // red values it reads and writes its result to the other red array, so red[c & 1] keeps the
     // field chunk c's check describes (the result, if that check converges).
-    const int32_t pass_blocks = packed ? packed_pass_blocks(p, packed_rows) : 0;
+    const int quads = packed_quads_of(g_fused_config);
+    const int32_t pass_blocks = packed ? packed_pass_blocks(p, packed_rows, quads) : 0;
     const int32_t partials = packed ? 2 * pass_blocks : residual_partials(p);
     PackedField pf{{(float*)(ws + L.packed), (float*)(ws + L.packed + L.packed_each)},
                    {(float*)(ws + L.packed + 2 * L.packed_each), (float*)(ws + L.packed + 3 * L.packed_each)},
-                   packed_pitch(nz, packed_rows)};
+                   packed_pitch(nz, packed_rows), state + kSolveBoxAt, (int32_t*)(ws + L.box_part)};
+    const int32_t* box = pf.box;
     float* red[2] = {pf.f[0], (float*)(ws + L.packed + 4 * L.packed_each)};
     hipEvent_t* ev;
     WBC_TRY(get_events(&ev));
     float *slots_host, *slots_mapped;
     WBC_TRY(get_host_slots(chunks, &slots_host, &slots_mapped));
-    zero_i32_kernel<<<1, 256, 0, stream>>>(state, 4);
+    zero_i32_kernel<<<1, 256, 0, stream>>>(state, kSolveBoxAt);
     WBC_CHECK_LAUNCH();
     if (packed) {
         WBC_TRY(launch_pack(p, field, density, pf, stream, red[1]));  // both red arrays: their face values
@@ -521,8 +530,8 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
     // The red pass that opens chunk q (q > 0, or q == chunks after the last chunk) with chunk q-1's
     // red check, then that check.
     auto red_check = [&](int32_t q) -> hipError_t {
-        WBC_TRY_ERR(launch_packed_colour(p, 0, packed_rows, 1, red[(q - 1) & 1], red[q & 1], pf.f[1], pf.d[0], state,
-                                         partial + pass_blocks, stream));
+        WBC_TRY_ERR(launch_packed_colour(p, 0, packed_rows, 1, red[(q - 1) & 1], red[q & 1], pf.f[1], pf.d[0], box,
+                                         state, partial + pass_blocks, stream, quads));
         return queue_check(q - 1);
     };
     // Sweep qs, with the check of its chunk when it ends one (colour-packed: the black terms now,
@@ -535,10 +544,10 @@ WBC_API int wbc_solve_steady_state(float omega, int32_t max_sweeps, float tolera
             if (s == 0 && c > 0)
                 WBC_TRY_ERR(red_check(c));
             else
-                WBC_TRY_ERR(launch_packed_colour(p, 0, packed_rows, 0, r, r, pf.f[1], pf.d[0], state, nullptr,
-                                                 stream));
-            WBC_TRY_ERR(launch_packed_colour(p, 1, packed_rows, last ? 2 : 0, pf.f[1], pf.f[1], r, pf.d[1], state,
-                                             last ? partial : nullptr, stream));
+                WBC_TRY_ERR(launch_packed_colour(p, 0, packed_rows, 0, r, r, pf.f[1], pf.d[0], box, state, nullptr,
+                                                 stream, quads));
+            WBC_TRY_ERR(launch_packed_colour(p, 1, packed_rows, last ? 2 : 0, pf.f[1], pf.f[1], r, pf.d[1], box,
+                                             state, last ? partial : nullptr, stream, quads));
         } else if (fused) {
             WBC_TRY_ERR(launch_sor_fused(p, buf[cur], buf[cur ^ 1], state, stream));
             cur ^= 1;

@@ -65,11 +65,23 @@ hipError_t launch_copy_f32(const float* a, float* b, int64_t n, hipStream_t s);
 // 16-byte load per array) the same rounded up to a multiple of 4. kPackedPadRows rows of padding
 // follow the last plane; the passes read (and never use) them past the last interior row.
 // Dirichlet faces, no pinned nodes, no production only.
+//
+// `box` (optional, kDensityBoxInts int32 on the device): the quad lanes' coordinates that hold every
+// node of nonzero density, {plane lo, plane hi, row lo, row hi, group lo, group hi} with plane
+// i - 1, row j - 1 and group k / 8 (the quad lane of packed index k / 2 in either colour), empty
+// when lo > hi. launch_pack writes it from the density it packs, through `box_part`
+// (kBoxPartInts int32 of scratch, given with the box); the quad passes given it skip the density
+// load of a lane outside it.
 struct PackedField {
     float* f[2];
     float* d[2];
     int32_t pitch;
+    int32_t* box;
+    int32_t* box_part;
 };
+constexpr int kDensityBoxInts = 6;
+constexpr int kPackBlocks = 2048;  // the most workgroups of a pack
+constexpr int kBoxPartInts = kPackBlocks * kDensityBoxInts;
 constexpr int kPackedPadRows = 8;
 // `rows` value that selects the quad pass in the functions below.
 constexpr int kQuadRows = 1;
@@ -94,23 +106,28 @@ hipError_t launch_pack(const SorParams& p, const float* field, const float* dens
 hipError_t launch_unpack(const SorParams& p, const PackedField& pf, float* field, hipStream_t s);
 // One sweep (both colour passes); each thread takes `rows` (2, 4 or 8) consecutive rows j, or with
 // kQuadRows four packed indices of one row (pf.pitch must be packed_pitch(nz, rows)).
-hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, const int32_t* done, hipStream_t s);
-// Workgroups of one colour pass at `rows`.
-int packed_pass_blocks(const SorParams& p, int rows);
+// `quads` (quad pass only): rounds of 64 lanes per wave, 1, 2 or 4.
+hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, const int32_t* done, hipStream_t s,
+                             int quads = 1);
+// Workgroups of one colour pass at `rows` (and `quads` for the quad pass).
+int packed_pass_blocks(const SorParams& p, int rows, int quads = 1);
 // One colour pass of the solve loop's colour-packed path with the residual check folded in.
 // check 0: in place in `own`. check 1: the pass also stores the residual terms of the values it
 // reads (one maximum per workgroup in partial[0, packed_pass_blocks)) and writes its new values
 // to `out`, leaving `own` as the field the check describes. check 2: in place, storing the terms
 // of the values it writes.
+// `box`: the density box of the pack (PackedField::box) or nullptr; only the quad pass reads it.
 hipError_t launch_packed_colour(const SorParams& p, int colour, int rows, int check, float* own, float* out,
-                                const float* other, const float* dens, const int32_t* done, float* partial,
-                                hipStream_t s);
+                                const float* other, const float* dens, const int32_t* box, const int32_t* done,
+                                float* partial, hipStream_t s, int quads = 1);
 
 // The sweep shape table (wbc_sor_fused.hip): the entry the solve loop uses, and the colour-packed
 // entries (which the solve loop runs on packed arrays for its whole duration): rows per thread
 // (kQuadRows for the quad pass), or 0 for an entry that is not colour-packed.
 extern int g_fused_config;
 int packed_rows_of(int cfg);
+// Rounds of quad lanes per wave of a quad-pass entry (1 for any other entry).
+int packed_quads_of(int cfg);
 
 // 2-D-tiled variant that marches along i (wbc_sor_march.hip); cfg indexes its own shape table.
 constexpr int kMarchConfigCount = 6;

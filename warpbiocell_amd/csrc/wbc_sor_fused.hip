@@ -205,8 +205,9 @@ __global__ void __launch_bounds__(256) copy_f32_kernel(const float* __restrict__
 // kind 2: 3-D tile 60 x TJ x TI, pipelined persistent workgroups (this file); kind 3: 3-D tile
 // 60 x TJ x TI with global->LDS copies (this file); kind 4: two colour passes on colour-packed
 // arrays, TJ rows per thread (wbc_sor_packed.hip); kind 5: two colour passes on colour-packed
-// arrays, four packed indices of one row per lane (the quad pass, wbc_sor_packed.hpp). Kinds 4
-// and 5 run only in the multi-sweep entry point and the solve loop.
+// arrays, four packed indices of one row per lane (the quad pass, wbc_sor_packed.hpp), TI rounds
+// of 64 lanes per wave (0 and 1: one). Kinds 4 and 5 run only in the multi-sweep entry point and
+// the solve loop.
 struct FusedConfig {
     int kind, tj, ti;
 };
@@ -215,7 +216,7 @@ constexpr FusedConfig kFusedConfigs[] = {{0, 4, 4}, {0, 4, 8}, {0, 8, 4}, {0, 8,
                                          {1, 28, 8}, {1, 28, 16}, {1, 12, 4}, {2, 4, 4}, {2, 4, 8},
                                          {2, 8, 4}, {2, 8, 8}, {3, 4, 4}, {3, 4, 8}, {3, 8, 4}, {3, 8, 8},
                                          {3, 2, 8}, {3, 8, 2}, {4, 2, 0}, {4, 4, 0}, {4, 8, 0},
-                                         {5, 4, 0}};
+                                         {5, 4, 0}, {5, 4, 2}, {5, 4, 4}};
 constexpr int kFusedConfigCount = sizeof(kFusedConfigs) / sizeof(kFusedConfigs[0]);
 
 int packed_rows_of(int cfg)
@@ -223,6 +224,11 @@ int packed_rows_of(int cfg)
     if (cfg < 0 || cfg >= kFusedConfigCount) return 0;
     if (kFusedConfigs[cfg].kind == 5) return kQuadRows;
     return kFusedConfigs[cfg].kind == 4 ? kFusedConfigs[cfg].tj : 0;
+}
+int packed_quads_of(int cfg)
+{
+    if (cfg < 0 || cfg >= kFusedConfigCount || kFusedConfigs[cfg].kind != 5) return 1;
+    return kFusedConfigs[cfg].ti > 1 ? kFusedConfigs[cfg].ti : 1;
 }
 int g_fused_config = 3;
 constexpr int kResidualIStride = 8;
@@ -407,8 +413,10 @@ WBC_API int wbc_sor_sweep_fused(float omega, float dx2_over_D, float uptake_max,
 
 // ---- several sweeps with one shape ----------------------------------------------------------
 namespace {
+// A second field for the tile shapes' ping-pong; the density box and the pack's partial boxes, and
+// four colour-packed arrays for the colour-packed shapes.
 struct SweepsLayout {
-    size_t tmp, packed, total;
+    size_t tmp, box, box_part, packed, total;
 };
 SweepsLayout sweeps_layout(int32_t nx, int32_t ny, int32_t nz)
 {
@@ -416,7 +424,9 @@ SweepsLayout sweeps_layout(int32_t nx, int32_t ny, int32_t nz)
     SweepsLayout L;
     const size_t nodes = (size_t)nx * ny * nz, half = (size_t)packed_floats_of(nx, ny, nz);
     L.tmp = 0;
-    L.packed = up(nodes * sizeof(float));
+    L.box = up(nodes * sizeof(float));
+    L.box_part = L.box + up(kDensityBoxInts * sizeof(int32_t));
+    L.packed = L.box_part + up(kBoxPartInts * sizeof(int32_t));
     L.total = L.packed + 4 * up(half * sizeof(float));
     return L;
 }
@@ -430,7 +440,7 @@ WBC_API size_t wbc_sor_sweeps_workspace_bytes(int32_t nx, int32_t ny, int32_t nz
 
 // `sweeps` sweeps with shape `cfg` (-1: the selected one) from src into dst (distinct buffers;
 // src unchanged). Tile shapes ping-pong through the workspace; the colour-packed entry packs src
-// once, runs its passes and unpacks into dst.
+// once (with the density box), runs its passes and unpacks into dst.
 WBC_API int wbc_sor_sweeps_cfg(int32_t cfg, int32_t sweeps, float omega, float dx2_over_D, float uptake_max,
                                float michaelis_k, const int32_t* fixed, const float* density, const float* production,
                                int32_t has_source, const float* source, float source_k, const float* src, float* dst,
@@ -452,9 +462,10 @@ WBC_API int wbc_sor_sweeps_cfg(int32_t cfg, int32_t sweeps, float omega, float d
         const size_t bytes = (L.total - L.packed) / 4;
         PackedField pf{{(float*)(ws + L.packed), (float*)(ws + L.packed + bytes)},
                        {(float*)(ws + L.packed + 2 * bytes), (float*)(ws + L.packed + 3 * bytes)},
-                       packed_pitch(nz, rows)};
+                       packed_pitch(nz, rows), (int32_t*)(ws + L.box), (int32_t*)(ws + L.box_part)};
         WBC_TRY(launch_pack(p, src, density, pf, stream));
-        for (int32_t s = 0; s < sweeps; ++s) WBC_TRY(launch_sor_packed(p, pf, rows, nullptr, stream));
+        const int quads = packed_quads_of(cfg);
+        for (int32_t s = 0; s < sweeps; ++s) WBC_TRY(launch_sor_packed(p, pf, rows, nullptr, stream, quads));
         return (int)launch_unpack(p, pf, dst, stream);
     }
     float* tmp = (float*)(ws + L.tmp);

@@ -22,26 +22,84 @@
 
 namespace wbc {
 
-// f0_copy (optional): a second copy of the colour-0 field.
+// f0_copy (optional): a second copy of the colour-0 field. box_part (optional): each workgroup
+// writes there, at kDensityBoxInts * blockIdx.x, the box of the quad-lane coordinates (i - 1,
+// j - 1, k / 8) of the nodes it packed whose density is not +-0 (lo > hi when none); box_reduce
+// merges them into PackedField::box. Plain stores and one reduction, not same-address atomics:
+// those serialise across the XCDs at ~25 ns each, ~0.25 ms for a 161^3 pack.
 __global__ void __launch_bounds__(256) pack_kernel(const float* __restrict__ field, const float* __restrict__ density,
                                                    float* __restrict__ f0, float* __restrict__ f1,
                                                    float* __restrict__ d0, float* __restrict__ d1,
-                                                   float* __restrict__ f0_copy, int nx, int ny, int nz, int pitch)
+                                                   float* __restrict__ f0_copy, int32_t* __restrict__ box_part, int nx,
+                                                   int ny, int nz, int pitch)
 {
+    __shared__ int32_t part_lds[4][kDensityBoxInts];
     const int64_t nodes = (int64_t)nx * ny * nz;
+    int b[kDensityBoxInts] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nodes; t += (int64_t)gridDim.x * 256) {
         const int k = (int)(t % nz);
         const int64_t ij = t / nz;  // i * ny + j
         const int j = (int)(ij % ny), i = (int)(ij / ny);
         const int64_t e = ij * pitch + (k >> 1);
+        const float dv = density[t];
         if (((i + j + k) & 1) == 0) {
             f0[e] = field[t];
-            d0[e] = density[t];
+            d0[e] = dv;
             if (f0_copy) f0_copy[e] = field[t];
         } else {
             f1[e] = field[t];
-            d1[e] = density[t];
+            d1[e] = dv;
         }
+        if (dv != 0.0f) {  // NaN included
+            const int c[3] = {i - 1, j - 1, k >> 3};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                b[2 * a] = min(b[2 * a], c[a]);
+                b[2 * a + 1] = max(b[2 * a + 1], c[a]);
+            }
+        }
+    }
+    if (!box_part) return;
+#pragma unroll
+    for (int a = 0; a < kDensityBoxInts; ++a) b[a] = (a & 1) ? wave_max_i32(b[a]) : wave_min_i32(b[a]);
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int a = 0; a < kDensityBoxInts; ++a) part_lds[threadIdx.x / kWave][a] = b[a];
+    }
+    __syncthreads();
+    if (threadIdx.x < kDensityBoxInts) {
+        const int a = (int)threadIdx.x;
+        int v = part_lds[0][a];
+        for (int w = 1; w < 4; ++w) v = (a & 1) ? max(v, part_lds[w][a]) : min(v, part_lds[w][a]);
+        box_part[(int64_t)blockIdx.x * kDensityBoxInts + a] = v;
+    }
+}
+
+// box = the union of the `blocks` partial boxes of pack_kernel (one workgroup of 1024 threads).
+__global__ void __launch_bounds__(1024) box_reduce_kernel(const int32_t* __restrict__ box_part, int blocks,
+                                                          int32_t* __restrict__ box)
+{
+    __shared__ int32_t part_lds[16][kDensityBoxInts];
+    int b[kDensityBoxInts] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
+    for (int k = (int)threadIdx.x; k < blocks; k += 1024) {
+#pragma unroll
+        for (int a = 0; a < kDensityBoxInts; ++a) {
+            const int v = box_part[(int64_t)k * kDensityBoxInts + a];
+            b[a] = (a & 1) ? max(b[a], v) : min(b[a], v);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < kDensityBoxInts; ++a) b[a] = (a & 1) ? wave_max_i32(b[a]) : wave_min_i32(b[a]);
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int a = 0; a < kDensityBoxInts; ++a) part_lds[threadIdx.x / kWave][a] = b[a];
+    }
+    __syncthreads();
+    if (threadIdx.x < kDensityBoxInts) {
+        const int a = (int)threadIdx.x;
+        int v = part_lds[0][a];
+        for (int w = 1; w < 16; ++w) v = (a & 1) ? max(v, part_lds[w][a]) : min(v, part_lds[w][a]);
+        box[a] = v;
     }
 }
 
@@ -95,22 +153,42 @@ __global__ void __launch_bounds__(256) sor_packed_kernel(SorParams p, float* __r
 }
 
 // One colour pass over the interior nodes of colour C with four packed indices per lane (the quad
-// pass, wbc_sor_packed.hpp); kRes as in sor_packed_kernel.
-template <int C, int kRes>
+// pass, wbc_sor_packed.hpp); kRes as in sor_packed_kernel. Each wave takes Q rounds of 64
+// consecutive quad lanes (lane l of round r: lane 64 (Q wave + r) + l of the pass) and issues the
+// loads of all Q rounds before it updates any, so a pass has Q times fewer waves to launch for the
+// same loads in flight.
+template <int C, int kRes, int Q>
 __global__ void __launch_bounds__(256) sor_quad_kernel(SorParams p, QuadGrid g, float* __restrict__ own,
                                                        float* __restrict__ out, const float* __restrict__ other,
-                                                       const float* __restrict__ dens, const int32_t* __restrict__ done,
-                                                       int q_finite, float* __restrict__ partial)
+                                                       const float* __restrict__ dens, const int32_t* __restrict__ box,
+                                                       const int32_t* __restrict__ done, int q_finite,
+                                                       float* __restrict__ partial)
 {
     __shared__ float wave_max_lds[4];
-    uint32_t t = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * 256u + threadIdx.x;
-    const bool in_range = t < (uint32_t)g.total;
-    if (!in_range) t = (uint32_t)g.total - 1;  // a duplicate of the last lane: loads, no store
-    float m = quad_update<C, kRes>(p, g, t, in_range, own, out, other, dens, done, q_finite);
-    if (m < 0.0f) return;  // the solve has converged (the same for every lane)
-    // A duplicate's wave neighbours are other duplicates, not the last lane's, so its outer
-    // k-neighbours (and the residual terms built on them) are not that lane's: it adds nothing.
-    if (!in_range) m = 0.0f;
+    int32_t b[kDensityBoxInts];
+    quad_box(b, box);
+    const int32_t stop = done ? *done : 0;
+    const uint32_t wave = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * 4u + threadIdx.x / kWave;
+    const uint32_t first = wave * (uint32_t)(kWave * Q) + (uint32_t)__lane_id();
+    QuadLoad L[Q];
+    bool in_range[Q];
+#pragma unroll
+    for (int r = 0; r < Q; ++r) {
+        uint32_t t = first + (uint32_t)(r * kWave);
+        in_range[r] = t < (uint32_t)g.total;
+        if (!in_range[r]) t = (uint32_t)g.total - 1;  // a duplicate of the last lane: loads, no store
+        quad_load<C>(L[r], g, t, own, other, dens, b);
+    }
+    if (stop) return;  // the solve has converged (the same for every lane)
+    float* dst = kRes == 1 ? out : own;
+    float m = 0.0f;
+#pragma unroll
+    for (int r = 0; r < Q; ++r) {
+        const float mr = quad_finish<kRes>(L[r], p, in_range[r], dst, q_finite);
+        // A duplicate's wave neighbours are other duplicates, not the last lane's, so its outer
+        // k-neighbours (and the residual terms built on them) are not that lane's: it adds nothing.
+        if (in_range[r]) m = fmaxf(m, mr);
+    }
     if (kRes) {
         m = wave_max(m);
         if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;
@@ -144,8 +222,12 @@ hipError_t launch_pack(const SorParams& p, const float* field, const float* dens
 {
     const int64_t nodes = (int64_t)p.nx * p.ny * p.nz;
     if (pf.pitch < (p.nz + 1) / 2 || pf.pitch > packed_pitch(p.nz, kQuadRows)) return hipErrorInvalidValue;
-    pack_kernel<<<grid_for(nodes < 2048 * 256 ? nodes : 2048 * 256, 256), 256, 0, s>>>(
-        field, density, pf.f[0], pf.f[1], pf.d[0], pf.d[1], f0_copy, p.nx, p.ny, p.nz, pf.pitch);
+    if ((pf.box == nullptr) != (pf.box_part == nullptr)) return hipErrorInvalidValue;
+    const int blocks = (int)grid_for(nodes < kPackBlocks * 256 ? nodes : kPackBlocks * 256, 256);
+    pack_kernel<<<blocks, 256, 0, s>>>(field, density, pf.f[0], pf.f[1], pf.d[0], pf.d[1], f0_copy, pf.box_part,
+                                       p.nx, p.ny, p.nz, pf.pitch);
+    WBC_TRY_ERR(hipGetLastError());
+    if (pf.box) box_reduce_kernel<<<1, 1024, 0, s>>>(pf.box_part, blocks, pf.box);
     return hipGetLastError();
 }
 
@@ -158,11 +240,11 @@ hipError_t launch_unpack(const SorParams& p, const PackedField& pf, float* field
     return hipGetLastError();
 }
 
-int packed_pass_blocks(const SorParams& p, int rows)
+int packed_pass_blocks(const SorParams& p, int rows, int quads)
 {
     if (rows == kQuadRows) {
         QuadGrid g;
-        return quad_grid_of(p, g) ? (int)grid_for(g.total, 256) : 0;
+        return quad_grid_of(p, g) ? (int)grid_for(g.total, 256 * quads) : 0;
     }
     const int64_t lanes = packed_lanes_per_plane(p, rows);
     return lanes < 0 ? 0 : (int)((lanes + 255) / 256) * (p.nx - 2);
@@ -181,15 +263,29 @@ static hipError_t launch_pass(const SorParams& p, float* own, float* out, const 
     return hipGetLastError();
 }
 
-template <int C, int kRes>
-static hipError_t launch_quad(const SorParams& p, float* own, float* out, const float* other, const float* dens,
-                              const int32_t* done, float* partial, hipStream_t s)
+template <int C, int kRes, int Q>
+static hipError_t launch_quad_q(const SorParams& p, float* own, float* out, const float* other, const float* dens,
+                                const int32_t* box, const int32_t* done, float* partial, hipStream_t s)
 {
     QuadGrid g;
     if (!quad_grid_of(p, g)) return hipErrorInvalidValue;
     const int qf = __builtin_isfinite(p.uptake_max) ? 1 : 0;
-    sor_quad_kernel<C, kRes><<<grid_for(g.total, 256), 256, 0, s>>>(p, g, own, out, other, dens, done, qf, partial);
+    sor_quad_kernel<C, kRes, Q>
+        <<<grid_for(g.total, 256 * Q), 256, 0, s>>>(p, g, own, out, other, dens, box, done, qf, partial);
     return hipGetLastError();
+}
+
+// The quad pass with `quads` (1, 2 or 4) rounds of lanes per wave.
+template <int C, int kRes>
+static hipError_t launch_quad(int quads, const SorParams& p, float* own, float* out, const float* other,
+                              const float* dens, const int32_t* box, const int32_t* done, float* partial, hipStream_t s)
+{
+    switch (quads) {
+    case 1: return launch_quad_q<C, kRes, 1>(p, own, out, other, dens, box, done, partial, s);
+    case 2: return launch_quad_q<C, kRes, 2>(p, own, out, other, dens, box, done, partial, s);
+    case 4: return launch_quad_q<C, kRes, 4>(p, own, out, other, dens, box, done, partial, s);
+    default: return hipErrorInvalidValue;
+    }
 }
 
 template <int RJ>
@@ -199,12 +295,13 @@ static hipError_t launch_packed_rows(const SorParams& p, PackedField& pf, const 
     return launch_pass<1, RJ, 0>(p, pf.f[1], nullptr, pf.f[0], pf.d[1], done, nullptr, s);
 }
 
-hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, const int32_t* done, hipStream_t s)
+hipError_t launch_sor_packed(const SorParams& p, PackedField& pf, int rows, const int32_t* done, hipStream_t s,
+                             int quads)
 {
     if (pf.pitch != packed_pitch(p.nz, rows)) return hipErrorInvalidValue;
     if (rows == kQuadRows) {
-        WBC_TRY_ERR((launch_quad<0, 0>(p, pf.f[0], nullptr, pf.f[1], pf.d[0], done, nullptr, s)));
-        return launch_quad<1, 0>(p, pf.f[1], nullptr, pf.f[0], pf.d[1], done, nullptr, s);
+        WBC_TRY_ERR((launch_quad<0, 0>(quads, p, pf.f[0], nullptr, pf.f[1], pf.d[0], pf.box, done, nullptr, s)));
+        return launch_quad<1, 0>(quads, p, pf.f[1], nullptr, pf.f[0], pf.d[1], pf.box, done, nullptr, s);
     }
     switch (rows) {
     case 2: return launch_packed_rows<2>(p, pf, done, s);
@@ -229,18 +326,19 @@ static hipError_t launch_colour_rows(const SorParams& p, int colour, int check, 
 }
 
 hipError_t launch_packed_colour(const SorParams& p, int colour, int rows, int check, float* own, float* out,
-                                const float* other, const float* dens, const int32_t* done, float* partial,
-                                hipStream_t s)
+                                const float* other, const float* dens, const int32_t* box, const int32_t* done,
+                                float* partial, hipStream_t s, int quads)
 {
     if (check != 0 && !partial) return hipErrorInvalidValue;
     if (check == 1 && (!out || out == own)) return hipErrorInvalidValue;
     if (rows == kQuadRows) {
         // The loop checks red values as the red pass reads them and black values as the black pass
         // writes them, as for the row passes.
-        if (colour == 0 && check == 0) return launch_quad<0, 0>(p, own, nullptr, other, dens, done, nullptr, s);
-        if (colour == 0 && check == 1) return launch_quad<0, 1>(p, own, out, other, dens, done, partial, s);
-        if (colour == 1 && check == 0) return launch_quad<1, 0>(p, own, nullptr, other, dens, done, nullptr, s);
-        if (colour == 1 && check == 2) return launch_quad<1, 2>(p, own, nullptr, other, dens, done, partial, s);
+        const int q = quads;
+        if (colour == 0 && check == 0) return launch_quad<0, 0>(q, p, own, nullptr, other, dens, box, done, nullptr, s);
+        if (colour == 0 && check == 1) return launch_quad<0, 1>(q, p, own, out, other, dens, box, done, partial, s);
+        if (colour == 1 && check == 0) return launch_quad<1, 0>(q, p, own, nullptr, other, dens, box, done, nullptr, s);
+        if (colour == 1 && check == 2) return launch_quad<1, 2>(q, p, own, nullptr, other, dens, box, done, partial, s);
         return hipErrorInvalidValue;
     }
     switch (rows) {

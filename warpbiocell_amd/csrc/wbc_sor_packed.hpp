@@ -219,71 +219,128 @@ __device__ __forceinline__ float lane_from_right(float v, float lane63)  // lane
     return __int_as_float(__builtin_amdgcn_update_dpp(__float_as_int(lane63), __float_as_int(v), 0x130, 0xf, 0xf, false));
 }
 
-// One lane of the quad pass of colour C (kernels/field_kernels.py:185-212, same expressions and
-// order: neighbour sum i-1, i+1, j-1, j+1, k-1, k+1). Writes the four values (unchanged where a
-// slot is not an interior node) to `out` (kRes == 1) or in place, and returns the maximum
-// residual term as packed_update does for kRes 1 and 2. When the solve's convergence latch
-// `done` is set the lane does nothing and returns -1; the latch is read after the lane's loads
-// are issued, so its latency overlaps theirs (0.1-0.2 us per 161^3 sweep).
-template <int C, int kRes>
-__device__ __forceinline__ float quad_update(const SorParams& p, const QuadGrid& g, uint32_t t, bool store,
-                                             const float* __restrict__ own, float* __restrict__ out,
-                                             const float* __restrict__ other, const float* __restrict__ dens,
-                                             const int32_t* __restrict__ done, int q_finite)
+// What one lane of a quad pass of colour C loads for plane i = pl + 1, row j = row + 1 and packed
+// indices kk0 = 4 q .. 4 q + 3 (the lane geometry above): the other colour's rows j-1, j, j+1 of
+// plane i and row j of planes i-1 and i+1, its own field and density, and the wave-edge and
+// row-end k-neighbours. With a density box (see PackedField) a lane outside it takes its four
+// densities as +0 without loading them: they are +0 or -0 there, and both give the same value in
+// every expression of quad_finish (c is +-0 where K + o > 0 and uptake_max is finite, NaN
+// otherwise, either way). b: the box, or {0, INT32_MAX, ...} for none.
+struct QuadLoad {
+    wbc_f4 jm, jc, jp, im, ip, o, d;
+    float edge_left, edge_right;
+    uint32_t w;  // byte offset of the lane's four slots in every packed array
+    int par, k0;
+    bool load_right;
+};
+
+__device__ __forceinline__ void quad_box(int32_t (&b)[kDensityBoxInts], const int32_t* __restrict__ box)
 {
-    const int32_t stop = done ? *done : 0;
+#pragma unroll
+    for (int e = 0; e < kDensityBoxInts; ++e) b[e] = (e & 1) ? INT32_MAX : 0;
+    if (box) {
+#pragma unroll
+        for (int e = 0; e < kDensityBoxInts; ++e) b[e] = box[e];
+    }
+}
+
+template <int C>
+__device__ __forceinline__ void quad_load(QuadLoad& L, const QuadGrid& g, uint32_t t, const float* __restrict__ own,
+                                          const float* __restrict__ other, const float* __restrict__ dens,
+                                          const int32_t (&b)[kDensityBoxInts])
+{
     const uint32_t pl = fast_div(t, g.plane);
     const uint32_t u = t - pl * g.plane.d;
     const uint32_t row = fast_div(u, g.row);
     const uint32_t q = u - row * g.row.d;
     const int i = (int)pl + 1, j = (int)row + 1;
     const int kk0 = 4 * (int)q;
-    const int par = (C + i + j) & 1;
+    L.par = (C + i + j) & 1;
+    L.k0 = 2 * kk0 + L.par;
     const uint32_t w = (uint32_t)i * g.plane_bytes + (uint32_t)j * g.row_bytes + (uint32_t)kk0 * 4u;
+    L.w = w;
     const char* ob = (const char*)other;
     auto at = [](const void* base, uint32_t off) { return *(const wbc_f4*)((const char*)base + off); };
-    const wbc_f4 jm = at(ob, w - g.row_bytes), jc = at(ob, w), jp = at(ob, w + g.row_bytes);
-    const wbc_f4 im = at(ob, w - g.plane_bytes), ip = at(ob, w + g.plane_bytes);
-    const wbc_f4 o = at(own, w), d = at(dens, w);
+    L.jm = at(ob, w - g.row_bytes);
+    L.jc = at(ob, w);
+    L.jp = at(ob, w + g.row_bytes);
+    L.im = at(ob, w - g.plane_bytes);
+    L.ip = at(ob, w + g.plane_bytes);
+    L.o = at(own, w);
+    L.d = wbc_f4{0.0f, 0.0f, 0.0f, 0.0f};
+    if ((int)pl >= b[0] && (int)pl <= b[1] && (int)row >= b[2] && (int)row <= b[3] && (int)q >= b[4] && (int)q <= b[5])
+        L.d = at(dens, w);
     const int lane = (int)__lane_id();
     // Lane 0 has no left neighbour in the wave; lane 63 has no right one, and the row's last
     // group has no lane to its right in the row: they load the value.
-    const bool load_right = lane == kWave - 1 || q == g.row.d - 1;
-    float edge_left = 0.0f, edge_right = 0.0f;
-    if (lane == 0) edge_left = *(const float*)(ob + w - 4);
-    if (load_right) edge_right = *(const float*)(ob + w + 16);
-    if (stop) return -1.0f;
-    const float from_right = lane_from_right(jc.x, edge_right);
-    const float ext[6] = {lane_from_left(jc.w, edge_left), jc.x, jc.y, jc.z, jc.w, load_right ? edge_right : from_right};
-    const int k0 = 2 * kk0 + par;
+    L.load_right = lane == kWave - 1 || q == g.row.d - 1;
+    L.edge_left = 0.0f;
+    L.edge_right = 0.0f;
+    if (lane == 0) L.edge_left = *(const float*)(ob + w - 4);
+    if (L.load_right) L.edge_right = *(const float*)(ob + w + 16);
+}
+
+// The update of one quad lane from its loads (kernels/field_kernels.py:185-212, same expressions
+// and order: neighbour sum i-1, i+1, j-1, j+1, k-1, k+1). Stores the four values (unchanged where
+// a slot is not an interior node) at the lane's slots of `dst` when `store`, and returns the
+// maximum residual term: kRes 1 of the values before the update, kRes 2 of the values written
+// (as packed_update). Every lane of the wave must take part (the k-neighbours cross lanes).
+template <int kRes>
+__device__ __forceinline__ float quad_finish(const QuadLoad& L, const SorParams& p, bool store, float* __restrict__ dst,
+                                             int q_finite)
+{
+    const float from_right = lane_from_right(L.jc.x, L.edge_right);
+    const float ext[6] = {lane_from_left(L.jc.w, L.edge_left), L.jc.x, L.jc.y, L.jc.z, L.jc.w,
+                          L.load_right ? L.edge_right : from_right};
+    const int par = L.par;
     wbc_f4 val;
     float m = 0.0f;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        val[e] = o[e];
-        const int k = k0 + 2 * e;
+        val[e] = L.o[e];
+        const int k = L.k0 + 2 * e;
         if (k < 1 || k > p.nz - 2) continue;
         const float km = par ? ext[e + 1] : ext[e];      // other at kk0 + e - 1 + par
         const float kp = par ? ext[e + 2] : ext[e + 1];  // other at kk0 + e + par
-        const float nb = im[e] + ip[e] + jm[e] + jp[e] + km + kp;
-        if (kRes == 1) m = fmaxf(m, packed_residual_term(p, nb, o[e], d[e], q_finite));
-        const float kpo = p.michaelis_k + o[e];
+        const float nb = L.im[e] + L.ip[e] + L.jm[e] + L.jp[e] + km + kp;
+        if (kRes == 1) m = fmaxf(m, packed_residual_term(p, nb, L.o[e], L.d[e], q_finite));
+        const float kpo = p.michaelis_k + L.o[e];
         float gs;
         // density == 0 makes c == +-0 whenever K + o > 0 and uptake_max is finite: 6 + h*c == 6.
-        if (d[e] == 0.0f && kpo > 0.0f && q_finite) {
+        if (L.d[e] == 0.0f && kpo > 0.0f && q_finite) {
             gs = div6(nb);
         } else {
-            const float c = d[e] * p.uptake_max / kpo;
+            const float c = L.d[e] * p.uptake_max / kpo;
             gs = nb / (6.0f + p.dx2_over_D * c);
         }
-        const float v = fmaxf((1.0f - p.omega) * o[e] + p.omega * gs, 0.0f);
+        const float v = fmaxf((1.0f - p.omega) * L.o[e] + p.omega * gs, 0.0f);
         val[e] = v;
-        if (kRes == 2) m = fmaxf(m, packed_residual_term(p, nb, v, d[e], q_finite));
+        if (kRes == 2) m = fmaxf(m, packed_residual_term(p, nb, v, L.d[e], q_finite));
     }
     // The own colour's slots are read and written by this lane only, so rewriting an unchanged
     // value (face nodes, the row's padding) races with nothing. Non-temporal as in packed_update.
-    if (store) __builtin_nontemporal_store(val, (wbc_f4*)((char*)(kRes == 1 ? out : const_cast<float*>(own)) + w));
+    if (store) __builtin_nontemporal_store(val, (wbc_f4*)((char*)dst + L.w));
     return m;
+}
+
+// One lane of the quad pass of colour C: quad_load then quad_finish, writing to `out` (kRes == 1)
+// or in place. When the solve's convergence latch `done` is set the lane does nothing and returns
+// -1; the latch is read after the lane's loads are issued, so its latency overlaps theirs (0.1-0.2
+// us per 161^3 sweep).
+template <int C, int kRes>
+__device__ __forceinline__ float quad_update(const SorParams& p, const QuadGrid& g, uint32_t t, bool store,
+                                             const float* __restrict__ own, float* __restrict__ out,
+                                             const float* __restrict__ other, const float* __restrict__ dens,
+                                             const int32_t* __restrict__ box, const int32_t* __restrict__ done,
+                                             int q_finite)
+{
+    int32_t b[kDensityBoxInts];
+    quad_box(b, box);
+    const int32_t stop = done ? *done : 0;
+    QuadLoad L;
+    quad_load<C>(L, g, t, own, other, dens, b);
+    if (stop) return -1.0f;
+    return quad_finish<kRes>(L, p, store, kRes == 1 ? out : const_cast<float*>(own), q_finite);
 }
 
 // ---- host helpers ---------------------------------------------------------------------------
