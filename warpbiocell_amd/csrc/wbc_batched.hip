@@ -896,12 +896,10 @@ __global__ void __launch_bounds__(256) wide_reset_kernel(int32_t* __restrict__ d
     }
 }
 
-// Concatenated unpacked field and density -> every replica's colour-packed arrays (kUnpack:
-// packed field -> unpacked field).
-template <bool kUnpack>
-__global__ void __launch_bounds__(256) wide_pack_kernel(const WideRep* __restrict__ wr,
-                                                         const int64_t* __restrict__ node_off, int R, int64_t nodes,
-                                                         float* __restrict__ field, const float* __restrict__ density)
+// Every replica's colour-packed field -> the concatenated unpacked field.
+__global__ void __launch_bounds__(256) wide_unpack_kernel(const WideRep* __restrict__ wr,
+                                                           const int64_t* __restrict__ node_off, int R, int64_t nodes,
+                                                           float* __restrict__ field)
 {
     for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nodes; t += (int64_t)gridDim.x * 256) {
         const int r = range_of(node_off, R, t);
@@ -912,13 +910,93 @@ __global__ void __launch_bounds__(256) wide_pack_kernel(const WideRep* __restric
         const int64_t ij = local / nz;
         const int j = (int)(ij % ny), i = (int)(ij / ny);
         const int64_t e = ij * w.pf.pitch + (k >> 1);
+        field[t] = w.pf.f[(i + j + k) & 1][e];
+    }
+}
+
+// Nodes one workgroup of wide_pack_box_kernel packs: 8 per thread.
+constexpr int64_t kWidePackNodes = 8 * 256;
+
+// Concatenated unpacked field and density -> every replica's colour-packed arrays, with the
+// replicas' density boxes (PackedField::box): workgroups [pack_off[r], pack_off[r+1]) pack replica
+// r's nodes, kWidePackNodes consecutive ones each, and write the box of those whose density is not
+// +-0 to box_part (kDensityBoxInts per workgroup); wide_box_kernel merges each replica's partial
+// boxes.
+__global__ void __launch_bounds__(256) wide_pack_box_kernel(const WideRep* __restrict__ wr,
+                                                             const int32_t* __restrict__ pack_off, int R,
+                                                             const float* __restrict__ field,
+                                                             const float* __restrict__ density,
+                                                             int32_t* __restrict__ box_part)
+{
+    __shared__ int32_t part_lds[4][kDensityBoxInts];
+    const int r = range_of(pack_off, R, (int32_t)blockIdx.x);
+    const WideRep& w = wr[r];
+    const int nx = w.p.nx, ny = w.p.ny, nz = w.p.nz;
+    const int64_t n = (int64_t)nx * ny * nz;
+    const int64_t first = (int64_t)(blockIdx.x - pack_off[r]) * kWidePackNodes;
+    const int64_t last = first + kWidePackNodes < n ? first + kWidePackNodes : n;
+    int b[kDensityBoxInts] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
+    for (int64_t local = first + threadIdx.x; local < last; local += 256) {
+        const int64_t t = w.node_offset + local;
+        const int k = (int)(local % nz);
+        const int64_t ij = local / nz;
+        const int j = (int)(ij % ny), i = (int)(ij / ny);
+        const int64_t e = ij * w.pf.pitch + (k >> 1);
         const int colour = (i + j + k) & 1;
-        if (kUnpack) {
-            field[t] = w.pf.f[colour][e];
-        } else {
-            w.pf.f[colour][e] = field[t];
-            w.pf.d[colour][e] = density[t];
+        const float dv = density[t];
+        w.pf.f[colour][e] = field[t];
+        w.pf.d[colour][e] = dv;
+        if (dv != 0.0f) {  // NaN included
+            const int c[3] = {i - 1, j - 1, k >> 3};
+#pragma unroll
+            for (int a = 0; a < 3; ++a) {
+                b[2 * a] = min(b[2 * a], c[a]);
+                b[2 * a + 1] = max(b[2 * a + 1], c[a]);
+            }
         }
+    }
+#pragma unroll
+    for (int a = 0; a < kDensityBoxInts; ++a) b[a] = (a & 1) ? wave_max_i32(b[a]) : wave_min_i32(b[a]);
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int a = 0; a < kDensityBoxInts; ++a) part_lds[threadIdx.x / kWave][a] = b[a];
+    }
+    __syncthreads();
+    if (threadIdx.x < kDensityBoxInts) {
+        const int a = (int)threadIdx.x;
+        int v = part_lds[0][a];
+        for (int q = 1; q < 4; ++q) v = (a & 1) ? max(v, part_lds[q][a]) : min(v, part_lds[q][a]);
+        box_part[(int64_t)blockIdx.x * kDensityBoxInts + a] = v;
+    }
+}
+
+// Workgroup r: replica r's density box from the partial boxes of its pack workgroups.
+__global__ void __launch_bounds__(256) wide_box_kernel(const WideRep* __restrict__ wr,
+                                                        const int32_t* __restrict__ pack_off,
+                                                        const int32_t* __restrict__ box_part)
+{
+    __shared__ int32_t part_lds[4][kDensityBoxInts];
+    const int r = (int)blockIdx.x;
+    int b[kDensityBoxInts] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
+    for (int k = pack_off[r] + (int)threadIdx.x; k < pack_off[r + 1]; k += 256) {
+#pragma unroll
+        for (int a = 0; a < kDensityBoxInts; ++a) {
+            const int v = box_part[(int64_t)k * kDensityBoxInts + a];
+            b[a] = (a & 1) ? max(b[a], v) : min(b[a], v);
+        }
+    }
+#pragma unroll
+    for (int a = 0; a < kDensityBoxInts; ++a) b[a] = (a & 1) ? wave_max_i32(b[a]) : wave_min_i32(b[a]);
+    if (__lane_id() == 0) {
+#pragma unroll
+        for (int a = 0; a < kDensityBoxInts; ++a) part_lds[threadIdx.x / kWave][a] = b[a];
+    }
+    __syncthreads();
+    if (threadIdx.x < kDensityBoxInts) {
+        const int a = (int)threadIdx.x;
+        int v = part_lds[0][a];
+        for (int q = 1; q < 4; ++q) v = (a & 1) ? max(v, part_lds[q][a]) : min(v, part_lds[q][a]);
+        wr[r].pf.box[a] = v;
     }
 }
 
@@ -964,10 +1042,10 @@ static int tiles_for(int tile, int nx, int ny, int nz)
 // ---- workspace layout -----------------------------------------------------------------------
 struct BatchLayout {
     size_t rep, state, bar, node_off, total_sweeps, partial, acc, density, buf1, buf2, rep_of_slot, cnt, start,
-        scan, key, tmp, id, order_ready, xyzr, wrep, rblk_off, wdone, wpartial, packed, dflag, dscan, dscan_ws,
-        daughters, cap_err, active, total;
+        scan, key, tmp, id, order_ready, xyzr, wrep, rblk_off, wdone, wpartial, packed, wbox, wbox_part, pack_off,
+        dflag, dscan, dscan_ws, daughters, cap_err, active, total;
     int64_t slots, nodes, buckets;
-    int32_t partial_stride, active_total, wide_res_blocks;
+    int32_t partial_stride, active_total, wide_res_blocks, pack_blocks;
     bool wide_ok;  // every replica's grid fits the colour-packed passes
 };
 
@@ -1018,10 +1096,11 @@ static bool layout_for(const wbc_replica* r, int32_t R, int32_t dim, BatchLayout
         if (s > max_tiles) max_tiles = s;
     }
     tiles = max_tiles;
-    int64_t res_blocks = 0, packed_bytes = 0;
+    int64_t res_blocks = 0, packed_bytes = 0, pack_blocks = 0;
     bool wide_ok = true;
     for (int i = 0; i < R; ++i) {
         const SorParams g = grid_params(r[i]);
+        pack_blocks += ((int64_t)g.nx * g.ny * g.nz + kWidePackNodes - 1) / kWidePackNodes;
         const int64_t rb = wide_res_blocks_of(g, kWideRows[0]), rq = wide_res_blocks_of(g, kQuadRows);
         if (rb < 0 || rq < 0 || packed_lanes_per_plane(g, kWideRows[0]) < 0 ||
             packed_floats(g) >= ((int64_t)1 << 31))
@@ -1029,8 +1108,8 @@ static bool layout_for(const wbc_replica* r, int32_t R, int32_t dim, BatchLayout
         res_blocks += rb < 0 || rq < 0 ? 0 : std::max(rb, rq);
         packed_bytes += 4 * (int64_t)((packed_floats(g) * sizeof(float) + 255) / 256 * 256);
     }
-    if (res_blocks >= ((int64_t)1 << 30)) wide_ok = false;
-    if (!wide_ok) packed_bytes = res_blocks = 0;
+    if (res_blocks >= ((int64_t)1 << 30) || pack_blocks >= ((int64_t)1 << 30)) wide_ok = false;
+    if (!wide_ok) packed_bytes = res_blocks = pack_blocks = 0;
     const int64_t buckets = (int64_t)R * dim * dim * dim;
     if (slots >= ((int64_t)1 << 31) || buckets >= ((int64_t)1 << 31) - 1 || tiles >= ((int64_t)1 << 30)) return false;
     auto up = [](size_t b) { return (b + 255) / 256 * 256; };
@@ -1059,6 +1138,9 @@ static bool layout_for(const wbc_replica* r, int32_t R, int32_t dim, BatchLayout
     L.wdone = off;        off += up(sizeof(int32_t) * R);
     L.wpartial = off;     off += up(sizeof(float) * res_blocks);
     L.packed = off;       off += up((size_t)packed_bytes);
+    L.wbox = off;         off += up(sizeof(int32_t) * kDensityBoxInts * R);
+    L.wbox_part = off;    off += up(sizeof(int32_t) * kDensityBoxInts * pack_blocks);
+    L.pack_off = off;     off += up(sizeof(int32_t) * (R + 1));
     L.dflag = off;        off += up(sizeof(int32_t) * slots);
     L.dscan = off;        off += up(sizeof(int32_t) * slots);
     L.dscan_ws = off;     off += up(scan_workspace_bytes(slots));
@@ -1067,6 +1149,7 @@ static bool layout_for(const wbc_replica* r, int32_t R, int32_t dim, BatchLayout
     L.active = off;       off += up(sizeof(int32_t));
     L.total = off;
     L.wide_res_blocks = (int32_t)res_blocks;
+    L.pack_blocks = (int32_t)pack_blocks;
     L.wide_ok = wide_ok;
     L.slots = slots;
     L.nodes = nodes;
@@ -1219,8 +1302,12 @@ static hipError_t wide_solve(wbc_batch* b, hipStream_t s)
     const float* density = (const float*)(ws + L.density);
     wide_reset_kernel<<<1, 64, 0, s>>>(done, st, R);
     WBC_TRY_ERR(hipGetLastError());
-    if (L.nodes > 0) {
-        wide_pack_kernel<false><<<2048, 256, 0, s>>>(wr, node_off, R, L.nodes, b->field, density);
+    if (L.pack_blocks > 0) {
+        const int32_t* pack_off = (const int32_t*)(ws + L.pack_off);
+        int32_t* box_part = (int32_t*)(ws + L.wbox_part);
+        wide_pack_box_kernel<<<L.pack_blocks, 256, 0, s>>>(wr, pack_off, R, b->field, density, box_part);
+        WBC_TRY_ERR(hipGetLastError());
+        wide_box_kernel<<<R, 256, 0, s>>>(wr, pack_off, box_part);
         WBC_TRY_ERR(hipGetLastError());
     }
     const int32_t chunks = (int32_t)b->checkpoints.size();
@@ -1288,7 +1375,7 @@ static hipError_t wide_solve(wbc_batch* b, hipStream_t s)
         if (!any) break;
     }
     if (L.nodes > 0) {
-        wide_pack_kernel<true><<<2048, 256, 0, s>>>(wr, node_off, R, L.nodes, b->field, density);
+        wide_unpack_kernel<<<2048, 256, 0, s>>>(wr, node_off, R, L.nodes, b->field);
         WBC_TRY_ERR(hipGetLastError());
     }
     return hipSuccess;
@@ -1397,7 +1484,7 @@ static hipError_t wide_setup(wbc_batch* b, const wbc_replica* replicas, const st
     const int R = b->R;
     b->wide.assign(R, WideRep{});
     size_t off = b->L.packed;
-    std::vector<int32_t> qs;
+    std::vector<int32_t> qs, pack_off(R + 1, 0);
     for (int r = 0; r < R; ++r) {
         const wbc_replica& h = replicas[r];
         WideRep& w = b->wide[r];
@@ -1408,7 +1495,10 @@ static hipError_t wide_setup(wbc_batch* b, const wbc_replica* replicas, const st
         w.pf.d[0] = (float*)(b->ws + off + 2 * each);
         w.pf.d[1] = (float*)(b->ws + off + 3 * each);
         w.pf.pitch = packed_pitch(w.p.nz, kWideRows[0]);  // set per schedule by wide_configure
+        w.pf.box = (int32_t*)(b->ws + b->L.wbox) + kDensityBoxInts * r;
         off += 4 * each;
+        const int64_t n = (int64_t)w.p.nx * w.p.ny * w.p.nz;
+        pack_off[r + 1] = pack_off[r] + (int32_t)((n + kWidePackNodes - 1) / kWidePackNodes);
         const int64_t rl = packed_lanes_per_plane(w.p, kResidualRows);
         w.rlanes = (int32_t)rl;
         w.rbpp = (int32_t)((rl + 255) / 256);
@@ -1424,6 +1514,8 @@ static hipError_t wide_setup(wbc_batch* b, const wbc_replica* replicas, const st
     std::sort(qs.begin(), qs.end());
     qs.erase(std::unique(qs.begin(), qs.end()), qs.end());
     b->checkpoints = qs;
+    if (pack_off[R] != b->L.pack_blocks) return hipErrorInvalidValue;
+    WBC_TRY_ERR(hipMemcpy(b->ws + b->L.pack_off, pack_off.data(), sizeof(int32_t) * (R + 1), hipMemcpyHostToDevice));
     for (int e = 0; e < 8; ++e) WBC_TRY_ERR(hipEventCreateWithFlags(&b->ev[e], hipEventDisableTiming));
     WBC_TRY_ERR(hipHostMalloc((void**)&b->flags_host, sizeof(int32_t) * qs.size() * R,
                               hipHostMallocMapped | hipHostMallocCoherent));
