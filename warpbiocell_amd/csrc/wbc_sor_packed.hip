@@ -165,9 +165,7 @@ __global__ void __launch_bounds__(256) sor_quad_kernel(SorParams p, QuadGrid g, 
                                                        float* __restrict__ partial)
 {
     __shared__ float wave_max_lds[4];
-    int32_t b[kDensityBoxInts];
-    quad_box(b, box);
-    const int32_t stop = done ? *done : 0;
+    const QuadCtl ctl = quad_ctl_load(box, done);
     const uint32_t wave = (uint32_t)xcd_tile(blockIdx.x, gridDim.x) * 4u + threadIdx.x / kWave;
     const uint32_t first = wave * (uint32_t)(kWave * Q) + (uint32_t)__lane_id();
     QuadLoad L[Q];
@@ -177,9 +175,11 @@ __global__ void __launch_bounds__(256) sor_quad_kernel(SorParams p, QuadGrid g, 
         uint32_t t = first + (uint32_t)(r * kWave);
         in_range[r] = t < (uint32_t)g.total;
         if (!in_range[r]) t = (uint32_t)g.total - 1;  // a duplicate of the last lane: loads, no store
-        quad_load<C>(L[r], g, t, own, other, dens, b);
+        quad_load<C>(L[r], g, t, own, other, dens, ctl);
     }
-    if (stop) return;  // the solve has converged (the same for every lane)
+#pragma unroll
+    for (int r = 0; r < Q; ++r) loads_before_here(L[r]);
+    if (ctl.stop) return;  // the solve has converged (the same for every lane)
     float* dst = kRes == 1 ? out : own;
     float m = 0.0f;
 #pragma unroll

@@ -207,8 +207,20 @@ __device__ __forceinline__ uint32_t fast_div(uint32_t n, const FastDiv& f)
 struct QuadGrid {
     FastDiv plane, row;  // divisions by the lanes per plane and the groups of 4 per row
     uint32_t row_bytes, plane_bytes;
+    uint32_t array_bytes;  // bytes of one packed array, below kNoLoad
     int32_t total;  // lanes of one pass
 };
+
+// A buffer-load offset past every packed array: the hardware's range check turns such a load
+// into zeros without a memory access (raw buffer loads return 0 for offsets at or past the
+// descriptor's size), which lets a lane skip a load without a branch.
+constexpr uint32_t kNoLoad = 0xFFFFFF00u;
+
+// Raw buffer descriptor of `bytes` bytes at `base` (32-bit data format, unswizzled).
+__device__ __forceinline__ __amdgpu_buffer_rsrc_t packed_rsrc(const float* base, uint32_t bytes)
+{
+    return __builtin_amdgcn_make_buffer_rsrc(const_cast<float*>(base), (short)0, (int)bytes, 0x00020000);
+}
 
 __device__ __forceinline__ float lane_from_left(float v, float lane0)  // lane L: lane L-1's v
 {
@@ -222,10 +234,13 @@ __device__ __forceinline__ float lane_from_right(float v, float lane63)  // lane
 // What one lane of a quad pass of colour C loads for plane i = pl + 1, row j = row + 1 and packed
 // indices kk0 = 4 q .. 4 q + 3 (the lane geometry above): the other colour's rows j-1, j, j+1 of
 // plane i and row j of planes i-1 and i+1, its own field and density, and the wave-edge and
-// row-end k-neighbours. With a density box (see PackedField) a lane outside it takes its four
-// densities as +0 without loading them: they are +0 or -0 there, and both give the same value in
-// every expression of quad_finish (c is +-0 where K + o > 0 and uptake_max is finite, NaN
-// otherwise, either way). b: the box, or {0, INT32_MAX, ...} for none.
+// row-end k-neighbours. The six field loads are issued first, then the loads that depend on the
+// lane: the two edge values (lanes 0 and 63 and the row's last group) and, with a density box
+// (see PackedField), the density only inside the box. Those are buffer loads whose offset is
+// kNoLoad where the lane needs no value, so the wave issues them without a branch and they
+// return +0. A density read as +0 outside the box is +0 or -0 there, and both give the same
+// value in every expression of quad_finish (c is +-0 where K + o > 0 and uptake_max is finite,
+// NaN otherwise, either way).
 struct QuadLoad {
     wbc_f4 jm, jc, jp, im, ip, o, d;
     float edge_left, edge_right;
@@ -234,20 +249,50 @@ struct QuadLoad {
     bool load_right;
 };
 
-__device__ __forceinline__ void quad_box(int32_t (&b)[kDensityBoxInts], const int32_t* __restrict__ box)
+typedef uint32_t wbc_u4 __attribute__((ext_vector_type(4)));
+typedef uint32_t wbc_u2 __attribute__((ext_vector_type(2)));
+
+// The density box and the solve's convergence latch of a quad pass, read with vector buffer loads
+// before the lanes' field loads: a scalar load would make the first field load wait for it (the
+// scalar counter is waited on as a whole for the kernel arguments the addresses need), while a
+// vector load returns in order ahead of the field loads, so the density load waits only for the
+// box and the latch is checked after the field loads have gone out. A null box reads as the
+// whole grid, a null latch as 0 (a zero-size descriptor: every load is out of range).
+struct QuadCtl {
+    wbc_u4 b03;
+    wbc_u2 b45;
+    uint32_t stop;
+    bool has_box;
+};
+
+__device__ __forceinline__ QuadCtl quad_ctl_load(const int32_t* __restrict__ box, const int32_t* __restrict__ done)
 {
-#pragma unroll
-    for (int e = 0; e < kDensityBoxInts; ++e) b[e] = (e & 1) ? INT32_MAX : 0;
-    if (box) {
-#pragma unroll
-        for (int e = 0; e < kDensityBoxInts; ++e) b[e] = box[e];
-    }
+    QuadCtl c;
+    c.has_box = box != nullptr;
+    const __amdgpu_buffer_rsrc_t br =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(box), (short)0, box ? 4 * kDensityBoxInts : 0,
+                                          0x00020000);
+    c.b03 = __builtin_amdgcn_raw_buffer_load_b128(br, 0, 0, 0);
+    c.b45 = __builtin_amdgcn_raw_buffer_load_b64(br, 16, 0, 0);
+    const __amdgpu_buffer_rsrc_t dr =
+        __builtin_amdgcn_make_buffer_rsrc(const_cast<int32_t*>(done), (short)0, done ? 4 : 0, 0x00020000);
+    c.stop = __builtin_amdgcn_raw_buffer_load_b32(dr, 0, 0, 0);
+    return c;
+}
+
+// Makes the lane's loads complete here: without it the compiler sinks loads whose values are
+// used only after the convergence-latch branch below that branch, and every wave then waits for
+// the latch before it issues them.
+__device__ __forceinline__ void loads_before_here(const QuadLoad& L)
+{
+    asm volatile("" ::"v"(L.jm), "v"(L.jc), "v"(L.jp), "v"(L.im), "v"(L.ip), "v"(L.o), "v"(L.d), "v"(L.edge_left),
+                 "v"(L.edge_right));
 }
 
 template <int C>
 __device__ __forceinline__ void quad_load(QuadLoad& L, const QuadGrid& g, uint32_t t, const float* __restrict__ own,
                                           const float* __restrict__ other, const float* __restrict__ dens,
-                                          const int32_t (&b)[kDensityBoxInts])
+                                          const QuadCtl& ctl)
 {
     const uint32_t pl = fast_div(t, g.plane);
     const uint32_t u = t - pl * g.plane.d;
@@ -267,17 +312,21 @@ __device__ __forceinline__ void quad_load(QuadLoad& L, const QuadGrid& g, uint32
     L.im = at(ob, w - g.plane_bytes);
     L.ip = at(ob, w + g.plane_bytes);
     L.o = at(own, w);
-    L.d = wbc_f4{0.0f, 0.0f, 0.0f, 0.0f};
-    if ((int)pl >= b[0] && (int)pl <= b[1] && (int)row >= b[2] && (int)row <= b[3] && (int)q >= b[4] && (int)q <= b[5])
-        L.d = at(dens, w);
+    __builtin_amdgcn_sched_barrier(0);  // the six loads above go out before the ones below
     const int lane = (int)__lane_id();
     // Lane 0 has no left neighbour in the wave; lane 63 has no right one, and the row's last
     // group has no lane to its right in the row: they load the value.
     L.load_right = lane == kWave - 1 || q == g.row.d - 1;
-    L.edge_left = 0.0f;
-    L.edge_right = 0.0f;
-    if (lane == 0) L.edge_left = *(const float*)(ob + w - 4);
-    if (L.load_right) L.edge_right = *(const float*)(ob + w + 16);
+    const __amdgpu_buffer_rsrc_t orsrc = packed_rsrc(other, g.array_bytes);
+    // The buffer loads return raw bits (unsigned): reinterpret them, never convert.
+    L.edge_left = __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(orsrc, lane == 0 ? w - 4u : kNoLoad, 0, 0));
+    L.edge_right =
+        __uint_as_float(__builtin_amdgcn_raw_buffer_load_b32(orsrc, L.load_right ? w + 16u : kNoLoad, 0, 0));
+    const bool in_box = !ctl.has_box || ((int)pl >= (int)ctl.b03.x && (int)pl <= (int)ctl.b03.y &&
+                                         (int)row >= (int)ctl.b03.z && (int)row <= (int)ctl.b03.w &&
+                                         (int)q >= (int)ctl.b45.x && (int)q <= (int)ctl.b45.y);
+    L.d = __builtin_bit_cast(
+        wbc_f4, __builtin_amdgcn_raw_buffer_load_b128(packed_rsrc(dens, g.array_bytes), in_box ? w : kNoLoad, 0, 0));
 }
 
 // The update of one quad lane from its loads (kernels/field_kernels.py:185-212, same expressions
@@ -334,12 +383,11 @@ __device__ __forceinline__ float quad_update(const SorParams& p, const QuadGrid&
                                              const int32_t* __restrict__ box, const int32_t* __restrict__ done,
                                              int q_finite)
 {
-    int32_t b[kDensityBoxInts];
-    quad_box(b, box);
-    const int32_t stop = done ? *done : 0;
+    const QuadCtl ctl = quad_ctl_load(box, done);
     QuadLoad L;
-    quad_load<C>(L, g, t, own, other, dens, b);
-    if (stop) return -1.0f;
+    quad_load<C>(L, g, t, own, other, dens, ctl);
+    loads_before_here(L);
+    if (ctl.stop) return -1.0f;
     return quad_finish<kRes>(L, p, store, kRes == 1 ? out : const_cast<float*>(own), q_finite);
 }
 
@@ -359,7 +407,8 @@ inline bool quad_grid_of(const SorParams& p, QuadGrid& g)
 {
     const int64_t pitch = packed_pitch(p.nz, kQuadRows);
     const int64_t groups = (p.nz / 2 + 3) / 4, lanes = (int64_t)(p.ny - 2) * groups, total = lanes * (p.nx - 2);
-    if (packed_floats(p) * 4 >= ((int64_t)1 << 32) || total >= INT32_MAX - 256) return false;
+    if (packed_floats(p) * 4 >= (int64_t)kNoLoad || total >= INT32_MAX - 256) return false;
+    g.array_bytes = (uint32_t)(packed_floats(p) * 4);
     g.plane = fast_div_of((uint32_t)lanes);
     g.row = fast_div_of((uint32_t)groups);
     g.row_bytes = (uint32_t)(pitch * 4);
