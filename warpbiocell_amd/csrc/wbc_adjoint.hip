@@ -56,7 +56,7 @@ __global__ void __launch_bounds__(256) fixed_to_float_kernel(const long long* __
                                                              float inv_vol, float* __restrict__ out)
 {
     int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x;
-    if (t < nodes) out[t] = (float)acc[t] / kFixedOne * inv_vol;
+    if (t < nodes) out[t] = (float)load_accumulated(acc + t) / kFixedOne * inv_vol;
 }
 
 // Clamped trilinear weights exactly as sample_trilinear (kernels/field_kernels.py:162-182).
@@ -139,7 +139,7 @@ __global__ void __launch_bounds__(256) adjoint_setup_kernel(AdjointParams a, con
     const float diag = 6.0f + a.h * (a.q_ref * a.R[t]) * a.michaelis_k / (kpo * kpo);
     const float id = 1.0f / diag;
     inv_diag[t] = id;
-    rhs[t] = g[t] * id;
+    rhs[t] = load_accumulated(g + t) * id;
 }
 
 // One colour of red-black SOR on A mu = g (in place): mu <- (1-w) mu + w (nb * inv_diag + rhs).
@@ -185,7 +185,7 @@ __global__ void __launch_bounds__(256) adjoint_setup_packed_kernel(AdjointParams
                 const float kpo = a.michaelis_k + a.O[t];
                 const float diag = 6.0f + a.h * (a.q_ref * a.R[t]) * a.michaelis_k / (kpo * kpo);
                 id = 1.0f / diag;
-                r = g[t] * id;
+                r = load_accumulated(g + t) * id;
             }
         }
         (c ? inv1 : inv0)[e] = id;
@@ -419,7 +419,7 @@ __global__ void __launch_bounds__(256) global_gradient_kernel(AdjointParams a, c
             const int64_t t = i * plane + (int64_t)j * a.nz + k;
             const bool face = i == 0 || j == 0 || k == 0 || i == a.nx - 1 || j == a.ny - 1 || k == a.nz - 1;
             if (face) {
-                accB += (double)g[t];
+                accB += (double)load_accumulated(g + t);
                 continue;
             }
             if (a.fixed && a.fixed[t] != 0) continue;

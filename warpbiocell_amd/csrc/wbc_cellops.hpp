@@ -65,10 +65,10 @@ __device__ __forceinline__ float finalize_node(const long long* __restrict__ acc
                                                float inv_vol, float w0, float w1, float w2, float w3,
                                                float* __restrict__ per_state, int32_t layers = 0xF)
 {
-    float d0 = (layers & 1) ? (float)acc[t] / kFixedOne * inv_vol : 0.0f;
-    float d1 = (layers & 2) ? (float)acc[nodes + t] / kFixedOne * inv_vol : 0.0f;
-    float d2 = (layers & 4) ? (float)acc[2 * nodes + t] / kFixedOne * inv_vol : 0.0f;
-    float d3 = (layers & 8) ? (float)acc[3 * nodes + t] / kFixedOne * inv_vol : 0.0f;
+    float d0 = (layers & 1) ? (float)load_accumulated(acc + t) / kFixedOne * inv_vol : 0.0f;
+    float d1 = (layers & 2) ? (float)load_accumulated(acc + nodes + t) / kFixedOne * inv_vol : 0.0f;
+    float d2 = (layers & 4) ? (float)load_accumulated(acc + 2 * nodes + t) / kFixedOne * inv_vol : 0.0f;
+    float d3 = (layers & 8) ? (float)load_accumulated(acc + 3 * nodes + t) / kFixedOne * inv_vol : 0.0f;
     if (per_state) {
         per_state[t] = d0;
         per_state[nodes + t] = d1;

@@ -203,6 +203,20 @@ __device__ __forceinline__ int wave_lane() { return (int)__lane_id(); }
 // Value of lane 63 broadcast to every lane.
 __device__ __forceinline__ int wave_last(int v) { return __builtin_amdgcn_readlane(v, 63); }
 
+// A value that earlier kernels accumulated with device-scope atomics (the int64 deposit
+// accumulators, the adjoint's sampled-gradient field), read with an agent-scope load. Plain loads
+// of such values have intermittently returned data the atomics never produced on MI355X (a few
+// nodes of a batched deposit accumulator, at random steps; the accumulator read back afterwards
+// was correct), and agent-scope loads have not; they cost one coherent load per value.
+__device__ __forceinline__ long long load_accumulated(const long long* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+__device__ __forceinline__ float load_accumulated(const float* p)
+{
+    return __hip_atomic_load(p, __ATOMIC_RELAXED, __HIP_MEMORY_SCOPE_AGENT);
+}
+
 __device__ __forceinline__ float wave_max(float v)
 {
     for (int off = 32; off > 0; off >>= 1) v = fmaxf(v, __shfl_xor(v, off, 64));
