@@ -253,6 +253,29 @@ def test_batched_steps_equal_the_oracle_step(oracle, device):
     batch.close()
 
 
+@pytest.mark.parametrize("schedule", ["default", "wide/quad"])
+def test_repeated_batched_runs_give_the_serial_result_every_time(device, schedule):
+    """Eight fresh batched runs of the same replicas, eager and captured alternately, each equal to
+    one serial run bit for bit. The finalize of the batched deposit reads accumulators that other
+    kernels built with atomics; plain loads there once returned wrong values for a few nodes in
+    some runs and not others, which a single run does not show."""
+    from warpbiocell_amd.simulation.batched import BatchedSweep
+
+    steps = 3
+    specs = [s for s in SPECS if s[0] in ("cube21", "hypoxic33", "growing", "sweep_lifecycle")]
+    serial = [build(s, device) for s in specs]
+    run_serial(serial, steps)
+    for run in range(8):
+        print(f"repeat {run}", flush=True)
+        batch = BatchedSweep([build(s, device) for s in specs], hash_dim=64)
+        use_schedule(batch, schedule)
+        for _ in range(steps):
+            batch.step(1, use_graph=bool(run & 1))
+        torch.cuda.synchronize()
+        assert_same_state(serial, batch)
+        batch.close()
+
+
 def test_capacity_overflow_places_nothing_for_that_replica(device):
     """A replica whose daughters would not fit: none are placed for it (its count stays), the
     others grow as serial steps do, and reading the counts raises CapacityError."""
