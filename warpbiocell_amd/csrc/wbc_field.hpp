@@ -22,14 +22,23 @@ struct SorParams {
 // divisor is known in advance", IEEE Trans. Computers 53(8), 2004). Checked against IEEE
 // division for every non-negative finite binary32 x: identical for all |x| >= 2^-120 (and by
 // symmetry for negative x), so smaller, infinite and NaN inputs take the division.
-__device__ __forceinline__ float div6(float x)
+// div6_fast is that product and correction alone, exact where div6_fast_ok holds.
+__device__ __forceinline__ bool div6_fast_ok(float x)
 {
     const float ax = fabsf(x);
-    if (!(ax >= 0x1p-120f && ax <= 3.0e38f)) return x / 6.0f;
+    return ax >= 0x1p-120f && ax <= 3.0e38f;
+}
+__device__ __forceinline__ float div6_fast(float x)
+{
     const float r6 = 1.0f / 6.0f;
     const float q0 = x * r6;
     const float r = fmaf(-q0, 6.0f, x);
     return fmaf(r, r6, q0);
+}
+__device__ __forceinline__ float div6(float x)
+{
+    if (!div6_fast_ok(x)) return x / 6.0f;
+    return div6_fast(x);
 }
 
 // Workgroup -> tile with neighbouring tiles on the same XCD: workgroups are dispatched to the 8

@@ -342,29 +342,46 @@ __device__ __forceinline__ float quad_finish(const QuadLoad& L, const SorParams&
     const float ext[6] = {lane_from_left(L.jc.w, L.edge_left), L.jc.x, L.jc.y, L.jc.z, L.jc.w,
                           L.load_right ? L.edge_right : from_right};
     const int par = L.par;
+    // The four nodes first take the quotient nb / 6 by div6_fast, and the lane branches once for
+    // the nodes that need a division: density in them (6 + h*c with c == +-0 is 6 exactly only
+    // where the density is +-0, K + o > 0 and uptake_max is finite) or nb outside div6_fast's
+    // range. Nodes k < 1 or k > nz - 2 (Dirichlet faces, the row's padding) keep their value.
+    float nb[4], gs[4];
+    bool valid[4], quick[4];
+    bool slow = false;
+#pragma unroll
+    for (int e = 0; e < 4; ++e) {
+        const int k = L.k0 + 2 * e;
+        valid[e] = k >= 1 && k <= p.nz - 2;
+        const float km = par ? ext[e + 1] : ext[e];      // other at kk0 + e - 1 + par
+        const float kp = par ? ext[e + 2] : ext[e + 1];  // other at kk0 + e + par
+        nb[e] = L.im[e] + L.ip[e] + L.jm[e] + L.jp[e] + km + kp;
+        const float kpo = p.michaelis_k + L.o[e];
+        quick[e] = L.d[e] == 0.0f && kpo > 0.0f && q_finite && div6_fast_ok(nb[e]);
+        gs[e] = div6_fast(nb[e]);
+        slow |= valid[e] && !quick[e];
+    }
+    if (slow) {
+#pragma unroll
+        for (int e = 0; e < 4; ++e) {
+            if (!valid[e] || quick[e]) continue;
+            const float kpo = p.michaelis_k + L.o[e];
+            if (L.d[e] == 0.0f && kpo > 0.0f && q_finite) {
+                gs[e] = nb[e] / 6.0f;
+            } else {
+                const float c = L.d[e] * p.uptake_max / kpo;
+                gs[e] = nb[e] / (6.0f + p.dx2_over_D * c);
+            }
+        }
+    }
     wbc_f4 val;
     float m = 0.0f;
 #pragma unroll
     for (int e = 0; e < 4; ++e) {
-        val[e] = L.o[e];
-        const int k = L.k0 + 2 * e;
-        if (k < 1 || k > p.nz - 2) continue;
-        const float km = par ? ext[e + 1] : ext[e];      // other at kk0 + e - 1 + par
-        const float kp = par ? ext[e + 2] : ext[e + 1];  // other at kk0 + e + par
-        const float nb = L.im[e] + L.ip[e] + L.jm[e] + L.jp[e] + km + kp;
-        if (kRes == 1) m = fmaxf(m, packed_residual_term(p, nb, L.o[e], L.d[e], q_finite));
-        const float kpo = p.michaelis_k + L.o[e];
-        float gs;
-        // density == 0 makes c == +-0 whenever K + o > 0 and uptake_max is finite: 6 + h*c == 6.
-        if (L.d[e] == 0.0f && kpo > 0.0f && q_finite) {
-            gs = div6(nb);
-        } else {
-            const float c = L.d[e] * p.uptake_max / kpo;
-            gs = nb / (6.0f + p.dx2_over_D * c);
-        }
-        const float v = fmaxf((1.0f - p.omega) * L.o[e] + p.omega * gs, 0.0f);
-        val[e] = v;
-        if (kRes == 2) m = fmaxf(m, packed_residual_term(p, nb, v, L.d[e], q_finite));
+        const float v = fmaxf((1.0f - p.omega) * L.o[e] + p.omega * gs[e], 0.0f);
+        val[e] = valid[e] ? v : L.o[e];
+        if (kRes == 1 && valid[e]) m = fmaxf(m, packed_residual_term(p, nb[e], L.o[e], L.d[e], q_finite));
+        if (kRes == 2 && valid[e]) m = fmaxf(m, packed_residual_term(p, nb[e], v, L.d[e], q_finite));
     }
     // The own colour's slots are read and written by this lane only, so rewriting an unchanged
     // value (face nodes, the row's padding) races with nothing. Non-temporal as in packed_update.
