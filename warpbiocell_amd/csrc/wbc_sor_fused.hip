@@ -14,7 +14,7 @@
 // field (12 B per node) against 24 B per node for two strided colour passes; the halo re-reads
 // are served by L2 because consecutive workgroups of one XCD get neighbouring tiles.
 //
-// Scope: Dirichlet faces (every BASELINE configuration); grids with a Neumann face use the
+// Scope: Dirichlet faces (every benchmark workload: 41^3 to 257^3); grids with a Neumann face use the
 // colour-pass kernels, whose mirror rule needs the ring mirrored after the red pass.
 #include "wbc_sor_tile.hpp"
 
