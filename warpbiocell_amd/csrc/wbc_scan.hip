@@ -36,10 +36,18 @@ __global__ void __launch_bounds__(kScanBlock) scan_reduce_tiles(const int* __res
     __shared__ int wave_sums[kScanBlock / kWave];
     const int64_t base = (int64_t)blockIdx.x * kScanTile;
     int s = 0;
+    if (base + kScanTile <= n) {  // a whole tile (the same branch for the workgroup)
+        int v[kScanItems];
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        int64_t t = base + (int64_t)k * kScanBlock + threadIdx.x;
-        if (t < n) s += in[t];
+        for (int k = 0; k < kScanItems; ++k) v[k] = in[base + k * kScanBlock + threadIdx.x];
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) s += v[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            int64_t t = base + (int64_t)k * kScanBlock + threadIdx.x;
+            if (t < n) s += in[t];
+        }
     }
     int total;
     block_exclusive_scan(s, wave_sums, total);
@@ -71,22 +79,42 @@ __global__ void __launch_bounds__(kScanBlock) scan_tiles(const int* __restrict__
 {
     __shared__ int tile[kScanTile + kScanTile / 32];  // padded against bank conflicts
     __shared__ int wave_sums[kScanBlock / kWave];
+    const int64_t base = (int64_t)blockIdx.x * kScanTile;
+    // The tile's values are loaded before the tile's offset is worked out, so the two latencies
+    // overlap.
+    int loaded[kScanItems];
+    if (base + kScanTile <= n) {  // a whole tile (the same branch for the workgroup)
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) loaded[k] = in[base + k * kScanBlock + threadIdx.x];
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int64_t t = base + k * kScanBlock + threadIdx.x;
+            loaded[k] = t < n ? in[t] : 0;
+        }
+    }
     int offset;
     if (self_prefix) {
+        // At most kSelfPrefixTiles earlier sums: every lane issues its loads before adding any (the
+        // index clamped to the last earlier tile, whose value lanes past it do not add).
         int s = 0;
-        for (int k = threadIdx.x; k < (int)blockIdx.x; k += kScanBlock) s += partial[k];
+        const int bx = (int)blockIdx.x;
+        if (bx > 0) {
+            int v[kSelfPrefixTiles / kScanBlock];
+#pragma unroll
+            for (int q = 0; q < kSelfPrefixTiles / kScanBlock; ++q)
+                v[q] = partial[min(q * kScanBlock + (int)threadIdx.x, bx - 1)];
+#pragma unroll
+            for (int q = 0; q < kSelfPrefixTiles / kScanBlock; ++q)
+                s += q * kScanBlock + (int)threadIdx.x < bx ? v[q] : 0;
+        }
         block_exclusive_scan(s, wave_sums, offset);
     } else {
         offset = partial[blockIdx.x];
     }
-    const int64_t base = (int64_t)blockIdx.x * kScanTile;
     auto slot = [](int e) { return e + (e >> 5); };
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        int e = k * kScanBlock + threadIdx.x;
-        int64_t t = base + e;
-        tile[slot(e)] = t < n ? in[t] : 0;
-    }
+    for (int k = 0; k < kScanItems; ++k) tile[slot(k * kScanBlock + threadIdx.x)] = loaded[k];
     __syncthreads();
     int vals[kScanItems];
     int s = 0;
@@ -105,10 +133,16 @@ __global__ void __launch_bounds__(kScanBlock) scan_tiles(const int* __restrict__
     }
     __syncthreads();
 #pragma unroll
-    for (int k = 0; k < kScanItems; ++k) {
-        int e = k * kScanBlock + threadIdx.x;
-        int64_t t = base + e;
-        if (t < n) out[t] = tile[slot(e)];
+    for (int k = 0; k < kScanItems; ++k) loaded[k] = tile[slot(k * kScanBlock + threadIdx.x)];
+    if (base + kScanTile <= n) {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) out[base + k * kScanBlock + threadIdx.x] = loaded[k];
+    } else {
+#pragma unroll
+        for (int k = 0; k < kScanItems; ++k) {
+            const int64_t t = base + k * kScanBlock + threadIdx.x;
+            if (t < n) out[t] = loaded[k];
+        }
     }
 }
 
