@@ -168,15 +168,16 @@ __global__ void __launch_bounds__(256) adjoint_setup_packed_kernel(AdjointParams
                                                                    int32_t pitch, float* __restrict__ inv0,
                                                                    float* __restrict__ inv1, float* __restrict__ rhs0,
                                                                    float* __restrict__ rhs1, float* __restrict__ mu0,
-                                                                   float* __restrict__ mu1)
+                                                                   float* __restrict__ mu1, FastDiv by_pitch,
+                                                                   FastDiv by_ny)
 {
-    const int64_t per = (int64_t)a.nx * a.ny * pitch;
+    const int64_t per = (int64_t)a.nx * a.ny * pitch;  // below 2^30 (the quad lane geometry's limit)
     for (int64_t s = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; s < 2 * per; s += (int64_t)gridDim.x * blockDim.x) {
         const int c = s >= per ? 1 : 0;
-        const int64_t e = s - c * per;
-        const int64_t row = e / pitch;
-        const int kk = (int)(e - row * pitch);
-        const int i = (int)(row / a.ny), j = (int)(row - (int64_t)i * a.ny);
+        const uint32_t e = (uint32_t)(s - c * per);
+        const uint32_t row = fast_div(e, by_pitch);
+        const int kk = (int)(e - row * (uint32_t)pitch);
+        const int i = (int)fast_div(row, by_ny), j = (int)row - i * a.ny;
         const int k = 2 * kk + ((c + i + j) & 1);
         float id = 0.0f, r = 0.0f;
         if (k < a.nz) {
@@ -332,14 +333,15 @@ __global__ void __launch_bounds__(64) adjoint_clear_kernel(int32_t* __restrict__
 // mu on the grid from the packed arrays (faces and pinned nodes hold their zeros).
 __global__ void __launch_bounds__(256) adjoint_unpack_kernel(int nx, int ny, int nz, int32_t pitch,
                                                              const float* __restrict__ mu0,
-                                                             const float* __restrict__ mu1, float* __restrict__ mu)
+                                                             const float* __restrict__ mu1, float* __restrict__ mu,
+                                                             FastDiv by_nz, FastDiv by_ny)
 {
-    const int64_t nodes = (int64_t)nx * ny * nz;
+    const int64_t nodes = (int64_t)nx * ny * nz;  // below 2^31 (the quad lane geometry's limit)
     for (int64_t t = (int64_t)blockIdx.x * blockDim.x + threadIdx.x; t < nodes; t += (int64_t)gridDim.x * blockDim.x) {
-        const int64_t row = t / nz;
-        const int k = (int)(t - row * nz);
-        const int i = (int)(row / ny), j = (int)(row - (int64_t)i * ny);
-        mu[t] = (((i + j + k) & 1) ? mu1 : mu0)[row * pitch + (k >> 1)];
+        const uint32_t row = fast_div((uint32_t)t, by_nz);
+        const int k = (int)((uint32_t)t - row * (uint32_t)nz);
+        const int i = (int)fast_div(row, by_ny), j = (int)row - i * ny;
+        mu[t] = (((i + j + k) & 1) ? mu1 : mu0)[(int64_t)row * pitch + (k >> 1)];
     }
 }
 
@@ -665,7 +667,8 @@ WBC_API int wbc_adjoint_solve(float omega, int32_t max_sweeps, float tolerance, 
     float *mu0 = pk[0], *mu1 = pk[1], *inv0 = pk[2], *inv1 = pk[3], *rhs0 = pk[4], *rhs1 = pk[5];
     const int32_t qblocks = packed ? (int32_t)grid_for(qg.total, 256) : 0;
     if (packed) {
-        adjoint_setup_packed_kernel<<<2048, 256, 0, stream>>>(a, g, pitch, inv0, inv1, rhs0, rhs1, mu0, mu1);
+        adjoint_setup_packed_kernel<<<2048, 256, 0, stream>>>(a, g, pitch, inv0, inv1, rhs0, rhs1, mu0, mu1,
+                                                              fast_div_of((uint32_t)pitch), fast_div_of((uint32_t)ny));
     } else {
         adjoint_setup_kernel<<<node_grid(nx, ny, nz), blk, 0, stream>>>(a, g, inv_diag, rhs, mu);
     }
@@ -746,7 +749,8 @@ WBC_API int wbc_adjoint_solve(float omega, int32_t max_sweeps, float tolerance, 
         if (res < tolerance) break;
     }
     if (packed) {
-        adjoint_unpack_kernel<<<2048, 256, 0, stream>>>(nx, ny, nz, pitch, mu0, mu1, mu);
+        adjoint_unpack_kernel<<<2048, 256, 0, stream>>>(nx, ny, nz, pitch, mu0, mu1, mu, fast_div_of((uint32_t)nz),
+                                                        fast_div_of((uint32_t)ny));
         WBC_CHECK_LAUNCH();
     }
     if (n > 0) {

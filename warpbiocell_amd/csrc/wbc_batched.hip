@@ -702,6 +702,7 @@ struct WideRep {
     int32_t pass_blocks;  // workgroups of one colour pass of the replica under the schedule
     int32_t res_blocks;   // workgroups of its residual (both colours) under the schedule
     uint32_t magic;       // nz2_magic_of(p)
+    FastDiv by_nz, by_ny; // divisions of a node index below 2^31 by nz and ny
     int32_t q_finite;
     int32_t max_sweeps, check_every;
     float tolerance;
@@ -896,24 +897,6 @@ __global__ void __launch_bounds__(256) wide_reset_kernel(int32_t* __restrict__ d
     }
 }
 
-// Every replica's colour-packed field -> the concatenated unpacked field.
-__global__ void __launch_bounds__(256) wide_unpack_kernel(const WideRep* __restrict__ wr,
-                                                           const int64_t* __restrict__ node_off, int R, int64_t nodes,
-                                                           float* __restrict__ field)
-{
-    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nodes; t += (int64_t)gridDim.x * 256) {
-        const int r = range_of(node_off, R, t);
-        const WideRep& w = wr[r];
-        const int64_t local = t - node_off[r];
-        const int nz = w.p.nz, ny = w.p.ny;
-        const int k = (int)(local % nz);
-        const int64_t ij = local / nz;
-        const int j = (int)(ij % ny), i = (int)(ij / ny);
-        const int64_t e = ij * w.pf.pitch + (k >> 1);
-        field[t] = w.pf.f[(i + j + k) & 1][e];
-    }
-}
-
 // Nodes one workgroup of wide_pack_box_kernel packs: 8 per thread.
 constexpr int64_t kWidePackNodes = 8 * 256;
 
@@ -938,10 +921,10 @@ __global__ void __launch_bounds__(256) wide_pack_box_kernel(const WideRep* __res
     int b[kDensityBoxInts] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
     for (int64_t local = first + threadIdx.x; local < last; local += 256) {
         const int64_t t = w.node_offset + local;
-        const int k = (int)(local % nz);
-        const int64_t ij = local / nz;
-        const int j = (int)(ij % ny), i = (int)(ij / ny);
-        const int64_t e = ij * w.pf.pitch + (k >> 1);
+        const uint32_t ij = fast_div((uint32_t)local, w.by_nz);  // local is below 2^31
+        const int k = (int)((uint32_t)local - ij * (uint32_t)nz);
+        const int i = (int)fast_div(ij, w.by_ny), j = (int)ij - i * ny;
+        const int64_t e = (int64_t)ij * w.pf.pitch + (k >> 1);
         const int colour = (i + j + k) & 1;
         const float dv = density[t];
         w.pf.f[colour][e] = field[t];
@@ -967,6 +950,26 @@ __global__ void __launch_bounds__(256) wide_pack_box_kernel(const WideRep* __res
         int v = part_lds[0][a];
         for (int q = 1; q < 4; ++q) v = (a & 1) ? max(v, part_lds[q][a]) : min(v, part_lds[q][a]);
         box_part[(int64_t)blockIdx.x * kDensityBoxInts + a] = v;
+    }
+}
+
+// Every replica's colour-packed field -> the concatenated unpacked field, with the pack's
+// workgroups: [pack_off[r], pack_off[r+1]) unpack replica r's nodes, kWidePackNodes each.
+__global__ void __launch_bounds__(256) wide_unpack_kernel(const WideRep* __restrict__ wr,
+                                                           const int32_t* __restrict__ pack_off, int R,
+                                                           float* __restrict__ field)
+{
+    const int r = range_of(pack_off, R, (int32_t)blockIdx.x);
+    const WideRep& w = wr[r];
+    const int64_t n = (int64_t)w.p.nx * w.p.ny * w.p.nz;
+    const int64_t first = (int64_t)(blockIdx.x - pack_off[r]) * kWidePackNodes;
+    const int64_t last = first + kWidePackNodes < n ? first + kWidePackNodes : n;
+    for (int64_t local = first + threadIdx.x; local < last; local += 256) {
+        const uint32_t ij = fast_div((uint32_t)local, w.by_nz);  // local is below 2^31
+        const int k = (int)((uint32_t)local - ij * (uint32_t)w.p.nz);
+        const int i = (int)fast_div(ij, w.by_ny), j = (int)ij - i * w.p.ny;
+        const int64_t e = (int64_t)ij * w.pf.pitch + (k >> 1);
+        field[w.node_offset + local] = w.pf.f[(i + j + k) & 1][e];
     }
 }
 
@@ -1213,6 +1216,8 @@ static hipError_t wide_configure(wbc_batch* b, int rows)
     int64_t acc = 0, racc = 0;
     for (int r = 0; r < R; ++r) {
         WideRep& w = b->wide[r];
+        w.by_nz = fast_div_of((uint32_t)w.p.nz);
+        w.by_ny = fast_div_of((uint32_t)w.p.ny);
         if (rows == kQuadRows) {
             if (!quad_grid_of(w.p, w.qg)) return hipErrorInvalidValue;
             w.pass_blocks = (int32_t)(((int64_t)w.qg.total + 255) / 256);
@@ -1298,7 +1303,6 @@ static hipError_t wide_solve(wbc_batch* b, hipStream_t s)
     float* partial = (float*)(ws + L.wpartial);
     SolveState* st = (SolveState*)(ws + L.state);
     long long* totals = (long long*)(ws + L.total_sweeps);
-    const int64_t* node_off = (const int64_t*)(ws + L.node_off);
     const float* density = (const float*)(ws + L.density);
     wide_reset_kernel<<<1, 64, 0, s>>>(done, st, R);
     WBC_TRY_ERR(hipGetLastError());
@@ -1374,8 +1378,8 @@ static hipError_t wide_solve(wbc_batch* b, hipStream_t s)
         ++checked;
         if (!any) break;
     }
-    if (L.nodes > 0) {
-        wide_unpack_kernel<<<2048, 256, 0, s>>>(wr, node_off, R, L.nodes, b->field);
+    if (L.pack_blocks > 0) {
+        wide_unpack_kernel<<<L.pack_blocks, 256, 0, s>>>(wr, (const int32_t*)(ws + L.pack_off), R, b->field);
         WBC_TRY_ERR(hipGetLastError());
     }
     return hipSuccess;

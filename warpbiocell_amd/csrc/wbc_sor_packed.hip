@@ -31,16 +31,16 @@ __global__ void __launch_bounds__(256) pack_kernel(const float* __restrict__ fie
                                                    float* __restrict__ f0, float* __restrict__ f1,
                                                    float* __restrict__ d0, float* __restrict__ d1,
                                                    float* __restrict__ f0_copy, int32_t* __restrict__ box_part, int nx,
-                                                   int ny, int nz, int pitch)
+                                                   int ny, int nz, int pitch, FastDiv by_nz, FastDiv by_ny)
 {
     __shared__ int32_t part_lds[4][kDensityBoxInts];
-    const int64_t nodes = (int64_t)nx * ny * nz;
+    const uint32_t nodes = (uint32_t)nx * (uint32_t)ny * (uint32_t)nz;  // below 2^31 (launch_pack)
     int b[kDensityBoxInts] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
-    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nodes; t += (int64_t)gridDim.x * 256) {
-        const int k = (int)(t % nz);
-        const int64_t ij = t / nz;  // i * ny + j
-        const int j = (int)(ij % ny), i = (int)(ij / ny);
-        const int64_t e = ij * pitch + (k >> 1);
+    for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < nodes; t += gridDim.x * 256) {
+        const uint32_t ij = fast_div(t, by_nz);  // i * ny + j
+        const int k = (int)(t - ij * (uint32_t)nz);
+        const int i = (int)fast_div(ij, by_ny), j = (int)ij - i * ny;
+        const int64_t e = (int64_t)ij * pitch + (k >> 1);
         const float dv = density[t];
         if (((i + j + k) & 1) == 0) {
             f0[e] = field[t];
@@ -104,14 +104,15 @@ __global__ void __launch_bounds__(1024) box_reduce_kernel(const int32_t* __restr
 }
 
 __global__ void __launch_bounds__(256) unpack_kernel(const float* __restrict__ f0, const float* __restrict__ f1,
-                                                     float* __restrict__ field, int nx, int ny, int nz, int pitch)
+                                                     float* __restrict__ field, int nx, int ny, int nz, int pitch,
+                                                     FastDiv by_nz, FastDiv by_ny)
 {
-    const int64_t nodes = (int64_t)nx * ny * nz;
-    for (int64_t t = (int64_t)blockIdx.x * 256 + threadIdx.x; t < nodes; t += (int64_t)gridDim.x * 256) {
-        const int k = (int)(t % nz);
-        const int64_t ij = t / nz;
-        const int j = (int)(ij % ny), i = (int)(ij / ny);
-        const int64_t e = ij * pitch + (k >> 1);
+    const uint32_t nodes = (uint32_t)nx * (uint32_t)ny * (uint32_t)nz;  // below 2^31 (launch_unpack)
+    for (uint32_t t = blockIdx.x * 256 + threadIdx.x; t < nodes; t += gridDim.x * 256) {
+        const uint32_t ij = fast_div(t, by_nz);
+        const int k = (int)(t - ij * (uint32_t)nz);
+        const int i = (int)fast_div(ij, by_ny), j = (int)ij - i * ny;
+        const int64_t e = (int64_t)ij * pitch + (k >> 1);
         field[t] = ((i + j + k) & 1) == 0 ? f0[e] : f1[e];
     }
 }
@@ -221,11 +222,13 @@ hipError_t launch_pack(const SorParams& p, const float* field, const float* dens
                        float* f0_copy)
 {
     const int64_t nodes = (int64_t)p.nx * p.ny * p.nz;
+    if (nodes >= ((int64_t)1 << 31)) return hipErrorInvalidValue;  // the kernel's 32-bit node index
     if (pf.pitch < (p.nz + 1) / 2 || pf.pitch > packed_pitch(p.nz, kQuadRows)) return hipErrorInvalidValue;
     if ((pf.box == nullptr) != (pf.box_part == nullptr)) return hipErrorInvalidValue;
     const int blocks = (int)grid_for(nodes < kPackBlocks * 256 ? nodes : kPackBlocks * 256, 256);
     pack_kernel<<<blocks, 256, 0, s>>>(field, density, pf.f[0], pf.f[1], pf.d[0], pf.d[1], f0_copy, pf.box_part,
-                                       p.nx, p.ny, p.nz, pf.pitch);
+                                       p.nx, p.ny, p.nz, pf.pitch, fast_div_of((uint32_t)p.nz),
+                                       fast_div_of((uint32_t)p.ny));
     WBC_TRY_ERR(hipGetLastError());
     if (pf.box) box_reduce_kernel<<<1, 1024, 0, s>>>(pf.box_part, blocks, pf.box);
     return hipGetLastError();
@@ -234,9 +237,10 @@ hipError_t launch_pack(const SorParams& p, const float* field, const float* dens
 hipError_t launch_unpack(const SorParams& p, const PackedField& pf, float* field, hipStream_t s)
 {
     const int64_t nodes = (int64_t)p.nx * p.ny * p.nz;
+    if (nodes >= ((int64_t)1 << 31)) return hipErrorInvalidValue;  // the kernel's 32-bit node index
     if (pf.pitch < (p.nz + 1) / 2 || pf.pitch > packed_pitch(p.nz, kQuadRows)) return hipErrorInvalidValue;
-    unpack_kernel<<<grid_for(nodes < 2048 * 256 ? nodes : 2048 * 256, 256), 256, 0, s>>>(pf.f[0], pf.f[1], field,
-                                                                                         p.nx, p.ny, p.nz, pf.pitch);
+    unpack_kernel<<<grid_for(nodes < 2048 * 256 ? nodes : 2048 * 256, 256), 256, 0, s>>>(
+        pf.f[0], pf.f[1], field, p.nx, p.ny, p.nz, pf.pitch, fast_div_of((uint32_t)p.nz), fast_div_of((uint32_t)p.ny));
     return hipGetLastError();
 }
 
