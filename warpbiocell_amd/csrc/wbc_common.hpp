@@ -203,6 +203,16 @@ __device__ __forceinline__ int wave_lane() { return (int)__lane_id(); }
 // Value of lane 63 broadcast to every lane.
 __device__ __forceinline__ int wave_last(int v) { return __builtin_amdgcn_readlane(v, 63); }
 
+// Workgroup -> tile with neighbouring tiles on the same XCD: workgroups are dispatched to the 8
+// XCDs round-robin by id, so XCD x receives the contiguous tile range this mapping gives it.
+__device__ __forceinline__ int xcd_tile(int b, int nb)
+{
+    // XCDs 0 .. r-1 take q + 1 tiles each, the others q; XCD x starts at x q + min(x, r).
+    const int q = nb >> 3, r = nb & 7;
+    const int xcd = b & 7, slot = b >> 3;
+    return xcd * q + min(xcd, r) + slot;
+}
+
 // A value that earlier kernels accumulated with device-scope atomics (the int64 deposit
 // accumulators, the adjoint's sampled-gradient field), read with an agent-scope load. Plain loads
 // of such values have intermittently returned data the atomics never produced on MI355X (a few

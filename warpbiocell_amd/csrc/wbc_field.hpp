@@ -41,16 +41,6 @@ __device__ __forceinline__ float div6(float x)
     return div6_fast(x);
 }
 
-// Workgroup -> tile with neighbouring tiles on the same XCD: workgroups are dispatched to the 8
-// XCDs round-robin by id, so XCD x receives the contiguous tile range this mapping gives it.
-__device__ __forceinline__ int xcd_tile(int b, int nb)
-{
-    // XCDs 0 .. r-1 take q + 1 tiles each, the others q; XCD x starts at x q + min(x, r).
-    const int q = nb >> 3, r = nb & 7;
-    const int xcd = b & 7, slot = b >> 3;
-    return xcd * q + min(xcd, r) + slot;
-}
-
 hipError_t launch_sor_color(const SorParams& p, int32_t color, float* field, const int32_t* done, hipStream_t s);
 
 // Fused red+black sweep src -> dst on LDS tiles (Dirichlet faces only: fused_supported).

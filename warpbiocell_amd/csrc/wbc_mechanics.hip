@@ -101,7 +101,9 @@ __global__ void __launch_bounds__(256) contact_kernel(const int32_t* __restrict_
                                                        int32_t* __restrict__ next_cnt,
                                                        int32_t* __restrict__ next_tile_sums)
 {
-    int32_t t = blockIdx.x * blockDim.x + threadIdx.x;
+    // Consecutive cells in bucket order share most of their candidates: an XCD takes a contiguous
+    // range of them, so those candidates come from its own L2.
+    int32_t t = xcd_tile(blockIdx.x, gridDim.x) * blockDim.x + threadIdx.x;
     if (t >= n) return;
     const int32_t i = id[t];
     const float4 P = xyzr[t];
