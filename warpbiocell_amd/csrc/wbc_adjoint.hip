@@ -301,14 +301,19 @@ __global__ void __launch_bounds__(1024) adjoint_finish_kernel(const float* __res
 {
     __shared__ float wm[16];
     float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
-    int b = threadIdx.x;
-    for (; b + 3 * 1024 < count; b += 4 * 1024) {
-        m0 = fmaxf(m0, partial[b]);
-        m1 = fmaxf(m1, partial[b + 1024]);
-        m2 = fmaxf(m2, partial[b + 2048]);
-        m3 = fmaxf(m3, partial[b + 3072]);
+    // Four loads in flight per lane, indices past the last partial clamped to it (a repeated value
+    // leaves the maximum as it is).
+    const int last = count - 1;
+    for (int b = threadIdx.x; b < count; b += 4 * 1024) {
+        const float v0 = partial[b];
+        const float v1 = partial[min(b + 1024, last)];
+        const float v2 = partial[min(b + 2048, last)];
+        const float v3 = partial[min(b + 3072, last)];
+        m0 = fmaxf(m0, v0);
+        m1 = fmaxf(m1, v1);
+        m2 = fmaxf(m2, v2);
+        m3 = fmaxf(m3, v3);
     }
-    for (; b < count; b += 1024) m0 = fmaxf(m0, partial[b]);
     if (state[0] != 0) return;
     float m = wave_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)));
     if ((threadIdx.x & 63) == 0) wm[threadIdx.x / 64] = m;

@@ -866,9 +866,21 @@ __global__ void __launch_bounds__(256) wide_finish_kernel(const WideRep* __restr
     const WideRep& w = wr[r];
     const bool check = done[r] == 0 && wide_is_check(w, q);
     if (check) {
-        float m = 0.0f;
-        for (int b = rblk_off[r] + (int)threadIdx.x; b < rblk_off[r + 1]; b += 256) m = fmaxf(m, partial[b]);
-        m = wave_max(m);
+        // Four loads in flight per lane, indices past the replica's last partial clamped to it (a
+        // repeated value leaves the maximum as it is).
+        float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
+        const int last = rblk_off[r + 1] - 1;
+        for (int b = rblk_off[r] + (int)threadIdx.x; b <= last; b += 4 * 256) {
+            const float v0 = partial[b];
+            const float v1 = partial[min(b + 256, last)];
+            const float v2 = partial[min(b + 512, last)];
+            const float v3 = partial[min(b + 768, last)];
+            m0 = fmaxf(m0, v0);
+            m1 = fmaxf(m1, v1);
+            m2 = fmaxf(m2, v2);
+            m3 = fmaxf(m3, v3);
+        }
+        float m = wave_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)));
         if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;
         __syncthreads();
         if (threadIdx.x == 0) {

@@ -169,14 +169,19 @@ __global__ void __launch_bounds__(kFinishThreads) residual_finish_kernel(const f
     constexpr int kWaves = kFinishThreads / kWave;
     __shared__ float wave_max_lds[kWaves];
     float m0 = 0.0f, m1 = 0.0f, m2 = 0.0f, m3 = 0.0f;
-    int b = threadIdx.x;
-    for (; b + 3 * kFinishThreads < count; b += 4 * kFinishThreads) {
-        m0 = fmaxf(m0, partial[b]);
-        m1 = fmaxf(m1, partial[b + kFinishThreads]);
-        m2 = fmaxf(m2, partial[b + 2 * kFinishThreads]);
-        m3 = fmaxf(m3, partial[b + 3 * kFinishThreads]);
+    // Indices past the last partial are clamped to it: a repeated value leaves the maximum as it
+    // is, and every lane issues its four loads together.
+    const int last = count - 1;
+    for (int b = threadIdx.x; b < count; b += 4 * kFinishThreads) {
+        const float v0 = partial[b];
+        const float v1 = partial[min(b + kFinishThreads, last)];
+        const float v2 = partial[min(b + 2 * kFinishThreads, last)];
+        const float v3 = partial[min(b + 3 * kFinishThreads, last)];
+        m0 = fmaxf(m0, v0);
+        m1 = fmaxf(m1, v1);
+        m2 = fmaxf(m2, v2);
+        m3 = fmaxf(m3, v3);
     }
-    for (; b < count; b += kFinishThreads) m0 = fmaxf(m0, partial[b]);
     if (state[0] != 0) return;  // a check after convergence leaves the latched results alone
     float m = wave_max(fmaxf(fmaxf(m0, m1), fmaxf(m2, m3)));
     if ((threadIdx.x & (kWave - 1)) == 0) wave_max_lds[threadIdx.x / kWave] = m;
