@@ -996,13 +996,7 @@ __global__ void __launch_bounds__(256) wide_box_kernel(const WideRep* __restrict
     __shared__ int32_t part_lds[4][kDensityBoxInts];
     const int r = (int)blockIdx.x;
     int b[kDensityBoxInts] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
-    for (int k = pack_off[r] + (int)threadIdx.x; k < pack_off[r + 1]; k += 256) {
-#pragma unroll
-        for (int a = 0; a < kDensityBoxInts; ++a) {
-            const int v = box_part[(int64_t)k * kDensityBoxInts + a];
-            b[a] = (a & 1) ? max(b[a], v) : min(b[a], v);
-        }
-    }
+    union_box_parts(box_part, pack_off[r], pack_off[r + 1] - 1, (int)threadIdx.x, 256, b);
 #pragma unroll
     for (int a = 0; a < kDensityBoxInts; ++a) b[a] = (a & 1) ? wave_max_i32(b[a]) : wave_min_i32(b[a]);
     if (__lane_id() == 0) {

@@ -81,13 +81,7 @@ __global__ void __launch_bounds__(1024) box_reduce_kernel(const int32_t* __restr
 {
     __shared__ int32_t part_lds[16][kDensityBoxInts];
     int b[kDensityBoxInts] = {INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN, INT32_MAX, INT32_MIN};
-    for (int k = (int)threadIdx.x; k < blocks; k += 1024) {
-#pragma unroll
-        for (int a = 0; a < kDensityBoxInts; ++a) {
-            const int v = box_part[(int64_t)k * kDensityBoxInts + a];
-            b[a] = (a & 1) ? max(b[a], v) : min(b[a], v);
-        }
-    }
+    union_box_parts(box_part, 0, blocks - 1, (int)threadIdx.x, 1024, b);
 #pragma unroll
     for (int a = 0; a < kDensityBoxInts; ++a) b[a] = (a & 1) ? wave_max_i32(b[a]) : wave_min_i32(b[a]);
     if (__lane_id() == 0) {

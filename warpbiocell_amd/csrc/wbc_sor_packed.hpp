@@ -178,6 +178,28 @@ __device__ __forceinline__ void packed_lane(int tile, int blocks_per_plane, int&
 // Addresses are 32-bit byte offsets from the arrays' bases (the host requires the packed array
 // below 4 GiB).
 
+// The union, into b, of the partial density boxes first .. last (inclusive) of `part`
+// (kDensityBoxInts int32 each), the calling lane taking first + lane, + stride, ...: four
+// partials per round with their indices clamped to `last` (a repeated box leaves the union as it
+// is), so a lane's loads go out together.
+__device__ __forceinline__ void union_box_parts(const int32_t* __restrict__ part, int first, int last, int lane,
+                                                int stride, int (&b)[kDensityBoxInts])
+{
+    for (int k = first + lane; k <= last; k += 4 * stride) {
+        int v[4][kDensityBoxInts];
+#pragma unroll
+        for (int u = 0; u < 4; ++u) {
+            const int64_t at = (int64_t)min(k + u * stride, last) * kDensityBoxInts;
+#pragma unroll
+            for (int a = 0; a < kDensityBoxInts; ++a) v[u][a] = part[at + a];
+        }
+#pragma unroll
+        for (int u = 0; u < 4; ++u)
+#pragma unroll
+            for (int a = 0; a < kDensityBoxInts; ++a) b[a] = (a & 1) ? max(b[a], v[u][a]) : min(b[a], v[u][a]);
+    }
+}
+
 typedef float wbc_f4 __attribute__((ext_vector_type(4)));
 
 // Unsigned division by an invariant divisor d >= 1, exact for every 32-bit numerator: with
